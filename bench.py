@@ -1,0 +1,235 @@
+#!/usr/bin/env python3
+"""Step benchmark: EAM-alloy Cu, 4,000,000 atoms, one MI355X.
+
+Workload (the 4M-atom EAM Cu configuration): FCC 100x100x100 unit cells,
+a = 3.615 A, periodic, Gaussian position jitter 0.02 A, velocities at 600 K,
+synthetic setfl EAM (Nr = Nrho = 5000, cutoff 5.5 A), Berendsen thermostat
+tau = 0.1 ps, dt = 2 fs.  A "step" is one reference time-loop iteration:
+velocity-Verlet start (Berendsen rescale, kick, drift, exact-order re-binning),
+the two EAM force passes, and the finishing kick with the group sums.
+
+Set-up goes through the public C API of libfmd.so (box, kinds, setfl loader,
+binary atom injection, upload); the timed steps call the same C-ABI seams
+(libfmdx.so) that fmd_dync_equilibrate drives.  Inputs are resident in HBM
+before timing starts.  The CPU baseline is the oracle (the FP64 restatement
+of the reference algorithm, single thread) timed on a bounded sample of the
+same workload.
+"""
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import fmd_amd as F  # noqa: E402
+from fmd_amd import workloads as W  # noqa: E402
+
+METRIC = "atom-steps/s EAM Cu 4M atoms (%HBM roofline); batched jobs/s + p99 vs plain"
+HBM_PEAK_GBS = 8000.0
+# algorithmic HBM bytes per atom of the EAM force kernel (pass 0): read x (24) + F' (8), write F (24)
+PASS0_BYTES = {F.FMDX_PREC_FP64: 56.0, F.FMDX_PREC_MIXED: 12.0 + 8.0 + 24.0}
+# algorithmic HBM bytes per atom-step of the whole EAM step (integrate 120 + bin 8 + rho 32 + force 56)
+STEP_BYTES = {F.FMDX_PREC_FP64: 216.0, F.FMDX_PREC_MIXED: 120.0 + 8.0 + 28.0 + 12.0 + 44.0}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def fmd_lib():
+    lib = F.load_fmd()
+    P = C.c_void_p
+    lib.fmd_create.restype = P
+    lib.fmd_free.argtypes = [P]
+    lib.fmd_box_setSize.argtypes = [P, C.c_double, C.c_double, C.c_double]
+    lib.fmd_box_setPBC.argtypes = [P, C.c_bool, C.c_bool, C.c_bool]
+    lib.fmd_matt_setAtomKinds.argtypes = [P, C.c_uint, P, P]
+    lib.fmd_pot_eam_alloy_load.restype = P
+    lib.fmd_pot_eam_alloy_load.argtypes = [P, C.c_char_p]
+    lib.fmd_pot_apply.argtypes = [P, C.c_uint, C.c_uint, P]
+    lib.fmd_ext_loadAtoms.argtypes = [P, C.c_uint, P, P, P, P]
+    lib.fmd_ext_distribute.restype = C.c_bool
+    lib.fmd_ext_distribute.argtypes = [P]
+    lib.fmd_ext_getDevice.restype = P
+    lib.fmd_ext_getDevice.argtypes = [P]
+    lib.fmd_ext_setPrecision.argtypes = [P, C.c_int]
+    return lib
+
+
+def make_system(cells, seed, setfl, precision):
+    """The EAM Cu workload on the device through the public API; returns (md, dev handle, natoms)."""
+    lib = fmd_lib()
+    a = W.CU["a"]
+    rng = np.random.default_rng(seed)
+    x = W.fcc_positions(cells, cells, cells, a)
+    x = x + rng.normal(0.0, 0.02, size=x.shape)
+    L = cells * a
+    x = np.mod(x, L)
+    v = W.maxwell_velocities(len(x), 63.546, 600.0, rng)
+    md = lib.fmd_create()
+    lib.fmd_box_setSize(md, L, L, L)
+    lib.fmd_box_setPBC(md, True, True, True)
+    names = (C.c_char_p * 1)(b"Cu")
+    mass = (C.c_double * 1)(63.546)
+    lib.fmd_matt_setAtomKinds(md, 1, C.cast(names, C.c_void_p), C.cast(mass, C.c_void_p))
+    pot = lib.fmd_pot_eam_alloy_load(md, setfl.encode())
+    if not pot:
+        raise RuntimeError("setfl load failed")
+    lib.fmd_pot_apply(md, 0, 0, pot)
+    x = np.ascontiguousarray(x)
+    v = np.ascontiguousarray(v)
+    lib.fmd_ext_loadAtoms(md, len(x), x.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p), None, None)
+    lib.fmd_ext_setPrecision(md, precision)
+    if not lib.fmd_ext_distribute(md):
+        raise RuntimeError("upload failed")
+    return lib, md, lib.fmd_ext_getDevice(md), len(x)
+
+
+def run_steps(fx, dev, nsteps, dt, tau, T0):
+    for _ in range(nsteps):
+        F._ck(fx.fmdx_vv_start(dev, -1, dt, 1, tau, T0), "vv_start")
+        F._ck(fx.fmdx_update_forces(dev, -1, 0), "update_forces")
+        F._ck(fx.fmdx_vv_finish(dev, -1, dt), "vv_finish")
+
+
+def gpu_bench(cells, steps, warmup, precision, setfl, seed=1):
+    fx = F.load_fmdx()
+    lib, md, dev, n = make_system(cells, seed, setfl, precision)
+    dt, tau, T0 = 2e-3, 0.1, 600.0
+    F._ck(fx.fmdx_update_forces(dev, -1, 0), "update_forces")
+    F._ck(fx.fmdx_thermo(dev, -1), "thermo")
+    run_steps(fx, dev, warmup, dt, tau, T0)
+    th = F.Thermo()
+    F._ck(fx.fmdx_read_thermo(dev, C.byref(th)), "read_thermo")
+    F._ck(fx.fmdx_profile_enable(dev, 1), "profile")
+    F._ck(fx.fmdx_event_record(dev, 0), "event")
+    t0 = time.perf_counter()
+    run_steps(fx, dev, steps, dt, tau, T0)
+    F._ck(fx.fmdx_event_record(dev, 1), "event")
+    F._ck(fx.fmdx_sync(dev), "sync")
+    wall = time.perf_counter() - t0
+    ms = C.c_float(0)
+    F._ck(fx.fmdx_event_elapsed_ms(dev, 0, 1, C.byref(ms)), "elapsed")
+    prof = (C.c_double * 8)()
+    cnt = C.c_uint(0)
+    F._ck(fx.fmdx_profile_read(dev, prof, C.byref(cnt)), "profile_read")
+    F._ck(fx.fmdx_profile_enable(dev, 0), "profile")
+    F._ck(fx.fmdx_read_thermo(dev, C.byref(th)), "read_thermo")
+    if th.err:
+        raise RuntimeError("device error word %d" % th.err)
+    res = dict(natoms=n, steps=steps, wall_s=wall, gpu_ms=ms.value,
+               pass1_ms=prof[0] / max(cnt.value, 1), pass0_ms=prof[1] / max(cnt.value, 1),
+               T=th.temperature, pe=th.pe, cap=th.cap, max_cell=th.max_cell_count)
+    lib.fmd_free(md)
+    return res
+
+
+def cpu_baseline(setfl, cells=16, steps=3, seed=1):
+    """The oracle (FP64 restatement of the reference, one thread) on a bounded sample."""
+    from oracle.oracle import Oracle
+    a = W.CU["a"]
+    rng = np.random.default_rng(seed)
+    x = W.fcc_positions(cells, cells, cells, a) + rng.normal(0.0, 0.02, size=(4 * cells ** 3, 3))
+    L = cells * a
+    x = np.mod(x, L)
+    v = W.maxwell_velocities(len(x), 63.546, 600.0, rng)
+    o = Oracle([L] * 3, [1, 1, 1], [63.546], ["Cu"])
+    o.load_eam(setfl)
+    o.apply_eam(0, 0)
+    o.make_grid()
+    o.add_atoms(x, v)
+    o.distribute()
+    o.forces()
+    o.thermo()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.vv_start(2e-3, thermostat=1, tau=0.1, T0=600.0)
+        o.forces()
+        o.vv_finish(2e-3)
+    dt = time.perf_counter() - t0
+    return dict(value=len(x) * steps / dt, natoms=len(x), steps=steps, seconds=dt)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cells", type=int, default=100, help="FCC unit cells per edge (100 -> 4M atoms)")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    if F.device_count() < 1:
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    prec = F.FMDX_PREC_FP64 if args.precision == "fp64" else F.FMDX_PREC_MIXED
+    with tempfile.TemporaryDirectory() as td:
+        setfl = os.path.join(td, "Cu_synth.eam.alloy")
+        W.write_setfl(setfl, nr=5000, nrho=5000)
+        log("bench: EAM Cu %d^3 FCC, %s, %d warmup + %d timed steps" % (args.cells, args.precision, args.warmup,
+                                                                         args.steps))
+        r = gpu_bench(args.cells, args.steps, args.warmup, prec, setfl)
+        log("bench: %s" % r)
+        other = None
+        if not args.no_mixed:
+            p2 = F.FMDX_PREC_MIXED if prec == F.FMDX_PREC_FP64 else F.FMDX_PREC_FP64
+            other = gpu_bench(args.cells, args.steps, args.warmup, p2, setfl)
+            log("bench: other precision %s" % other)
+        cpu = None
+        if not args.no_cpu_baseline:
+            cpu = cpu_baseline(setfl)
+            log("bench: cpu baseline %s" % cpu)
+    n, K = r["natoms"], r["steps"]
+    ms_step = r["gpu_ms"] / K
+    value = n * K / (r["gpu_ms"] * 1e-3)
+    achieved = PASS0_BYTES[prec] * n / (r["pass0_ms"] * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == F.FMDX_PREC_FP64 else "f32-forces/f64-integration",
+        "data": "synthetic",
+        "config": {"workload": "EAM-alloy Cu FCC %dx%dx%d (%d atoms), a=3.615 A, PBC, jitter 0.02 A, 600 K, "
+                               "Berendsen tau=0.1 ps, dt=2 fs, synthetic setfl Nr=Nrho=5000 rc=5.5 A"
+                               % (args.cells, args.cells, args.cells, n),
+                   "natoms": n, "precision": args.precision},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_pairs64<0> (EAM force pass)" if prec == F.FMDX_PREC_FP64 else "k_pairs32<0>",
+                     "kernel_ms": r["pass0_ms"], "algorithmic_bytes_per_atom": PASS0_BYTES[prec]},
+        "step_roofline_frac": STEP_BYTES[prec] * value / (HBM_PEAK_GBS * 1e9),
+        "pass1_ms": r["pass1_ms"],
+    }
+    if other is not None:
+        out["other_precision"] = {"precision": "mixed" if prec == F.FMDX_PREC_FP64 else "fp64",
+                                  "value": other["natoms"] * other["steps"] / (other["gpu_ms"] * 1e-3),
+                                  "ms_per_step": other["gpu_ms"] / other["steps"],
+                                  "pass0_ms": other["pass0_ms"], "pass1_ms": other["pass1_ms"]}
+    if cpu is not None:
+        out["cpu_baseline"] = {"value": cpu["value"], "unit": "atom-steps/s", "cores": 1, "kind": "port",
+                               "sample": "oracle (FP64 reference restatement) on EAM Cu %d atoms (16^3 FCC), "
+                                         "%d Berendsen VV steps" % (cpu["natoms"], cpu["steps"])}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

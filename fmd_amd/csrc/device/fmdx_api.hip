@@ -1,0 +1,785 @@
+/*
+ * fmdx_api.hip -- the C-ABI entry points of libfmdx.so (declared in fmdx.h).
+ *
+ * Owns the device memory of one system, converts the reference's spline
+ * tables into per-interval coefficients, and enqueues the step kernels on
+ * the system's stream.  Nothing here computes physics on the host.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <vector>
+#include "fmdx_internal.h"
+
+#define NPROF 64   /* force calls whose kernels are timed in profile mode */
+
+struct fmdx_system
+{
+    int dev;
+    hipStream_t st;
+    int precision;
+    KDesc D;
+    unsigned natoms_uploaded;
+    int uploaded;
+    double mass[MAXK];
+    PairParam hp[MAXK * MAXK];
+    PairParam *dpairs;
+    /* EAM */
+    double4 *Fc, *rhoc, *phic;
+    float4 *Fc32, *rhoc32, *phic32;
+    /* TTM */
+    int ttm_on;
+    TtmState T;
+    void *ttm_mem[16];
+    int nttm_mem;
+    /* timing */
+    hipEvent_t ev[16];
+    int nev;
+    int profile;
+    hipEvent_t pev[NPROF * 3];
+    unsigned prof_n;
+};
+
+#define CK(call)                                   \
+    do                                             \
+    {                                              \
+        hipError_t e_ = (call);                    \
+        if (e_ != hipSuccess) return FMDX_EHIP;    \
+    } while (0)
+
+template <class T>
+static int dalloc(T **p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    if (hipMalloc((void **)p, n * sizeof(T)) != hipSuccess) return FMDX_ENOMEM;
+    return FMDX_OK;
+}
+
+template <class T>
+static void dfree(T *&p)
+{
+    if (p) hipFree((void *)p);
+    p = nullptr;
+}
+
+extern "C" const char *fmdx_strerror(int code)
+{
+    switch (code)
+    {
+        case FMDX_OK: return "ok";
+        case FMDX_EHIP: return "HIP runtime call failed";
+        case FMDX_EARG: return "invalid argument";
+        case FMDX_ENOMEM: return "device allocation failed";
+        case FMDX_ESTATE: return "call out of order";
+        case FMDX_ENODEV: return "no GPU visible";
+    }
+    return "unknown";
+}
+
+extern "C" int fmdx_device_count(int *n)
+{
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *n = c;
+    return FMDX_OK;
+}
+
+/* per-slot arrays of a layout with capacity cap */
+static void free_slots(KDesc &D)
+{
+    dfree(D.x); dfree(D.y); dfree(D.z); dfree(D.vx); dfree(D.vy); dfree(D.vz);
+    dfree(D.fx); dfree(D.fy); dfree(D.fz); dfree(D.aux);
+    dfree(D.xr); dfree(D.yr); dfree(D.zr);
+    dfree(D.grp); dfree(D.aid); dfree(D.kind);
+    dfree(D.mv); dfree(D.code); dfree(D.bkey); dfree(D.bid);
+    dfree(D.mvx); dfree(D.mvy); dfree(D.mvz); dfree(D.mvvx); dfree(D.mvvy); dfree(D.mvvz);
+    dfree(D.mvgrp); dfree(D.mvaid); dfree(D.mvkind);
+    dfree(D.part);
+}
+
+static int alloc_slots(KDesc &D, int cap, unsigned nmov)
+{
+    size_t ns = (size_t)D.ncell * cap;
+    D.cap = cap;
+    int r = 0;
+    r |= dalloc(&D.x, ns); r |= dalloc(&D.y, ns); r |= dalloc(&D.z, ns);
+    r |= dalloc(&D.vx, ns); r |= dalloc(&D.vy, ns); r |= dalloc(&D.vz, ns);
+    r |= dalloc(&D.fx, ns); r |= dalloc(&D.fy, ns); r |= dalloc(&D.fz, ns);
+    r |= dalloc(&D.aux, ns);
+    r |= dalloc(&D.xr, ns); r |= dalloc(&D.yr, ns); r |= dalloc(&D.zr, ns);
+    r |= dalloc(&D.grp, ns); r |= dalloc(&D.aid, ns); r |= dalloc(&D.kind, ns);
+    r |= dalloc(&D.mv, ns); r |= dalloc(&D.code, ns);
+    r |= dalloc(&D.bkey, ns); r |= dalloc(&D.bid, ns);
+    r |= dalloc(&D.mvx, nmov); r |= dalloc(&D.mvy, nmov); r |= dalloc(&D.mvz, nmov);
+    r |= dalloc(&D.mvvx, nmov); r |= dalloc(&D.mvvy, nmov); r |= dalloc(&D.mvvz, nmov);
+    r |= dalloc(&D.mvgrp, nmov); r |= dalloc(&D.mvaid, nmov); r |= dalloc(&D.mvkind, nmov);
+    size_t nblk = (ns + 255) / 256 + 1;
+    r |= dalloc(&D.part, nblk * 8);
+    if (r) { free_slots(D); return FMDX_ENOMEM; }
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_system_create(fmdx_system_t **out, const double l[3], const int pbc[3], const int nc[3],
+                                  const double cellh[3], int nkinds, const double *mass, int precision)
+{
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return FMDX_ENODEV;
+    if (nkinds < 1 || nkinds > MAXK) return FMDX_EARG;
+    for (int d = 0; d < 3; d++)
+        if (nc[d] < 1 || !(cellh[d] > 0) || !(l[d] > 0)) return FMDX_EARG;
+    long ncell = (long)nc[0] * nc[1] * nc[2];
+    if (ncell > (1L << 30)) return FMDX_EARG;
+
+    fmdx_system_t *s = (fmdx_system_t *)calloc(1, sizeof(fmdx_system_t));
+    if (!s) return FMDX_ENOMEM;
+    CK(hipGetDevice(&s->dev));
+    if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) { free(s); return FMDX_EHIP; }
+    s->precision = precision;
+    KDesc &D = s->D;
+    D.nc0 = nc[0]; D.nc1 = nc[1]; D.nc2 = nc[2];
+    D.ncell = (int)ncell;
+    D.pbc0 = pbc[0]; D.pbc1 = pbc[1]; D.pbc2 = pbc[2];
+    D.L0 = l[0]; D.L1 = l[1]; D.L2 = l[2];
+    D.h0 = cellh[0]; D.h1 = cellh[1]; D.h2 = cellh[2];
+    D.nkinds = nkinds;
+    D.single_cat = -1;
+    for (int k = 0; k < MAXK; k++)
+    {
+        D.mass[k] = k < nkinds ? mass[k] : 1.0;
+        s->mass[k] = D.mass[k];
+        D.elem[k] = -1;
+    }
+    int r = 0;
+    r |= dalloc(&D.cnt, ncell);
+    r |= dalloc(&D.nleave, ncell); r |= dalloc(&D.narr, ncell);
+    r |= dalloc(&D.nearly, ncell); r |= dalloc(&D.fill, ncell);
+    r |= dalloc(&D.counters, 8);
+    r |= dalloc(&D.cell_pe, ncell); r |= dalloc(&D.cell_emb, ncell);
+    r |= dalloc(&D.thermo, 16);
+    r |= dalloc(&s->dpairs, MAXK * MAXK);
+    if (r) { fmdx_system_destroy(s); return FMDX_ENOMEM; }
+    hipMemsetAsync(D.cnt, 0, sizeof(int) * ncell, s->st);
+    hipMemsetAsync(D.nleave, 0, sizeof(int) * ncell, s->st);
+    hipMemsetAsync(D.narr, 0, sizeof(int) * ncell, s->st);
+    hipMemsetAsync(D.nearly, 0, sizeof(int) * ncell, s->st);
+    hipMemsetAsync(D.fill, 0, sizeof(int) * ncell, s->st);
+    hipMemsetAsync(D.counters, 0, sizeof(unsigned) * 8, s->st);
+    hipMemsetAsync(D.cell_pe, 0, sizeof(double) * ncell, s->st);
+    hipMemsetAsync(D.cell_emb, 0, sizeof(double) * ncell, s->st);
+    hipMemsetAsync(D.thermo, 0, sizeof(double) * 16, s->st);
+    memset(s->hp, 0, sizeof s->hp);
+    D.pairs = s->dpairs;
+    for (int i = 0; i < 16; i++) hipEventCreate(&s->ev[i]);
+    s->nev = 16;
+    for (int i = 0; i < NPROF * 3; i++) hipEventCreate(&s->pev[i]);
+    if (hipStreamSynchronize(s->st) != hipSuccess) { fmdx_system_destroy(s); return FMDX_EHIP; }
+    *out = s;
+    return FMDX_OK;
+}
+
+extern "C" void fmdx_system_destroy(fmdx_system_t *s)
+{
+    if (!s) return;
+    if (s->st) hipStreamSynchronize(s->st);
+    KDesc &D = s->D;
+    free_slots(D);
+    dfree(D.cnt); dfree(D.nleave); dfree(D.narr); dfree(D.nearly); dfree(D.fill);
+    dfree(D.counters); dfree(D.cell_pe); dfree(D.cell_emb); dfree(D.thermo);
+    dfree(s->dpairs);
+    dfree(s->Fc); dfree(s->rhoc); dfree(s->phic);
+    dfree(s->Fc32); dfree(s->rhoc32); dfree(s->phic32);
+    for (int i = 0; i < s->nttm_mem; i++) hipFree(s->ttm_mem[i]);
+    for (int i = 0; i < s->nev; i++) hipEventDestroy(s->ev[i]);
+    for (int i = 0; i < NPROF * 3; i++)
+        if (s->pev[i]) hipEventDestroy(s->pev[i]);
+    if (s->st) hipStreamDestroy(s->st);
+    free(s);
+}
+
+extern "C" int fmdx_set_precision(fmdx_system_t *s, int precision)
+{
+    if (!s || (precision != FMDX_PREC_FP64 && precision != FMDX_PREC_MIXED)) return FMDX_EARG;
+    if (precision == FMDX_PREC_MIXED && s->D.has_eam && !s->rhoc32) return FMDX_ESTATE;
+    s->precision = precision;
+    return FMDX_OK;
+}
+
+static void refresh_pair_summary(fmdx_system_t *s)
+{
+    KDesc &D = s->D;
+    int nk = D.nkinds;
+    double rcmax2 = 0, rceam2 = 0;
+    int cat = -2, has_eam = 0;
+    for (int i = 0; i < nk; i++)
+        for (int j = 0; j < nk; j++)
+        {
+            const PairParam &p = s->hp[i * nk + j];
+            if (p.rc2 > rcmax2) rcmax2 = p.rc2;
+            if (p.cat == FMDX_POT_EAM)
+            {
+                has_eam = 1;
+                if (p.rc2 > rceam2) rceam2 = p.rc2;
+            }
+            if (cat == -2) cat = p.cat;
+            else if (cat != p.cat) cat = -1;
+        }
+    D.rcmax2 = rcmax2;
+    D.rceam2 = rceam2;
+    D.has_eam = has_eam;
+    D.single_cat = cat;
+}
+
+extern "C" int fmdx_set_pair(fmdx_system_t *s, int k1, int k2, int cat, double rc2, double p0, double p1, double p2)
+{
+    if (!s) return FMDX_EARG;
+    int nk = s->D.nkinds;
+    if (k1 < 0 || k2 < 0 || k1 >= nk || k2 >= nk) return FMDX_EARG;
+    if (cat < FMDX_POT_NONE || cat > FMDX_POT_MORSE) return FMDX_EARG;
+    PairParam p;
+    memset(&p, 0, sizeof p);
+    p.cat = cat;
+    p.rc2 = rc2;
+    p.p0 = p0; p.p1 = p1; p.p2 = p2;
+    if (cat == FMDX_POT_EAM)
+    {
+        p.iloc = (int)p0;
+        p.jloc = (int)p1;
+    }
+    s->hp[k1 * nk + k2] = p;
+    if (cat == FMDX_POT_EAM) { int t = p.iloc; p.iloc = p.jloc; p.jloc = t; }
+    s->hp[k2 * nk + k1] = p;
+    refresh_pair_summary(s);
+    CK(hipMemcpyAsync(s->dpairs, s->hp, sizeof(PairParam) * nk * nk, hipMemcpyHostToDevice, s->st));
+    CK(hipStreamSynchronize(s->st));
+    return FMDX_OK;
+}
+
+/* reference spline (y, yDD) on knots x_k = k*h -> per-interval coefficients in b = x/h - k */
+static void spline_coeffs(const double *y, const double *ydd, int n, double h, double4 *out)
+{
+    const double h26 = h * h / 6.0;
+    for (int k = 0; k < n - 1; k++)
+    {
+        double ylo = y[k], yhi = y[k + 1], dlo = ydd[k], dhi = ydd[k + 1];
+        double4 q;
+        q.x = ylo;
+        q.y = (yhi - ylo) - h26 * (2.0 * dlo + dhi);
+        q.z = 3.0 * h26 * dlo;
+        q.w = h26 * (dhi - dlo);
+        out[k] = q;
+    }
+}
+
+extern "C" int fmdx_set_eam(fmdx_system_t *s, int nelem, int nrho, double drho, int nr2, double dr2, double rc2,
+                            const double *F, const double *FDD, const double *rho, const double *rhoDD,
+                            const double *phi, const double *phiDD, const int *elem_of_kind)
+{
+    if (!s || nelem < 1 || nelem > MAXE || nrho < 3 || nr2 < 3 || !(drho > 0) || !(dr2 > 0)) return FMDX_EARG;
+    const long mr = nrho - 1, m2 = nr2 - 1;
+    std::vector<double4> hF((size_t)nelem * mr), hR((size_t)nelem * m2), hP((size_t)nelem * nelem * m2);
+    for (int e = 0; e < nelem; e++)
+    {
+        spline_coeffs(F + (size_t)e * nrho, FDD + (size_t)e * nrho, nrho, drho, &hF[e * mr]);
+        spline_coeffs(rho + (size_t)e * nr2, rhoDD + (size_t)e * nr2, nr2, dr2, &hR[e * m2]);
+        for (int f = 0; f < nelem; f++)
+            spline_coeffs(phi + ((size_t)e * nelem + f) * nr2, phiDD + ((size_t)e * nelem + f) * nr2, nr2, dr2,
+                          &hP[(e * nelem + f) * m2]);
+    }
+    std::vector<float4> fF(hF.size()), fR(hR.size()), fP(hP.size());
+    auto cvt = [](const std::vector<double4> &a, std::vector<float4> &b) {
+        for (size_t i = 0; i < a.size(); i++)
+            b[i] = make_float4((float)a[i].x, (float)a[i].y, (float)a[i].z, (float)a[i].w);
+    };
+    cvt(hF, fF); cvt(hR, fR); cvt(hP, fP);
+    dfree(s->Fc); dfree(s->rhoc); dfree(s->phic); dfree(s->Fc32); dfree(s->rhoc32); dfree(s->phic32);
+    int r = 0;
+    r |= dalloc(&s->Fc, hF.size()); r |= dalloc(&s->rhoc, hR.size()); r |= dalloc(&s->phic, hP.size());
+    r |= dalloc(&s->Fc32, fF.size()); r |= dalloc(&s->rhoc32, fR.size()); r |= dalloc(&s->phic32, fP.size());
+    if (r) return FMDX_ENOMEM;
+    CK(hipMemcpy(s->Fc, hF.data(), hF.size() * sizeof(double4), hipMemcpyHostToDevice));
+    CK(hipMemcpy(s->rhoc, hR.data(), hR.size() * sizeof(double4), hipMemcpyHostToDevice));
+    CK(hipMemcpy(s->phic, hP.data(), hP.size() * sizeof(double4), hipMemcpyHostToDevice));
+    CK(hipMemcpy(s->Fc32, fF.data(), fF.size() * sizeof(float4), hipMemcpyHostToDevice));
+    CK(hipMemcpy(s->rhoc32, fR.data(), fR.size() * sizeof(float4), hipMemcpyHostToDevice));
+    CK(hipMemcpy(s->phic32, fP.data(), fP.size() * sizeof(float4), hipMemcpyHostToDevice));
+    EamDev &E = s->D.eam;
+    E.ne = nelem; E.nrho = nrho; E.nr2 = nr2;
+    E.drho = drho; E.inv_drho = 1.0 / drho; E.dr2 = dr2; E.inv_dr2 = 1.0 / dr2; E.rc2 = rc2;
+    E.Fc = s->Fc; E.rhoc = s->rhoc; E.phic = s->phic;
+    E.Fc32 = s->Fc32; E.rhoc32 = s->rhoc32; E.phic32 = s->phic32;
+    for (int k = 0; k < s->D.nkinds; k++) s->D.elem[k] = elem_of_kind ? elem_of_kind[k] : -1;
+    return FMDX_OK;
+}
+
+static int choose_cap(unsigned maxc)
+{
+    unsigned c = maxc + maxc / 2 + 8;
+    if (c < 16) c = 16;
+    c = (c + 7) & ~7u;
+    if (c > 256) c = 256;
+    return (int)c;
+}
+
+extern "C" int fmdx_upload(fmdx_system_t *s, unsigned natoms, const unsigned *cell_count, const double *x,
+                           const double *v, const int *group, const unsigned *atomid, const unsigned *kind)
+{
+    if (!s || !cell_count) return FMDX_EARG;
+    KDesc &D = s->D;
+    std::vector<int> start(D.ncell + 1);
+    unsigned maxc = 0;
+    long acc = 0;
+    for (int c = 0; c < D.ncell; c++)
+    {
+        start[c] = (int)acc;
+        acc += cell_count[c];
+        if (cell_count[c] > maxc) maxc = cell_count[c];
+    }
+    start[D.ncell] = (int)acc;
+    if ((unsigned long)acc != natoms) return FMDX_EARG;
+    if (maxc > 256) return FMDX_EARG;
+    std::vector<uint8_t> k8(natoms ? natoms : 1);
+    for (unsigned i = 0; i < natoms; i++)
+    {
+        unsigned k = kind ? kind[i] : 0;
+        if ((int)k >= D.nkinds) return FMDX_EARG;
+        k8[i] = (uint8_t)k;
+    }
+    int cap = choose_cap(maxc);
+    hipStreamSynchronize(s->st);
+    free_slots(D);
+    int r = alloc_slots(D, cap, natoms + 64);
+    if (r) return r;
+    int *dstart = nullptr;
+    double *dx = nullptr, *dv = nullptr;
+    int *dg = nullptr;
+    unsigned *da = nullptr;
+    uint8_t *dk = nullptr;
+    r = 0;
+    r |= dalloc(&dstart, D.ncell + 1);
+    r |= dalloc(&dx, 3 * (size_t)natoms); r |= dalloc(&dv, 3 * (size_t)natoms);
+    r |= dalloc(&dg, natoms); r |= dalloc(&da, natoms); r |= dalloc(&dk, natoms);
+    if (r) return FMDX_ENOMEM;
+    std::vector<int> g(natoms ? natoms : 1, 0);
+    std::vector<unsigned> aid(natoms ? natoms : 1, 0);
+    std::vector<double> vz(3 * (size_t)(natoms ? natoms : 1), 0.0);
+    for (unsigned i = 0; i < natoms; i++)
+    {
+        if (group) g[i] = group[i];
+        aid[i] = atomid ? atomid[i] : i;
+    }
+    CK(hipMemcpy(dstart, start.data(), sizeof(int) * (D.ncell + 1), hipMemcpyHostToDevice));
+    CK(hipMemcpy(D.cnt, cell_count, sizeof(int) * D.ncell, hipMemcpyHostToDevice));
+    if (natoms)
+    {
+        CK(hipMemcpy(dx, x, sizeof(double) * 3 * natoms, hipMemcpyHostToDevice));
+        CK(hipMemcpy(dv, v ? v : vz.data(), sizeof(double) * 3 * natoms, hipMemcpyHostToDevice));
+        CK(hipMemcpy(dg, g.data(), sizeof(int) * natoms, hipMemcpyHostToDevice));
+        CK(hipMemcpy(da, aid.data(), sizeof(unsigned) * natoms, hipMemcpyHostToDevice));
+        CK(hipMemcpy(dk, k8.data(), natoms, hipMemcpyHostToDevice));
+    }
+    CK(launch_scatter(D, s->st, dstart, dx, dv, dg, da, dk));
+    unsigned ctr[8] = {0, 0, maxc, 0, 0, 0, 0, 0};
+    CK(hipMemcpyAsync(D.counters, ctr, sizeof ctr, hipMemcpyHostToDevice, s->st));
+    CK(hipStreamSynchronize(s->st));
+    dfree(dstart); dfree(dx); dfree(dv); dfree(dg); dfree(da); dfree(dk);
+    s->natoms_uploaded = natoms;
+    s->uploaded = 1;
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_natoms(fmdx_system_t *s, unsigned *n)
+{
+    if (!s || !n) return FMDX_EARG;
+    unsigned ctr[8];
+    CK(hipMemcpyAsync(ctr, s->D.counters, sizeof ctr, hipMemcpyDeviceToHost, s->st));
+    CK(hipStreamSynchronize(s->st));
+    *n = s->natoms_uploaded - ctr[1];
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_ncells(fmdx_system_t *s, unsigned *n)
+{
+    if (!s || !n) return FMDX_EARG;
+    *n = (unsigned)s->D.ncell;
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_download(fmdx_system_t *s, unsigned *natoms, unsigned *cell_count, double *x, double *v,
+                             double *f, double *aux, int *group, unsigned *atomid, unsigned *kind)
+{
+    if (!s || !s->uploaded) return FMDX_ESTATE;
+    KDesc &D = s->D;
+    std::vector<unsigned> cnt(D.ncell);
+    CK(hipMemcpyAsync(cnt.data(), D.cnt, sizeof(int) * D.ncell, hipMemcpyDeviceToHost, s->st));
+    CK(hipStreamSynchronize(s->st));
+    std::vector<int> start(D.ncell + 1);
+    long acc = 0;
+    for (int c = 0; c < D.ncell; c++) { start[c] = (int)acc; acc += cnt[c]; }
+    start[D.ncell] = (int)acc;
+    unsigned n = (unsigned)acc;
+    if (natoms) *natoms = n;
+    if (cell_count) memcpy(cell_count, cnt.data(), sizeof(unsigned) * D.ncell);
+    if (!x && !v && !f && !aux && !group && !atomid && !kind) return FMDX_OK;
+    int *dstart = nullptr;
+    double *dx = nullptr, *dv = nullptr, *df = nullptr, *daux = nullptr;
+    int *dg = nullptr;
+    unsigned *da = nullptr;
+    uint8_t *dk = nullptr;
+    int r = 0;
+    r |= dalloc(&dstart, D.ncell + 1);
+    r |= dalloc(&dx, 3 * (size_t)n); r |= dalloc(&dv, 3 * (size_t)n); r |= dalloc(&df, 3 * (size_t)n);
+    r |= dalloc(&daux, n); r |= dalloc(&dg, n); r |= dalloc(&da, n); r |= dalloc(&dk, n);
+    if (r) return FMDX_ENOMEM;
+    CK(hipMemcpyAsync(dstart, start.data(), sizeof(int) * (D.ncell + 1), hipMemcpyHostToDevice, s->st));
+    CK(launch_gather(D, s->st, dstart, dx, dv, df, daux, dg, da, dk));
+    if (n)
+    {
+        if (x) CK(hipMemcpyAsync(x, dx, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s->st));
+        if (v) CK(hipMemcpyAsync(v, dv, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s->st));
+        if (f) CK(hipMemcpyAsync(f, df, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s->st));
+        if (aux) CK(hipMemcpyAsync(aux, daux, sizeof(double) * n, hipMemcpyDeviceToHost, s->st));
+        if (group) CK(hipMemcpyAsync(group, dg, sizeof(int) * n, hipMemcpyDeviceToHost, s->st));
+        if (atomid) CK(hipMemcpyAsync(atomid, da, sizeof(unsigned) * n, hipMemcpyDeviceToHost, s->st));
+    }
+    std::vector<uint8_t> k8(n ? n : 1);
+    if (kind && n) CK(hipMemcpyAsync(k8.data(), dk, n, hipMemcpyDeviceToHost, s->st));
+    CK(hipStreamSynchronize(s->st));
+    if (kind)
+        for (unsigned i = 0; i < n; i++) kind[i] = k8[i];
+    dfree(dstart); dfree(dx); dfree(dv); dfree(df); dfree(daux); dfree(dg); dfree(da); dfree(dk);
+    return FMDX_OK;
+}
+
+static TtmDev ttm_dev(fmdx_system_t *s, int on)
+{
+    TtmDev T;
+    memset(&T, 0, sizeof T);
+    if (on && s->ttm_on)
+    {
+        T.on = 1;
+        T.ncz = s->T.ncz;
+        T.tcellh_z = s->T.tcellh_z;
+        T.xi = s->T.xi;
+        T.vcm = s->T.vcm;
+    }
+    return T;
+}
+
+extern "C" int fmdx_update_forces(fmdx_system_t *s, int active_group, int ttm_friction)
+{
+    if (!s || !s->uploaded) return FMDX_ESTATE;
+    if (s->D.has_eam && !s->rhoc) return FMDX_ESTATE;
+    const TtmDev T = ttm_dev(s, ttm_friction);
+    if (s->profile && s->prof_n < NPROF)
+    {
+        hipEvent_t *e = &s->pev[3 * s->prof_n];
+        CK(hipEventRecord(e[0], s->st));
+        CK(launch_pass1(s->D, s->st, active_group, s->precision, T));
+        CK(hipEventRecord(e[1], s->st));
+        CK(launch_pass0(s->D, s->st, active_group, s->precision, T));
+        CK(hipEventRecord(e[2], s->st));
+        CK(launch_reduce_pe(s->D, s->st));
+        s->prof_n++;
+    }
+    else
+        CK(launch_forces(s->D, s->st, active_group, s->precision, T));
+    return FMDX_OK;
+}
+
+static void half_dt_over_m(const fmdx_system_t *s, double dt, double *h)
+{
+    /* md->timestep * 0.5 / mass, integrators.c:67 */
+    for (int k = 0; k < MAXK; k++) h[k] = dt * 0.5 / s->mass[k];
+}
+
+extern "C" int fmdx_vv_start(fmdx_system_t *s, int active_group, double dt, int thermostat, double tau, double T0)
+{
+    if (!s || !s->uploaded) return FMDX_ESTATE;
+    double h[MAXK];
+    half_dt_over_m(s, dt, h);
+    CK(launch_vv_start(s->D, s->st, active_group, dt, h, thermostat, tau, T0));
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_vv_finish(fmdx_system_t *s, int active_group, double dt)
+{
+    if (!s || !s->uploaded) return FMDX_ESTATE;
+    double h[MAXK];
+    half_dt_over_m(s, dt, h);
+    CK(launch_vv_finish(s->D, s->st, active_group, h));
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_se_step(fmdx_system_t *s, int active_group, double dt)
+{
+    if (!s || !s->uploaded) return FMDX_ESTATE;
+    CK(launch_se_step(s->D, s->st, active_group, dt));
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_refresh(fmdx_system_t *s)
+{
+    if (!s || !s->uploaded) return FMDX_ESTATE;
+    CK(launch_flag_only(s->D, s->st));
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_thermo(fmdx_system_t *s, int active_group)
+{
+    if (!s || !s->uploaded) return FMDX_ESTATE;
+    CK(launch_thermo(s->D, s->st, active_group));
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_add_velocity(fmdx_system_t *s, int group, double vx, double vy, double vz)
+{
+    if (!s || !s->uploaded) return FMDX_ESTATE;
+    CK(launch_add_velocity(s->D, s->st, group, vx, vy, vz));
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_read_thermo(fmdx_system_t *s, fmdx_thermo_t *out)
+{
+    if (!s || !out) return FMDX_EARG;
+    double th[16];
+    unsigned ctr[8];
+    CK(hipMemcpyAsync(th, s->D.thermo, sizeof th, hipMemcpyDeviceToHost, s->st));
+    CK(hipMemcpyAsync(ctr, s->D.counters, sizeof ctr, hipMemcpyDeviceToHost, s->st));
+    CK(hipStreamSynchronize(s->st));
+    out->pe = th[TH_PE];
+    out->embed = th[TH_EMB];
+    out->mv2 = th[TH_MV2];
+    out->ke = th[TH_KE];
+    out->temperature = th[TH_T];
+    out->momentum[0] = th[TH_PX];
+    out->momentum[1] = th[TH_PY];
+    out->momentum[2] = th[TH_PZ];
+    out->n_active = th[TH_N];
+    out->natoms = s->natoms_uploaded - ctr[1];
+    out->max_cell_count = ctr[2];
+    out->err = ctr[3];
+    out->cap = (unsigned)s->D.cap;
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_clear_errors(fmdx_system_t *s)
+{
+    if (!s) return FMDX_EARG;
+    unsigned z = 0;
+    CK(hipMemcpyAsync(&s->D.counters[3], &z, sizeof z, hipMemcpyHostToDevice, s->st));
+    CK(hipStreamSynchronize(s->st));
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_sync(fmdx_system_t *s)
+{
+    if (!s) return FMDX_EARG;
+    CK(hipStreamSynchronize(s->st));
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_set_capacity(fmdx_system_t *s, unsigned cap)
+{
+    if (!s || !s->uploaded || cap < 8 || cap > 256) return FMDX_EARG;
+    unsigned ctr[8];
+    CK(hipMemcpyAsync(ctr, s->D.counters, sizeof ctr, hipMemcpyDeviceToHost, s->st));
+    CK(hipStreamSynchronize(s->st));
+    if (cap < ctr[2]) return FMDX_EARG;
+    KDesc N = s->D;
+    N.x = N.y = N.z = N.vx = N.vy = N.vz = N.fx = N.fy = N.fz = N.aux = nullptr;
+    N.xr = N.yr = N.zr = nullptr;
+    N.grp = nullptr; N.aid = nullptr; N.kind = nullptr; N.mv = nullptr; N.code = nullptr;
+    N.bkey = nullptr; N.bid = nullptr;
+    N.mvx = N.mvy = N.mvz = N.mvvx = N.mvvy = N.mvvz = nullptr;
+    N.mvgrp = nullptr; N.mvaid = nullptr; N.mvkind = nullptr; N.part = nullptr;
+    int r = alloc_slots(N, (int)cap, s->natoms_uploaded + 64);
+    if (r) return r;
+    CK(launch_relayout(s->D, N, s->st));
+    CK(hipStreamSynchronize(s->st));
+    free_slots(s->D);
+    s->D = N;
+    return FMDX_OK;
+}
+
+/* ------------------------------------------------------------------ TTM */
+
+template <class T>
+static int ttm_alloc(fmdx_system_t *s, T **p, size_t n)
+{
+    int r = dalloc(p, n);
+    if (r) return r;
+    s->ttm_mem[s->nttm_mem++] = (void *)*p;
+    hipMemsetAsync(*p, 0, n * sizeof(T), s->st);
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_ttm_setup(fmdx_system_t *s, const fmdx_ttm_t *c, const double *Te_init)
+{
+    if (!s || !c || c->ncz < 1 || c->ncz > FMDX_TTM_MAX_CELLS || c->ratio < 1) return FMDX_EARG;
+    if (c->type != 1 && c->type != 2) return FMDX_EARG;
+    if (c->type == 2 && (c->nC < 2 || c->nG < 2 || !c->T_C || !c->Ct || !c->C_DD || !c->T_G || !c->Gt || !c->G_DD))
+        return FMDX_EARG;
+    for (int i = 0; i < s->nttm_mem; i++) hipFree(s->ttm_mem[i]);
+    s->nttm_mem = 0;
+    TtmState &S = s->T;
+    memset(&S, 0, sizeof S);
+    S.type = c->type; S.ncz = c->ncz; S.n = c->ncz + 2; S.ratio = c->ratio;
+    S.tcellh_z = c->tcellh_z; S.tvol = c->tcell_volume;
+    S.C_gamma = c->C_gamma; S.K = c->K; S.G = c->G;
+    S.Kzh_A = c->Kzh_A; S.Kzh_B = c->Kzh_B; S.Kzh_v2 = c->Kzh_v2;
+    S.nC = c->nC; S.nG = c->nG;
+    S.laser_factor = c->laser_factor; S.laser_absdepth_inv = c->laser_absdepth_inv;
+    S.laser_2sig2_inv = c->laser_2sig2_inv; S.laser_t0 = c->laser_t0; S.laser_tdiff = c->laser_tdiff;
+    S.frontsurf = c->frontsurf; S.min_atoms = c->min_atoms;
+    int r = 0;
+    r |= ttm_alloc(s, &S.mass, S.n); r |= ttm_alloc(s, &S.vcm, 3 * S.n); r |= ttm_alloc(s, &S.Ti, S.n);
+    r |= ttm_alloc(s, &S.Te, S.n); r |= ttm_alloc(s, &S.Te2, S.n); r |= ttm_alloc(s, &S.xi, S.n);
+    r |= ttm_alloc(s, &S.Ke, S.n); r |= ttm_alloc(s, &S.Ce, S.n); r |= ttm_alloc(s, &S.Gc, S.n);
+    r |= ttm_alloc(s, &S.num, S.n);
+    if (r) return r;
+    if (c->type == 2)
+    {
+        double *tc, *ct, *cdd, *tg, *gt, *gdd;
+        r |= ttm_alloc(s, &tc, c->nC); r |= ttm_alloc(s, &ct, c->nC); r |= ttm_alloc(s, &cdd, c->nC);
+        r |= ttm_alloc(s, &tg, c->nG); r |= ttm_alloc(s, &gt, c->nG); r |= ttm_alloc(s, &gdd, c->nG);
+        if (r) return r;
+        CK(hipMemcpyAsync(tc, c->T_C, sizeof(double) * c->nC, hipMemcpyHostToDevice, s->st));
+        CK(hipMemcpyAsync(ct, c->Ct, sizeof(double) * c->nC, hipMemcpyHostToDevice, s->st));
+        CK(hipMemcpyAsync(cdd, c->C_DD, sizeof(double) * c->nC, hipMemcpyHostToDevice, s->st));
+        CK(hipMemcpyAsync(tg, c->T_G, sizeof(double) * c->nG, hipMemcpyHostToDevice, s->st));
+        CK(hipMemcpyAsync(gt, c->Gt, sizeof(double) * c->nG, hipMemcpyHostToDevice, s->st));
+        CK(hipMemcpyAsync(gdd, c->G_DD, sizeof(double) * c->nG, hipMemcpyHostToDevice, s->st));
+        S.T_C = tc; S.Ct = ct; S.C_DD = cdd; S.T_G = tg; S.Gt = gt; S.G_DD = gdd;
+    }
+    if (Te_init) CK(hipMemcpyAsync(S.Te, Te_init, sizeof(double) * S.n, hipMemcpyHostToDevice, s->st));
+    CK(hipStreamSynchronize(s->st));
+    s->ttm_on = 1;
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_ttm_set_frontsurf(fmdx_system_t *s, double frontsurf, unsigned min_atoms)
+{
+    if (!s || !s->ttm_on) return FMDX_ESTATE;
+    s->T.frontsurf = frontsurf;
+    s->T.min_atoms = min_atoms;
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_ttm_fields(fmdx_system_t *s)
+{
+    if (!s || !s->ttm_on || !s->uploaded) return FMDX_ESTATE;
+    CK(launch_ttm_fields(s->D, s->T, s->st));
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_ttm_update(fmdx_system_t *s, double time, double dt)
+{
+    if (!s || !s->ttm_on || !s->uploaded) return FMDX_ESTATE;
+    CK(launch_ttm_fields(s->D, s->T, s->st));
+    CK(launch_ttm_presolve(s->T, &s->D.counters[3], s->st));
+    CK(launch_ttm_solve(s->T, time, dt, &s->D.counters[3], s->st));
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_ttm_presolve(fmdx_system_t *s)
+{
+    if (!s || !s->ttm_on) return FMDX_ESTATE;
+    CK(launch_ttm_presolve(s->T, &s->D.counters[3], s->st));
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_ttm_solve(fmdx_system_t *s, double time, double dt)
+{
+    if (!s || !s->ttm_on) return FMDX_ESTATE;
+    CK(launch_ttm_solve(s->T, time, dt, &s->D.counters[3], s->st));
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_ttm_download(fmdx_system_t *s, double *Te, double *Ti, double *xi, double *vcm, unsigned *num,
+                                 double *mass)
+{
+    if (!s || !s->ttm_on) return FMDX_ESTATE;
+    const TtmState &S = s->T;
+    if (Te) CK(hipMemcpyAsync(Te, S.Te, sizeof(double) * S.n, hipMemcpyDeviceToHost, s->st));
+    if (Ti) CK(hipMemcpyAsync(Ti, S.Ti, sizeof(double) * S.n, hipMemcpyDeviceToHost, s->st));
+    if (xi) CK(hipMemcpyAsync(xi, S.xi, sizeof(double) * S.n, hipMemcpyDeviceToHost, s->st));
+    if (vcm) CK(hipMemcpyAsync(vcm, S.vcm, sizeof(double) * 3 * S.n, hipMemcpyDeviceToHost, s->st));
+    if (num) CK(hipMemcpyAsync(num, S.num, sizeof(unsigned) * S.n, hipMemcpyDeviceToHost, s->st));
+    if (mass) CK(hipMemcpyAsync(mass, S.mass, sizeof(double) * S.n, hipMemcpyDeviceToHost, s->st));
+    CK(hipStreamSynchronize(s->st));
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_ttm_upload_te(fmdx_system_t *s, const double *Te)
+{
+    if (!s || !s->ttm_on || !Te) return FMDX_ESTATE;
+    CK(hipMemcpyAsync(s->T.Te, Te, sizeof(double) * s->T.n, hipMemcpyHostToDevice, s->st));
+    CK(hipStreamSynchronize(s->st));
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_find_limits(fmdx_system_t *s, int group, double lo[3], double hi[3])
+{
+    if (!s || !s->uploaded) return FMDX_ESTATE;
+    unsigned n = 0;
+    int r = fmdx_download(s, &n, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    if (r) return r;
+    std::vector<double> x(3 * (size_t)(n ? n : 1));
+    std::vector<int> g(n ? n : 1);
+    r = fmdx_download(s, &n, nullptr, x.data(), nullptr, nullptr, nullptr, g.data(), nullptr, nullptr);
+    if (r) return r;
+    for (int d = 0; d < 3; d++) { lo[d] = 1.7976931348623157e308; hi[d] = 2.2250738585072014e-308; }
+    for (unsigned i = 0; i < n; i++)
+        if (group == -1 || g[i] == group)
+            for (int d = 0; d < 3; d++)
+            {
+                if (x[3 * i + d] < lo[d]) lo[d] = x[3 * i + d];
+                if (x[3 * i + d] > hi[d]) hi[d] = x[3 * i + d];
+            }
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_event_record(fmdx_system_t *s, int slot)
+{
+    if (!s || slot < 0 || slot >= 14) return FMDX_EARG;
+    CK(hipEventRecord(s->ev[slot], s->st));
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_event_elapsed_ms(fmdx_system_t *s, int a, int b, float *ms)
+{
+    if (!s || a < 0 || b < 0 || a >= 14 || b >= 14) return FMDX_EARG;
+    CK(hipEventSynchronize(s->ev[b]));
+    CK(hipEventElapsedTime(ms, s->ev[a], s->ev[b]));
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_profile_enable(fmdx_system_t *s, int on)
+{
+    if (!s) return FMDX_EARG;
+    s->profile = on;
+    s->prof_n = 0;
+    return FMDX_OK;
+}
+
+/* ms[0] = pass-1 (EAM density) kernel time, ms[1] = pass-0 (force) kernel time,
+   ms[2] = both, summed over the profiled force calls; count = number of calls */
+extern "C" int fmdx_profile_read(fmdx_system_t *s, double *ms, unsigned *count)
+{
+    if (!s || !ms) return FMDX_EARG;
+    for (int i = 0; i < 8; i++) ms[i] = 0.0;
+    CK(hipStreamSynchronize(s->st));
+    for (unsigned k = 0; k < s->prof_n; k++)
+    {
+        float a = 0, b = 0;
+        CK(hipEventElapsedTime(&a, s->pev[3 * k], s->pev[3 * k + 1]));
+        CK(hipEventElapsedTime(&b, s->pev[3 * k + 1], s->pev[3 * k + 2]));
+        ms[0] += a;
+        ms[1] += b;
+        ms[2] += a + b;
+    }
+    if (count) *count = s->prof_n;
+    return FMDX_OK;
+}

@@ -1,0 +1,152 @@
+/*
+ * fmdx_internal.h -- device-side layout shared by the gfx950 kernels.
+ *
+ * HBM layout of one system (all arrays SoA):
+ *   cells   : interior cells only, flat index c = (ix*ny + iy)*nz + iz, the
+ *             reference's interior storage order (subdomain.c:201-222, ns=1).
+ *             There is no ghost layer: periodic images are formed inside the
+ *             kernels by shifting positions by the box length.
+ *   slots   : cell c owns slots [c*cap, c*cap + cnt[c]); the within-cell order
+ *             of those slots is the reference's within-cell atom order.
+ *   per slot: x y z vx vy vz fx fy fz aux (f64), xr yr zr (f32, cell-relative,
+ *             mixed build), grp (i32), aid (u32), kind (u8), mv (i32 binning
+ *             destination), code (u8 migration code)
+ */
+#ifndef FMDX_INTERNAL_H
+#define FMDX_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "fmdx.h"
+
+#define KB_EV 8.6173303e-5     /* general.h:46 */
+#define MAXK FMDX_MAX_KINDS
+#define MAXE FMDX_MAX_ELEMENTS
+#define NPART_MAX 4096         /* blocks reduced by the single-block reductions */
+
+/* mover destination codes */
+#define MV_STAY (-1)
+#define MV_DELETE (-2)
+
+struct PairParam
+{
+    int cat;
+    int iloc, jloc;
+    int pad;
+    double rc2;
+    double p0, p1, p2;
+};
+
+/* per-interval cubic coefficients of a reference natural spline:
+   value = c0 + b*(c1 + b*(c2 + b*c3)),  b = x/h - k  (cspline.h:26-27 rewritten in b) */
+struct EamDev
+{
+    int ne, nrho, nr2;
+    double drho, inv_drho, dr2, inv_dr2, rc2;
+    const double4 *Fc;     /* [ne][nrho-1] */
+    const double4 *rhoc;   /* [ne][nr2-1]  */
+    const double4 *phic;   /* [ne*ne][nr2-1] */
+    const float4 *Fc32, *rhoc32, *phic32;
+};
+
+struct TtmDev
+{
+    int on;
+    int ncz;
+    double tcellh_z;
+    const double *xi;      /* [ncz+2] */
+    const double *vcm;     /* [3*(ncz+2)] */
+};
+
+struct KDesc
+{
+    /* geometry */
+    int nc0, nc1, nc2, ncell;
+    int cap;
+    int pbc0, pbc1, pbc2;
+    double L0, L1, L2;
+    double h0, h1, h2;
+    /* slots */
+    double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz, *aux;
+    float *xr, *yr, *zr;
+    int *grp;
+    unsigned *aid;
+    uint8_t *kind;
+    int *cnt;
+    /* binning scratch */
+    int *mv;
+    uint8_t *code;
+    int *nleave, *narr, *nearly, *fill;
+    unsigned long long *bkey;   /* [ncell*cap] arrival buckets */
+    int *bid;
+    double *mvx, *mvy, *mvz, *mvvx, *mvvy, *mvvz;   /* mover staging */
+    int *mvgrp;
+    unsigned *mvaid;
+    uint8_t *mvkind;
+    unsigned *counters;   /* [0] movers, [1] deleted, [2] max cell count, [3] error word */
+    /* potentials */
+    int nkinds;
+    int single_cat;        /* category when one category is used everywhere, else -1 */
+    int has_eam;
+    double rcmax2, rceam2;
+    double mass[MAXK];
+    int elem[MAXK];
+    const PairParam *pairs;   /* [MAXK*MAXK] */
+    EamDev eam;
+    /* reductions */
+    double *cell_pe;       /* [ncell] pair energy per home cell */
+    double *cell_emb;      /* [ncell] embedding energy per home cell */
+    double *part;          /* [NPART_MAX*8] per-block thermo partials */
+    double *thermo;        /* fmdx_thermo_t mirror, doubles first */
+};
+
+/* 1-D TTM state resident on the device (arrays of ncz+2 turi cells, ghosts included) */
+struct TtmState
+{
+    int type, ncz, n, ratio;
+    double tcellh_z, tvol;
+    double C_gamma, K, G, Kzh_A, Kzh_B, Kzh_v2;
+    int nC, nG;
+    const double *T_C, *Ct, *C_DD, *T_G, *Gt, *G_DD;
+    double laser_factor, laser_absdepth_inv, laser_2sig2_inv, laser_t0, laser_tdiff;
+    double frontsurf;
+    unsigned min_atoms;
+    double *mass, *vcm, *Ti, *Te, *Te2, *xi, *Ke, *Ce, *Gc;
+    unsigned *num;
+};
+
+/* thermo record layout in device memory (doubles) */
+enum
+{
+    TH_PE = 0, TH_EMB, TH_MV2, TH_KE, TH_T, TH_PX, TH_PY, TH_PZ, TH_N, TH_NDOUBLES
+};
+
+static __device__ __forceinline__ void set_err(const KDesc &D, unsigned bits)
+{
+    atomicOr(&D.counters[3], bits);
+}
+
+/* kernel launchers (defined in the kernel translation units) */
+hipError_t launch_vv_start(const KDesc &D, hipStream_t st, int ag, double dt, const double *hdtm,
+                           int thermostat, double tau, double T0);
+hipError_t launch_vv_finish(const KDesc &D, hipStream_t st, int ag, const double *hdtm);
+hipError_t launch_se_step(const KDesc &D, hipStream_t st, int ag, double dt);
+hipError_t launch_thermo(const KDesc &D, hipStream_t st, int ag);
+hipError_t launch_flag_only(const KDesc &D, hipStream_t st);
+hipError_t launch_bin(const KDesc &D, hipStream_t st);
+hipError_t launch_reduce_pe(const KDesc &D, hipStream_t st);
+hipError_t launch_forces(const KDesc &D, hipStream_t st, int ag, int precision, const TtmDev &T);
+hipError_t launch_pass1(const KDesc &D, hipStream_t st, int ag, int precision, const TtmDev &T);
+hipError_t launch_pass0(const KDesc &D, hipStream_t st, int ag, int precision, const TtmDev &T);
+hipError_t launch_add_velocity(const KDesc &D, hipStream_t st, int group, double vx, double vy, double vz);
+hipError_t launch_relayout(const KDesc &Dold, const KDesc &Dnew, hipStream_t st);
+hipError_t launch_scatter(const KDesc &D, hipStream_t st, const int *cell_start, const double *x, const double *v,
+                          const int *grp, const unsigned *aid, const uint8_t *kind);
+hipError_t launch_gather(const KDesc &D, hipStream_t st, const int *cell_start, double *x, double *v, double *f,
+                         double *aux, int *grp, unsigned *aid, uint8_t *kind);
+hipError_t launch_ttm_fields(const KDesc &D, const TtmState &S, hipStream_t st);
+hipError_t launch_ttm_presolve(const TtmState &S, unsigned *err, hipStream_t st);
+hipError_t launch_ttm_solve(const TtmState &S, double time, double dt, unsigned *err, hipStream_t st);
+hipError_t launch_fill(double *p, long n, double v, hipStream_t st);
+
+#endif /* FMDX_INTERNAL_H */

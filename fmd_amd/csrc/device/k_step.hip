@@ -1,0 +1,607 @@
+/*
+ * k_step.hip -- O(N) step kernels: velocity Verlet / Berendsen / symplectic
+ * Euler, the exact-order cell re-binning, and the deterministic reductions.
+ *
+ * Integration arithmetic follows the reference expression by expression with
+ * FMA contraction disabled, so one FP64 step from identical inputs gives the
+ * reference's bits (integrators.c:63-70, 115-121, 144-145).
+ *
+ * Re-binning reproduces _fmd_refreshGrid + _fmd_particles_migrate
+ * (misc.c:268-308, md-ghost.c:827-869) without their sequential walk:
+ *   1. every atom finds its destination (fused into the integrator kernels);
+ *      arrivals per destination are counted, and "early" arrivals -- direct
+ *      moves from a cell of lower storage index, which the reference appends
+ *      before the destination cell is itself scanned -- are counted apart;
+ *   2. per cell with leavers, the reference's swap-with-last scan
+ *      (cell.c:151-160) is replayed on indices only; it gives each leaver its
+ *      detection rank, and leavers are staged with their migration shift;
+ *   3. per affected cell, arrivals are ordered by
+ *      (migration code c2,c1,c0 ; source cell ; detection rank), the scan is
+ *      replayed once more to move stayers and early arrivals into place, and
+ *      the remaining arrivals are appended.
+ * The order key encodes the migration history: an atom that crossed the box
+ * in dimension d was appended during phase d of the migration (lower face
+ * before upper face), after every atom that reached the same cell earlier.
+ */
+#include "fmdx_internal.h"
+
+#define TPB 256
+
+struct Hd
+{
+    double v[MAXK];
+};
+
+static __device__ __forceinline__ void cell_coords(const KDesc &D, int c, int &cx, int &cy, int &cz)
+{
+    cz = c % D.nc2;
+    int t = c / D.nc2;
+    cy = t % D.nc1;
+    cx = t / D.nc1;
+}
+
+/* destination of one atom after it moved; misc.c:281-300 and md-ghost.c:827-869 */
+static __device__ void bin_flag(const KDesc &D, int c, int slot, double x, double y, double z)
+{
+#pragma clang fp contract(off)
+    int cx, cy, cz;
+    cell_coords(D, c, cx, cy, cz);
+    int j0 = (int)floor(x / D.h0) + 1;
+    int j1 = (int)floor(y / D.h1) + 1;
+    int j2 = (int)floor(z / D.h2) + 1;
+    if (j0 < 0 || j0 >= D.nc0 + 2 || j1 < 0 || j1 >= D.nc1 + 2 || j2 < 0 || j2 >= D.nc2 + 2)
+    {
+        set_err(D, FMDX_DEV_ERR_POSITION);
+        D.mv[slot] = MV_STAY;
+        return;
+    }
+    if (j0 == cx + 1 && j1 == cy + 1 && j2 == cz + 1)
+    {
+        D.mv[slot] = MV_STAY;
+        return;
+    }
+    int f[3], cd[3];
+    bool del = false;
+    const int j[3] = {j0, j1, j2};
+    const int n[3] = {D.nc0, D.nc1, D.nc2};
+    const int p[3] = {D.pbc0, D.pbc1, D.pbc2};
+    for (int d = 0; d < 3; d++)
+    {
+        cd[d] = 0;
+        if (j[d] == 0)
+        {
+            if (p[d]) { f[d] = n[d] - 1; cd[d] = 1; }
+            else del = true;
+        }
+        else if (j[d] == n[d] + 1)
+        {
+            if (p[d]) { f[d] = 0; cd[d] = 2; }
+            else del = true;
+        }
+        else
+            f[d] = j[d] - 1;
+    }
+    int code = cd[2] * 16 + cd[1] * 4 + cd[0];
+    atomicAdd(&D.nleave[c], 1);
+    D.code[slot] = (uint8_t)code;
+    if (del)
+    {
+        D.mv[slot] = MV_DELETE;
+        return;
+    }
+    int dest = (f[0] * D.nc1 + f[1]) * D.nc2 + f[2];
+    D.mv[slot] = dest;
+    atomicAdd(&D.narr[dest], 1);
+    if (code == 0 && c < dest) atomicAdd(&D.nearly[dest], 1);
+}
+
+/* VelocityVerlet_startStep (integrators.c:40-75) fused with the binning flag */
+__global__ __launch_bounds__(TPB) void k_vv_start(KDesc D, int ag, double dt, Hd hd, int thermostat,
+                                                   double tau, double T0)
+{
+#pragma clang fp contract(off)
+    if (blockIdx.x == 0 && threadIdx.x == 0) D.counters[0] = 0;
+    long slot = (long)blockIdx.x * TPB + threadIdx.x;
+    if (slot >= (long)D.ncell * D.cap) return;
+    int c = (int)(slot / D.cap);
+    int k = (int)(slot - (long)c * D.cap);
+    if (k >= D.cnt[c]) return;
+    double x = D.x[slot], y = D.y[slot], z = D.z[slot];
+    if (ag < 0 || D.grp[slot] == ag)
+    {
+        double lambda = 1.0;
+        if (thermostat)
+        {
+            double T = D.thermo[TH_T];
+            lambda = sqrt(1 + dt / tau * (T0 / T - 1));
+        }
+        double h = hd.v[D.kind[slot]];
+        double vx = D.vx[slot], vy = D.vy[slot], vz = D.vz[slot];
+        if (thermostat)
+        {
+            vx *= lambda;
+            vy *= lambda;
+            vz *= lambda;
+        }
+        vx += h * D.fx[slot];
+        x += dt * vx;
+        vy += h * D.fy[slot];
+        y += dt * vy;
+        vz += h * D.fz[slot];
+        z += dt * vz;
+        D.vx[slot] = vx; D.vy[slot] = vy; D.vz[slot] = vz;
+        D.x[slot] = x; D.y[slot] = y; D.z[slot] = z;
+    }
+    bin_flag(D, c, (int)slot, x, y, z);
+}
+
+/* SymplecticEuler_takeOneStep, first half (integrators.c:134-147) + binning flag */
+__global__ __launch_bounds__(TPB) void k_se_step(KDesc D, int ag, double dt)
+{
+#pragma clang fp contract(off)
+    if (blockIdx.x == 0 && threadIdx.x == 0) D.counters[0] = 0;
+    long slot = (long)blockIdx.x * TPB + threadIdx.x;
+    if (slot >= (long)D.ncell * D.cap) return;
+    int c = (int)(slot / D.cap);
+    int k = (int)(slot - (long)c * D.cap);
+    if (k >= D.cnt[c]) return;
+    double x = D.x[slot], y = D.y[slot], z = D.z[slot];
+    if (ag < 0 || D.grp[slot] == ag)
+    {
+        double m = D.mass[D.kind[slot]];
+        double vx = D.vx[slot], vy = D.vy[slot], vz = D.vz[slot];
+        vx += dt * D.fx[slot] / m;
+        x += dt * vx;
+        vy += dt * D.fy[slot] / m;
+        y += dt * vy;
+        vz += dt * D.fz[slot] / m;
+        z += dt * vz;
+        D.vx[slot] = vx; D.vy[slot] = vy; D.vz[slot] = vz;
+        D.x[slot] = x; D.y[slot] = y; D.z[slot] = z;
+    }
+    bin_flag(D, c, (int)slot, x, y, z);
+}
+
+/* binning flag without motion (_fmd_refreshGrid called on its own) */
+__global__ __launch_bounds__(TPB) void k_flag_only(KDesc D)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) D.counters[0] = 0;
+    long slot = (long)blockIdx.x * TPB + threadIdx.x;
+    if (slot >= (long)D.ncell * D.cap) return;
+    int c = (int)(slot / D.cap);
+    int k = (int)(slot - (long)c * D.cap);
+    if (k >= D.cnt[c]) return;
+    bin_flag(D, c, (int)slot, D.x[slot], D.y[slot], D.z[slot]);
+}
+
+/* per-block deterministic tree sum of NQ values into part[blockIdx.x*8 + q] */
+template <int NQ>
+static __device__ void block_partials(const KDesc &D, double (&q)[NQ])
+{
+    __shared__ double sh[NQ][TPB];
+    for (int i = 0; i < NQ; i++) sh[i][threadIdx.x] = q[i];
+    __syncthreads();
+    for (int s = TPB / 2; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s)
+            for (int i = 0; i < NQ; i++) sh[i][threadIdx.x] += sh[i][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int i = 0; i < NQ; i++) D.part[(long)blockIdx.x * 8 + i] = sh[i][0];
+}
+
+/* VelocityVerlet_finishStep, integrators.c:94-126: second kick + group sums */
+__global__ __launch_bounds__(TPB) void k_vv_finish(KDesc D, int ag, Hd hd)
+{
+#pragma clang fp contract(off)
+    long slot = (long)blockIdx.x * TPB + threadIdx.x;
+    double q[5] = {0, 0, 0, 0, 0};
+    if (slot < (long)D.ncell * D.cap)
+    {
+        int c = (int)(slot / D.cap);
+        int k = (int)(slot - (long)c * D.cap);
+        if (k < D.cnt[c] && (ag < 0 || D.grp[slot] == ag))
+        {
+            int kd = D.kind[slot];
+            double h = hd.v[kd], m = D.mass[kd];
+            double vx = D.vx[slot] + h * D.fx[slot];
+            double vy = D.vy[slot] + h * D.fy[slot];
+            double vz = D.vz[slot] + h * D.fz[slot];
+            D.vx[slot] = vx; D.vy[slot] = vy; D.vz[slot] = vz;
+            q[0] = m * (vx * vx + vy * vy + vz * vz);
+            q[1] = m * vx;
+            q[2] = m * vy;
+            q[3] = m * vz;
+            q[4] = 1.0;
+        }
+    }
+    block_partials<5>(D, q);
+}
+
+/* group sums without a kick: matter.c:31-68 and integrators.c:154-176 */
+__global__ __launch_bounds__(TPB) void k_thermo(KDesc D, int ag)
+{
+#pragma clang fp contract(off)
+    long slot = (long)blockIdx.x * TPB + threadIdx.x;
+    double q[5] = {0, 0, 0, 0, 0};
+    if (slot < (long)D.ncell * D.cap)
+    {
+        int c = (int)(slot / D.cap);
+        int k = (int)(slot - (long)c * D.cap);
+        if (k < D.cnt[c] && (ag < 0 || D.grp[slot] == ag))
+        {
+            double m = D.mass[D.kind[slot]];
+            double vx = D.vx[slot], vy = D.vy[slot], vz = D.vz[slot];
+            q[0] = m * (vx * vx + vy * vy + vz * vz);
+            q[1] = m * vx;
+            q[2] = m * vy;
+            q[3] = m * vz;
+            q[4] = 1.0;
+        }
+    }
+    block_partials<5>(D, q);
+}
+
+/* fixed-order reduction of the per-block partials: updateGroupProperties, integrators.c:78-92 */
+__global__ __launch_bounds__(1024) void k_reduce_thermo(KDesc D, int nblocks)
+{
+#pragma clang fp contract(off)
+    __shared__ double sh[5][1024];
+    double q[5] = {0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nblocks; b += 1024)
+        for (int i = 0; i < 5; i++) q[i] += D.part[(long)b * 8 + i];
+    for (int i = 0; i < 5; i++) sh[i][threadIdx.x] = q[i];
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s)
+            for (int i = 0; i < 5; i++) sh[i][threadIdx.x] += sh[i][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        double mv2 = sh[0][0];
+        double n = sh[4][0];
+        D.thermo[TH_MV2] = mv2;
+        D.thermo[TH_KE] = 0.5 * mv2;
+        D.thermo[TH_T] = mv2 / (3.0 * n * KB_EV);
+        D.thermo[TH_PX] = sh[1][0];
+        D.thermo[TH_PY] = sh[2][0];
+        D.thermo[TH_PZ] = sh[3][0];
+        D.thermo[TH_N] = n;
+    }
+}
+
+/* fixed-order sum of the per-cell pair and embedding energies */
+__global__ __launch_bounds__(1024) void k_reduce_pe(KDesc D)
+{
+    __shared__ double sh[2][1024];
+    double pe = 0, emb = 0;
+    for (int c = threadIdx.x; c < D.ncell; c += 1024)
+    {
+        pe += D.cell_pe[c];
+        emb += D.cell_emb[c];
+    }
+    sh[0][threadIdx.x] = pe;
+    sh[1][threadIdx.x] = emb;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s)
+        {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        D.thermo[TH_PE] = sh[0][0] + sh[1][0];
+        D.thermo[TH_EMB] = sh[1][0];
+    }
+}
+
+/* ----------------------------------------------------------- binning */
+
+static __device__ __forceinline__ void copy_slot(const KDesc &D, long from, long to)
+{
+    D.x[to] = D.x[from]; D.y[to] = D.y[from]; D.z[to] = D.z[from];
+    D.vx[to] = D.vx[from]; D.vy[to] = D.vy[from]; D.vz[to] = D.vz[from];
+    D.grp[to] = D.grp[from]; D.aid[to] = D.aid[from]; D.kind[to] = D.kind[from];
+}
+
+static __device__ __forceinline__ void copy_staged(const KDesc &D, int mid, long to)
+{
+    D.x[to] = D.mvx[mid]; D.y[to] = D.mvy[mid]; D.z[to] = D.mvz[mid];
+    D.vx[to] = D.mvvx[mid]; D.vy[to] = D.mvvy[mid]; D.vz[to] = D.mvvz[mid];
+    D.grp[to] = D.mvgrp[mid]; D.aid[to] = D.mvaid[mid]; D.kind[to] = D.mvkind[mid];
+}
+
+/* replay of the swap-remove scan (misc.c:279-303) to rank and stage leavers */
+__global__ __launch_bounds__(TPB) void k_bin_rank(KDesc D)
+{
+#pragma clang fp contract(off)
+    int c = blockIdx.x * TPB + threadIdx.x;
+    if (c >= D.ncell) return;
+    if (D.nleave[c] == 0) return;
+    const int nc = D.cnt[c];
+    const long base = (long)c * D.cap;
+    int i = 0, n = nc + D.nearly[c], cur = 0, rank = 0;
+    while (i < n)
+    {
+        if (cur < nc && D.mv[base + cur] != MV_STAY)
+        {
+            long s = base + cur;
+            int dst = D.mv[s];
+            if (dst == MV_DELETE)
+                atomicAdd(&D.counters[1], 1u);
+            else
+            {
+                int mid = (int)atomicAdd(&D.counters[0], 1u);
+                int code = D.code[s];
+                double x = D.x[s], y = D.y[s], z = D.z[s];
+                /* ccopy_for_migrate, md-ghost.c:132-140: x += L (lower face) or x += -L (upper) */
+                if ((code & 3) == 1) x += D.L0;
+                else if ((code & 3) == 2) x += -D.L0;
+                if (((code >> 2) & 3) == 1) y += D.L1;
+                else if (((code >> 2) & 3) == 2) y += -D.L1;
+                if (((code >> 4) & 3) == 1) z += D.L2;
+                else if (((code >> 4) & 3) == 2) z += -D.L2;
+                D.mvx[mid] = x; D.mvy[mid] = y; D.mvz[mid] = z;
+                D.mvvx[mid] = D.vx[s]; D.mvvy[mid] = D.vy[s]; D.mvvz[mid] = D.vz[s];
+                D.mvgrp[mid] = D.grp[s]; D.mvaid[mid] = D.aid[s]; D.mvkind[mid] = D.kind[s];
+                unsigned long long key = ((unsigned long long)code << 58) |
+                                         ((unsigned long long)c << 16) | (unsigned long long)rank;
+                int pos = atomicAdd(&D.fill[dst], 1);
+                if (pos < D.cap)
+                {
+                    D.bkey[(long)dst * D.cap + pos] = key;
+                    D.bid[(long)dst * D.cap + pos] = mid;
+                }
+                else
+                    set_err(D, FMDX_DEV_ERR_OVERFLOW);
+            }
+            rank++;
+            n--;
+            if (n != i) cur = n;
+        }
+        else
+        {
+            i++;
+            cur = i;
+        }
+    }
+}
+
+/* place stayers, early and late arrivals of every affected cell */
+__global__ __launch_bounds__(TPB) void k_bin_apply(KDesc D)
+{
+    int c = blockIdx.x * TPB + threadIdx.x;
+    if (c >= D.ncell) return;
+    const int nl = D.nleave[c], na = D.narr[c];
+    if (nl == 0 && na == 0) return;
+    const int nc = D.cnt[c];
+    const int A = D.nearly[c];
+    const long base = (long)c * D.cap;
+    const int nb = na < D.cap ? na : D.cap;
+    /* insertion sort of the arrival bucket by key */
+    for (int a = 1; a < nb; a++)
+    {
+        unsigned long long k = D.bkey[base + a];
+        int id = D.bid[base + a];
+        int b = a - 1;
+        while (b >= 0 && D.bkey[base + b] > k)
+        {
+            D.bkey[base + b + 1] = D.bkey[base + b];
+            D.bid[base + b + 1] = D.bid[base + b];
+            b--;
+        }
+        D.bkey[base + b + 1] = k;
+        D.bid[base + b + 1] = id;
+    }
+    int i = 0, n = nc + A, cur = 0;
+    while (i < n)
+    {
+        if (cur < nc && D.mv[base + cur] != MV_STAY)
+        {
+            n--;
+            if (n != i) cur = n;
+        }
+        else
+        {
+            if (cur >= nc)
+            {
+                int q = cur - nc;
+                if (q < nb && i < D.cap) copy_staged(D, D.bid[base + q], base + i);
+            }
+            else if (cur != i)
+                copy_slot(D, base + cur, base + i);
+            i++;
+            cur = i;
+        }
+    }
+    int total = n;
+    for (int q = A; q < na; q++)
+    {
+        if (q < nb && total < D.cap) copy_staged(D, D.bid[base + q], base + total);
+        total++;
+    }
+    if (total > D.cap)
+    {
+        set_err(D, FMDX_DEV_ERR_OVERFLOW);
+        total = D.cap;
+    }
+    D.cnt[c] = total;
+    atomicMax(&D.counters[2], (unsigned)total);
+    D.nleave[c] = 0;
+    D.narr[c] = 0;
+    D.nearly[c] = 0;
+    D.fill[c] = 0;
+}
+
+__global__ __launch_bounds__(TPB) void k_add_velocity(KDesc D, int group, double vx, double vy, double vz)
+{
+#pragma clang fp contract(off)
+    long slot = (long)blockIdx.x * TPB + threadIdx.x;
+    if (slot >= (long)D.ncell * D.cap) return;
+    int c = (int)(slot / D.cap);
+    int k = (int)(slot - (long)c * D.cap);
+    if (k >= D.cnt[c]) return;
+    if (group == -1 || D.grp[slot] == group)
+    {
+        D.vx[slot] += vx;
+        D.vy[slot] += vy;
+        D.vz[slot] += vz;
+    }
+}
+
+/* copy every cell into a layout with another slot capacity */
+__global__ __launch_bounds__(TPB) void k_relayout(KDesc A, KDesc B)
+{
+    long slot = (long)blockIdx.x * TPB + threadIdx.x;
+    if (slot >= (long)A.ncell * A.cap) return;
+    int c = (int)(slot / A.cap);
+    int k = (int)(slot - (long)c * A.cap);
+    if (k >= A.cnt[c] || k >= B.cap) return;
+    long t = (long)c * B.cap + k;
+    B.x[t] = A.x[slot]; B.y[t] = A.y[slot]; B.z[t] = A.z[slot];
+    B.vx[t] = A.vx[slot]; B.vy[t] = A.vy[slot]; B.vz[t] = A.vz[slot];
+    B.fx[t] = A.fx[slot]; B.fy[t] = A.fy[slot]; B.fz[t] = A.fz[slot];
+    B.aux[t] = A.aux[slot];
+    B.grp[t] = A.grp[slot]; B.aid[t] = A.aid[slot]; B.kind[t] = A.kind[slot];
+}
+
+/* ------------------------------------------------------------ launchers */
+
+static inline unsigned nblk(const KDesc &D) { return (unsigned)(((long)D.ncell * D.cap + TPB - 1) / TPB); }
+
+static Hd make_hd(const double *hdtm)
+{
+    Hd h;
+    for (int i = 0; i < MAXK; i++) h.v[i] = hdtm ? hdtm[i] : 0.0;
+    return h;
+}
+
+hipError_t launch_bin(const KDesc &D, hipStream_t st)
+{
+    unsigned g = (D.ncell + TPB - 1) / TPB;
+    hipLaunchKernelGGL(k_bin_rank, dim3(g), dim3(TPB), 0, st, D);
+    hipLaunchKernelGGL(k_bin_apply, dim3(g), dim3(TPB), 0, st, D);
+    return hipGetLastError();
+}
+
+hipError_t launch_vv_start(const KDesc &D, hipStream_t st, int ag, double dt, const double *hdtm,
+                           int thermostat, double tau, double T0)
+{
+    hipLaunchKernelGGL(k_vv_start, dim3(nblk(D)), dim3(TPB), 0, st, D, ag, dt, make_hd(hdtm), thermostat, tau, T0);
+    return launch_bin(D, st);
+}
+
+hipError_t launch_se_step(const KDesc &D, hipStream_t st, int ag, double dt)
+{
+    hipLaunchKernelGGL(k_se_step, dim3(nblk(D)), dim3(TPB), 0, st, D, ag, dt);
+    hipError_t e = launch_bin(D, st);
+    if (e != hipSuccess) return e;
+    return launch_thermo(D, st, ag);
+}
+
+hipError_t launch_flag_only(const KDesc &D, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_flag_only, dim3(nblk(D)), dim3(TPB), 0, st, D);
+    return launch_bin(D, st);
+}
+
+hipError_t launch_vv_finish(const KDesc &D, hipStream_t st, int ag, const double *hdtm)
+{
+    unsigned g = nblk(D);
+    hipLaunchKernelGGL(k_vv_finish, dim3(g), dim3(TPB), 0, st, D, ag, make_hd(hdtm));
+    hipLaunchKernelGGL(k_reduce_thermo, dim3(1), dim3(1024), 0, st, D, (int)g);
+    return hipGetLastError();
+}
+
+hipError_t launch_thermo(const KDesc &D, hipStream_t st, int ag)
+{
+    unsigned g = nblk(D);
+    hipLaunchKernelGGL(k_thermo, dim3(g), dim3(TPB), 0, st, D, ag);
+    hipLaunchKernelGGL(k_reduce_thermo, dim3(1), dim3(1024), 0, st, D, (int)g);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_pe(const KDesc &D, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reduce_pe, dim3(1), dim3(1024), 0, st, D);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_velocity(const KDesc &D, hipStream_t st, int group, double vx, double vy, double vz)
+{
+    hipLaunchKernelGGL(k_add_velocity, dim3(nblk(D)), dim3(TPB), 0, st, D, group, vx, vy, vz);
+    return hipGetLastError();
+}
+
+hipError_t launch_relayout(const KDesc &A, const KDesc &B, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_relayout, dim3(nblk(A)), dim3(TPB), 0, st, A, B);
+    return hipGetLastError();
+}
+
+/* compact cell-ordered arrays -> slots (transfer_from_globalgrid_to_rank0_grid, matter.c:329-363) */
+__global__ __launch_bounds__(TPB) void k_scatter(KDesc D, const int *cell_start, const double *x, const double *v,
+                                                  const int *grp, const unsigned *aid, const uint8_t *kind)
+{
+    long slot = (long)blockIdx.x * TPB + threadIdx.x;
+    if (slot >= (long)D.ncell * D.cap) return;
+    int c = (int)(slot / D.cap);
+    int k = (int)(slot - (long)c * D.cap);
+    if (k >= D.cnt[c]) return;
+    long a = (long)cell_start[c] + k;
+    D.x[slot] = x[3 * a]; D.y[slot] = x[3 * a + 1]; D.z[slot] = x[3 * a + 2];
+    D.vx[slot] = v[3 * a]; D.vy[slot] = v[3 * a + 1]; D.vz[slot] = v[3 * a + 2];
+    D.fx[slot] = 0.0; D.fy[slot] = 0.0; D.fz[slot] = 0.0; D.aux[slot] = 0.0;
+    D.grp[slot] = grp[a]; D.aid[slot] = aid[a]; D.kind[slot] = kind[a];
+}
+
+/* slots -> compact cell-ordered arrays */
+__global__ __launch_bounds__(TPB) void k_gather(KDesc D, const int *cell_start, double *x, double *v, double *f,
+                                                 double *aux, int *grp, unsigned *aid, uint8_t *kind)
+{
+    long slot = (long)blockIdx.x * TPB + threadIdx.x;
+    if (slot >= (long)D.ncell * D.cap) return;
+    int c = (int)(slot / D.cap);
+    int k = (int)(slot - (long)c * D.cap);
+    if (k >= D.cnt[c]) return;
+    long a = (long)cell_start[c] + k;
+    x[3 * a] = D.x[slot]; x[3 * a + 1] = D.y[slot]; x[3 * a + 2] = D.z[slot];
+    v[3 * a] = D.vx[slot]; v[3 * a + 1] = D.vy[slot]; v[3 * a + 2] = D.vz[slot];
+    f[3 * a] = D.fx[slot]; f[3 * a + 1] = D.fy[slot]; f[3 * a + 2] = D.fz[slot];
+    aux[a] = D.aux[slot];
+    grp[a] = D.grp[slot]; aid[a] = D.aid[slot]; kind[a] = D.kind[slot];
+}
+
+__global__ __launch_bounds__(TPB) void k_fill(double *p, long n, double v)
+{
+    long i = (long)blockIdx.x * TPB + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t launch_scatter(const KDesc &D, hipStream_t st, const int *cell_start, const double *x, const double *v,
+                          const int *grp, const unsigned *aid, const uint8_t *kind)
+{
+    hipLaunchKernelGGL(k_scatter, dim3(nblk(D)), dim3(TPB), 0, st, D, cell_start, x, v, grp, aid, kind);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(const KDesc &D, hipStream_t st, const int *cell_start, double *x, double *v, double *f,
+                         double *aux, int *grp, unsigned *aid, uint8_t *kind)
+{
+    hipLaunchKernelGGL(k_gather, dim3(nblk(D)), dim3(TPB), 0, st, D, cell_start, x, v, f, aux, grp, aid, kind);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(double *p, long n, double v, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0, st, p, n, v);
+    return hipGetLastError();
+}

@@ -1,0 +1,91 @@
+"""GPU parity of the force kernels against the oracle on identical inputs.
+
+Bar: FP64 build, per-atom forces within 1e-10 of max|F| and energies within
+1e-10 relative; mixed build within 1e-5 of max|F|.
+"""
+import numpy as np
+import pytest
+
+import fmd_amd as F
+from tests.pairsys import build, lattice_fcc, rel_err
+
+pytestmark = pytest.mark.gpu
+
+AR = dict(sig=3.405, eps=0.0104, rc=2.5 * 3.405, a=5.26, m=39.948)
+
+
+def _compare(o, d, tol_f, tol_e, ag=-1):
+    o.forces(ag)
+    d.forces(ag)
+    eo = o.energies()
+    got = d.download()
+    ref = o.atoms()
+    assert np.array_equal(got["cell_count"], ref["cell_count"])
+    assert np.array_equal(got["atomid"], ref["atomid"])
+    ef = rel_err(got["f"], ref["f"])
+    assert ef < tol_f, "force error %.3e" % ef
+    t = d.read()
+    assert t.err == 0
+    pe_err = abs(t.pe - eo["pe"]) / max(abs(eo["pe"]), 1e-300)
+    assert pe_err < tol_e, "pe error %.3e (%r vs %r)" % (pe_err, t.pe, eo["pe"])
+    return got, ref
+
+
+def test_lj_argon_4000(rng):
+    x = lattice_fcc((10, 10, 10), AR["a"], rng, jitter=0.15)
+    L = 10 * AR["a"]
+    o, d = build([L] * 3, [1, 1, 1], [AR["m"]], ["Ar"], [("lj", 0, 0, AR["sig"], AR["eps"], AR["rc"])], x)
+    _compare(o, d, 1e-10, 1e-10)
+
+
+def test_morse_cu_mixed_pbc(rng):
+    x = lattice_fcc((8, 8, 9), 3.615, rng, jitter=0.05)
+    L = [8 * 3.615, 8 * 3.615, 9 * 3.615 + 12.0]
+    o, d = build(L, [1, 1, 0], [63.546], ["Cu"], [("morse", 0, 0, 0.3429, 1.3588, 2.866, 7.0)], x)
+    _compare(o, d, 1e-10, 1e-10)
+
+
+def test_eam_cu01_fixture(rng, cu01):
+    x = lattice_fcc((8, 8, 8), 3.615, rng, jitter=0.04)
+    L = [8 * 3.615] * 3
+    o, d = build(L, [1, 1, 1], [63.546], ["Cu"], [("eam", cu01, [(0, 0)])], x)
+    got, ref = _compare(o, d, 1e-10, 1e-10)
+    # pass 1 leaves F'(rho) per atom in aux, as the reference leaves it in vaream
+    assert rel_err(got["aux"], ref["aux"]) < 1e-10
+
+
+def test_eam_synthetic_setfl(rng, synth_setfl):
+    x = lattice_fcc((7, 7, 7), 3.615, rng, jitter=0.02)
+    L = [7 * 3.615] * 3
+    o, d = build(L, [1, 1, 1], [63.546], ["Cu"], [("eam", synth_setfl, [(0, 0)])], x)
+    _compare(o, d, 1e-10, 1e-10)
+
+
+def test_hybrid_eam_lj_two_kinds_with_group(rng, cu01):
+    # Cu (EAM) and Ar (LJ Ar-Ar, LJ Cu-Ar) as in examples/02_collision.c
+    xcu = lattice_fcc((5, 5, 5), 3.6316, rng, jitter=0.03, origin=(2.0, 2.0, 2.0))
+    xar = lattice_fcc((4, 4, 4), 5.26, rng, jitter=0.05, origin=(22.0, 2.0, 2.0))
+    x = np.vstack([xcu, xar])
+    kind = np.r_[np.zeros(len(xcu)), np.ones(len(xar))].astype(np.int32)
+    group = np.r_[np.zeros(len(xcu)), np.ones(len(xar))].astype(np.int32)
+    L = [48.0, 30.0, 30.0]
+    pots = [("eam", cu01, [(0, 0)]), ("lj", 1, 1, 3.40, 0.0104, 2.5 * 3.40), ("lj", 0, 1, 2.87, 0.0652, 2.5 * 2.87)]
+    o, d = build(L, [1, 1, 1], [63.546, 39.948], ["Cu", "Ar"], pots, x, group=group, kind=kind)
+    _compare(o, d, 1e-10, 1e-10)
+    _compare(o, d, 1e-10, 1e-10, ag=0)
+    _compare(o, d, 1e-10, 1e-10, ag=1)
+
+
+def test_mixed_precision_eam(rng, cu01):
+    x = lattice_fcc((8, 8, 8), 3.615, rng, jitter=0.04)
+    L = [8 * 3.615] * 3
+    o, d = build(L, [1, 1, 1], [63.546], ["Cu"], [("eam", cu01, [(0, 0)])], x, precision=F.FMDX_PREC_MIXED)
+    _compare(o, d, 1e-5, 1e-6)
+
+
+def test_mixed_precision_lj(rng):
+    x = lattice_fcc((10, 10, 10), AR["a"], rng, jitter=0.15)
+    L = 10 * AR["a"]
+    o, d = build([L] * 3, [1, 1, 1], [AR["m"]], ["Ar"], [("lj", 0, 0, AR["sig"], AR["eps"], AR["rc"])], x,
+                 precision=F.FMDX_PREC_MIXED)
+    _compare(o, d, 1e-5, 1e-6)
