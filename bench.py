@@ -132,7 +132,7 @@ def gpu_bench(cells, steps, warmup, precision, setfl, seed=1):
     return res
 
 
-def cpu_baseline(setfl, cells=16, steps=3, seed=1):
+def cpu_baseline(setfl, cells=32, steps=20, seed=1):
     """The oracle (FP64 restatement of the reference, one thread) on a bounded sample."""
     from oracle.oracle import Oracle
     a = W.CU["a"]
@@ -226,8 +226,9 @@ def main():
                                   "pass0_ms": other["pass0_ms"], "pass1_ms": other["pass1_ms"]}
     if cpu is not None:
         out["cpu_baseline"] = {"value": cpu["value"], "unit": "atom-steps/s", "cores": 1, "kind": "port",
-                               "sample": "oracle (FP64 reference restatement) on EAM Cu %d atoms (16^3 FCC), "
-                                         "%d Berendsen VV steps" % (cpu["natoms"], cpu["steps"])}
+                               "sample": "oracle (FP64 reference restatement, one thread) on EAM Cu %d atoms "
+                                         "(32^3 FCC, same setfl and lattice recipe), %d Berendsen VV steps, %.1f s"
+                                         % (cpu["natoms"], cpu["steps"], cpu["seconds"])}
     print(json.dumps(out), flush=True)
 
 

@@ -379,6 +379,13 @@ extern "C" int fmdx_upload(fmdx_system_t *s, unsigned natoms, const unsigned *ce
         CK(hipMemcpy(da, aid.data(), sizeof(unsigned) * natoms, hipMemcpyHostToDevice));
         CK(hipMemcpy(dk, k8.data(), natoms, hipMemcpyHostToDevice));
     }
+    /* binning moves atoms into slots that held none; their force words must
+       read zero until the next force evaluation, as for a fresh host cell */
+    const long nslot = (long)D.ncell * D.cap;
+    CK(launch_fill(D.fx, nslot, 0.0, s->st));
+    CK(launch_fill(D.fy, nslot, 0.0, s->st));
+    CK(launch_fill(D.fz, nslot, 0.0, s->st));
+    CK(launch_fill(D.aux, nslot, 0.0, s->st));
     CK(launch_scatter(D, s->st, dstart, dx, dv, dg, da, dk));
     unsigned ctr[8] = {0, 0, maxc, 0, 0, 0, 0, 0};
     CK(hipMemcpyAsync(D.counters, ctr, sizeof ctr, hipMemcpyHostToDevice, s->st));
@@ -471,51 +478,56 @@ extern "C" int fmdx_update_forces(fmdx_system_t *s, int active_group, int ttm_fr
 {
     if (!s || !s->uploaded) return FMDX_ESTATE;
     if (s->D.has_eam && !s->rhoc) return FMDX_ESTATE;
-    const TtmDev T = ttm_dev(s, ttm_friction);
+    s->D.ag = active_group;
+    s->D.ttm = ttm_dev(s, ttm_friction);
     if (s->profile && s->prof_n < NPROF)
     {
         hipEvent_t *e = &s->pev[3 * s->prof_n];
         CK(hipEventRecord(e[0], s->st));
-        CK(launch_pass1(s->D, s->st, active_group, s->precision, T));
+        CK(launch_pass1(s->D, s->st, s->precision));
         CK(hipEventRecord(e[1], s->st));
-        CK(launch_pass0(s->D, s->st, active_group, s->precision, T));
+        CK(launch_pass0(s->D, s->st, s->precision));
         CK(hipEventRecord(e[2], s->st));
         CK(launch_reduce_pe(s->D, s->st));
         s->prof_n++;
     }
     else
-        CK(launch_forces(s->D, s->st, active_group, s->precision, T));
+        CK(launch_forces(s->D, s->st, s->precision));
     return FMDX_OK;
 }
 
-static void half_dt_over_m(const fmdx_system_t *s, double dt, double *h)
+/* md->timestep * 0.5 / mass, integrators.c:67, evaluated on the host with the same operations */
+static void set_step(KDesc &D, const double *mass, int ag, double dt, int thermostat, double tau, double T0)
 {
-    /* md->timestep * 0.5 / mass, integrators.c:67 */
-    for (int k = 0; k < MAXK; k++) h[k] = dt * 0.5 / s->mass[k];
+    D.ag = ag;
+    D.dt = dt;
+    D.thermostat = thermostat;
+    D.tau = tau;
+    D.T0 = T0;
+    for (int k = 0; k < MAXK; k++) D.hdtm[k] = dt * 0.5 / mass[k];
 }
 
 extern "C" int fmdx_vv_start(fmdx_system_t *s, int active_group, double dt, int thermostat, double tau, double T0)
 {
     if (!s || !s->uploaded) return FMDX_ESTATE;
-    double h[MAXK];
-    half_dt_over_m(s, dt, h);
-    CK(launch_vv_start(s->D, s->st, active_group, dt, h, thermostat, tau, T0));
+    set_step(s->D, s->mass, active_group, dt, thermostat, tau, T0);
+    CK(launch_vv_start(s->D, s->st));
     return FMDX_OK;
 }
 
 extern "C" int fmdx_vv_finish(fmdx_system_t *s, int active_group, double dt)
 {
     if (!s || !s->uploaded) return FMDX_ESTATE;
-    double h[MAXK];
-    half_dt_over_m(s, dt, h);
-    CK(launch_vv_finish(s->D, s->st, active_group, h));
+    set_step(s->D, s->mass, active_group, dt, 0, 1.0, 0.0);
+    CK(launch_vv_finish(s->D, s->st));
     return FMDX_OK;
 }
 
 extern "C" int fmdx_se_step(fmdx_system_t *s, int active_group, double dt)
 {
     if (!s || !s->uploaded) return FMDX_ESTATE;
-    CK(launch_se_step(s->D, s->st, active_group, dt));
+    set_step(s->D, s->mass, active_group, dt, 0, 1.0, 0.0);
+    CK(launch_se_step(s->D, s->st));
     return FMDX_OK;
 }
 
@@ -529,7 +541,8 @@ extern "C" int fmdx_refresh(fmdx_system_t *s)
 extern "C" int fmdx_thermo(fmdx_system_t *s, int active_group)
 {
     if (!s || !s->uploaded) return FMDX_ESTATE;
-    CK(launch_thermo(s->D, s->st, active_group));
+    s->D.ag = active_group;
+    CK(launch_thermo(s->D, s->st));
     return FMDX_OK;
 }
 
@@ -596,6 +609,11 @@ extern "C" int fmdx_set_capacity(fmdx_system_t *s, unsigned cap)
     N.mvgrp = nullptr; N.mvaid = nullptr; N.mvkind = nullptr; N.part = nullptr;
     int r = alloc_slots(N, (int)cap, s->natoms_uploaded + 64);
     if (r) return r;
+    const long nslot = (long)N.ncell * N.cap;
+    CK(launch_fill(N.fx, nslot, 0.0, s->st));
+    CK(launch_fill(N.fy, nslot, 0.0, s->st));
+    CK(launch_fill(N.fz, nslot, 0.0, s->st));
+    CK(launch_fill(N.aux, nslot, 0.0, s->st));
     CK(launch_relayout(s->D, N, s->st));
     CK(hipStreamSynchronize(s->st));
     free_slots(s->D);

@@ -98,6 +98,19 @@ struct KDesc
     double *cell_emb;      /* [ncell] embedding energy per home cell */
     double *part;          /* [NPART_MAX*8] per-block thermo partials */
     double *thermo;        /* fmdx_thermo_t mirror, doubles first */
+    /* step parameters, set by the host before a launch */
+    int ag;                /* active group, -1 = every atom */
+    int thermostat;        /* Berendsen rescale in the start step */
+    double dt, tau, T0;
+    double hdtm[MAXK];     /* dt * 0.5 / mass per kind (integrators.c:67) */
+    TtmDev ttm;            /* TTM friction source (forces.c:292-330) */
+};
+
+/* many systems in one launch: block b works on block items[b].y of system items[b].x */
+struct BatchMap
+{
+    const KDesc *descs;
+    const int2 *items;
 };
 
 /* 1-D TTM state resident on the device (arrays of ncz+2 turi cells, ghosts included) */
@@ -126,18 +139,37 @@ static __device__ __forceinline__ void set_err(const KDesc &D, unsigned bits)
     atomicOr(&D.counters[3], bits);
 }
 
-/* kernel launchers (defined in the kernel translation units) */
-hipError_t launch_vv_start(const KDesc &D, hipStream_t st, int ag, double dt, const double *hdtm,
-                           int thermostat, double tau, double T0);
-hipError_t launch_vv_finish(const KDesc &D, hipStream_t st, int ag, const double *hdtm);
-hipError_t launch_se_step(const KDesc &D, hipStream_t st, int ag, double dt);
-hipError_t launch_thermo(const KDesc &D, hipStream_t st, int ag);
+/* work lists of a batch of systems (device arrays) */
+struct BatchPlan
+{
+    const KDesc *descs;      /* [nsys] */
+    int nsys;
+    const int2 *slot_items;  /* (system, block of 256 slots) */
+    int n_slot;
+    const int2 *cellblk_items; /* (system, block of 256 cells) */
+    int n_cellblk;
+    const int2 *cell_items;  /* (system, cell), one wavefront each */
+    int n_cell;
+    int max_nkinds;
+};
+
+/* kernel launchers (defined in the kernel translation units); the step
+   parameters (active group, dt, thermostat, TTM friction) travel in KDesc */
+hipError_t launch_vv_start(const KDesc &D, hipStream_t st);
+hipError_t launch_vv_finish(const KDesc &D, hipStream_t st);
+hipError_t launch_se_step(const KDesc &D, hipStream_t st);
+hipError_t launch_thermo(const KDesc &D, hipStream_t st);
 hipError_t launch_flag_only(const KDesc &D, hipStream_t st);
 hipError_t launch_bin(const KDesc &D, hipStream_t st);
 hipError_t launch_reduce_pe(const KDesc &D, hipStream_t st);
-hipError_t launch_forces(const KDesc &D, hipStream_t st, int ag, int precision, const TtmDev &T);
-hipError_t launch_pass1(const KDesc &D, hipStream_t st, int ag, int precision, const TtmDev &T);
-hipError_t launch_pass0(const KDesc &D, hipStream_t st, int ag, int precision, const TtmDev &T);
+hipError_t launch_forces(const KDesc &D, hipStream_t st, int precision);
+hipError_t launch_pass1(const KDesc &D, hipStream_t st, int precision);
+hipError_t launch_pass0(const KDesc &D, hipStream_t st, int precision);
+hipError_t launchb_vv_start(const BatchPlan &P, hipStream_t st);
+hipError_t launchb_vv_finish(const BatchPlan &P, hipStream_t st);
+hipError_t launchb_thermo(const BatchPlan &P, hipStream_t st);
+hipError_t launchb_reduce_pe(const BatchPlan &P, hipStream_t st);
+hipError_t launchb_forces(const BatchPlan &P, hipStream_t st, int precision);
 hipError_t launch_add_velocity(const KDesc &D, hipStream_t st, int group, double vx, double vy, double vz);
 hipError_t launch_relayout(const KDesc &Dold, const KDesc &Dnew, hipStream_t st);
 hipError_t launch_scatter(const KDesc &D, hipStream_t st, const int *cell_start, const double *x, const double *v,

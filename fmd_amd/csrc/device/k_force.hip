@@ -59,6 +59,13 @@ struct Acc
     double fx, fy, fz, e;
 };
 
+/* per-wave LDS scratch: the hit buffer during the scan, the stripe sums afterwards */
+union ShScratch
+{
+    uint16_t hb[HB][WG];
+    double red[4][WG];
+};
+
 /* Kahan-compensated FP32 accumulator */
 struct KahanF
 {
@@ -138,15 +145,22 @@ static __device__ __forceinline__ void hit64(const KDesc &D, const PairParam *pp
 }
 
 template <int PASS>
-__global__ __launch_bounds__(WG) void k_pairs64(KDesc D, int ag, TtmDev T)
+static __device__ void b_pairs64(const KDesc &D, const int c)
 {
+    const int ag = D.ag;
+    const TtmDev &T = D.ttm;
+    if (PASS == 1 && !D.has_eam)
+    {
+        if (threadIdx.x == 0) D.cell_emb[c] = 0.0;
+        return;
+    }
     __shared__ int nb_base[27], nb_cnt[27];
     __shared__ double nb_sx[27], nb_sy[27], nb_sz[27];
-    __shared__ unsigned hb[HB][WG];
-    __shared__ double red[4][WG];
-    __shared__ PairParam pp[MAXK * MAXK];
+    __shared__ ShScratch sm;
+    extern __shared__ PairParam pp[];   /* nkinds^2 entries, sized at launch */
+    uint16_t(&hb)[HB][WG] = sm.hb;
+    double(&red)[4][WG] = sm.red;
 
-    const int c = blockIdx.x;
     const int lane = threadIdx.x;
     const int nkk = D.nkinds * D.nkinds;
     for (int i = lane; i < nkk; i += WG) pp[i] = D.pairs[i];
@@ -210,7 +224,7 @@ __global__ __launch_bounds__(WG) void k_pairs64(KDesc D, int ag, TtmDev T)
                     hit = r2 < rcut2;
                     if (hit && ag >= 0 && D.grp[js] != ag) hit = false;
                 }
-                if (hit) hb[fill++][lane] = ((unsigned)k << 8) | (unsigned)j;
+                if (hit) hb[fill++][lane] = (uint16_t)((k << 8) | j);
                 if (wave_any(fill == HB))
                 {
                     for (int h = 0; wave_any(h < fill); h++)
@@ -243,6 +257,7 @@ __global__ __launch_bounds__(WG) void k_pairs64(KDesc D, int ag, TtmDev T)
                 hit64<PASS>(D, pp, D.nkinds, ki, Fpi, dx, dy, dz, js, acc);
             }
         }
+        __syncthreads();   /* the hit buffer and the reduction scratch share LDS */
         red[0][lane] = acc.fx; red[1][lane] = acc.fy; red[2][lane] = acc.fz; red[3][lane] = acc.e;
         __syncthreads();
         double ecell = 0.0;
@@ -301,6 +316,17 @@ __global__ __launch_bounds__(WG) void k_pairs64(KDesc D, int ag, TtmDev T)
         if (PASS == 1) D.cell_emb[c] = cell_acc;
         else D.cell_pe[c] = cell_acc;
     }
+}
+
+template <int PASS>
+__global__ __launch_bounds__(WG) void k_pairs64(KDesc D) { b_pairs64<PASS>(D, blockIdx.x); }
+
+template <int PASS>
+__global__ __launch_bounds__(WG) void kb_pairs64(BatchMap M)
+{
+    const int2 it = M.items[blockIdx.x];
+    const KDesc D = M.descs[it.x];
+    b_pairs64<PASS>(D, it.y);
 }
 
 /* ------------------------------------------------------------- mixed */
@@ -372,15 +398,22 @@ static __device__ __forceinline__ void hit32(const KDesc &D, const PairParam *pp
 }
 
 template <int PASS>
-__global__ __launch_bounds__(WG) void k_pairs32(KDesc D, int ag, TtmDev T)
+static __device__ void b_pairs32(const KDesc &D, const int c)
 {
+    const int ag = D.ag;
+    const TtmDev &T = D.ttm;
+    if (PASS == 1 && !D.has_eam)
+    {
+        if (threadIdx.x == 0) D.cell_emb[c] = 0.0;
+        return;
+    }
     __shared__ int nb_base[27], nb_cnt[27];
     __shared__ float nb_ox[27], nb_oy[27], nb_oz[27];   /* neighbour-cell origin minus home origin */
-    __shared__ unsigned hb[HB][WG];
-    __shared__ double red[4][WG];
-    __shared__ PairParam pp[MAXK * MAXK];
+    __shared__ ShScratch sm;
+    extern __shared__ PairParam pp[];   /* nkinds^2 entries, sized at launch */
+    uint16_t(&hb)[HB][WG] = sm.hb;
+    double(&red)[4][WG] = sm.red;
 
-    const int c = blockIdx.x;
     const int lane = threadIdx.x;
     const int nkk = D.nkinds * D.nkinds;
     for (int i = lane; i < nkk; i += WG) pp[i] = D.pairs[i];
@@ -449,7 +482,7 @@ __global__ __launch_bounds__(WG) void k_pairs32(KDesc D, int ag, TtmDev T)
                     hit = r2 < rcut2;
                     if (hit && ag >= 0 && D.grp[js] != ag) hit = false;
                 }
-                if (hit) hb[fill++][lane] = ((unsigned)k << 8) | (unsigned)j;
+                if (hit) hb[fill++][lane] = (uint16_t)((k << 8) | j);
                 if (wave_any(fill == HB))
                 {
                     for (int h = 0; wave_any(h < fill); h++)
@@ -482,6 +515,7 @@ __global__ __launch_bounds__(WG) void k_pairs32(KDesc D, int ag, TtmDev T)
                 hit32<PASS>(D, pp, D.nkinds, ki, Fpi, dx, dy, dz, js, ax, ay, az, ae);
             }
         }
+        __syncthreads();   /* the hit buffer and the reduction scratch share LDS */
         red[0][lane] = (double)ax.s - (double)ax.c;
         red[1][lane] = (double)ay.s - (double)ay.c;
         red[2][lane] = (double)az.s - (double)az.c;
@@ -547,10 +581,21 @@ __global__ __launch_bounds__(WG) void k_pairs32(KDesc D, int ag, TtmDev T)
     }
 }
 
-/* FP32 positions relative to the origin of the home cell, from the FP64 positions */
-__global__ __launch_bounds__(256) void k_relpos(KDesc D)
+template <int PASS>
+__global__ __launch_bounds__(WG) void k_pairs32(KDesc D) { b_pairs32<PASS>(D, blockIdx.x); }
+
+template <int PASS>
+__global__ __launch_bounds__(WG) void kb_pairs32(BatchMap M)
 {
-    long slot = (long)blockIdx.x * 256 + threadIdx.x;
+    const int2 it = M.items[blockIdx.x];
+    const KDesc D = M.descs[it.x];
+    b_pairs32<PASS>(D, it.y);
+}
+
+/* FP32 positions relative to the origin of the home cell, from the FP64 positions */
+static __device__ void b_relpos(const KDesc &D, int blk)
+{
+    long slot = (long)blk * 256 + threadIdx.x;
     if (slot >= (long)D.ncell * D.cap) return;
     int c = (int)(slot / D.cap);
     int k = (int)(slot - (long)c * D.cap);
@@ -560,38 +605,68 @@ __global__ __launch_bounds__(256) void k_relpos(KDesc D)
     D.yr[slot] = (float)(D.y[slot] - cy * D.h1);
     D.zr[slot] = (float)(D.z[slot] - cz * D.h2);
 }
+__global__ __launch_bounds__(256) void k_relpos(KDesc D) { b_relpos(D, blockIdx.x); }
+__global__ __launch_bounds__(256) void kb_relpos(BatchMap M)
+{
+    const int2 it = M.items[blockIdx.x];
+    const KDesc D = M.descs[it.x];
+    b_relpos(D, it.y);
+}
+
+static inline size_t pp_bytes(const KDesc &D) { return sizeof(PairParam) * D.nkinds * D.nkinds; }
 
 /* pass 1 (EAM density + F'(rho)) and pass 0 (forces + energy), launched separately so that
    callers can time each kernel with events on the stream */
-hipError_t launch_pass1(const KDesc &D, hipStream_t st, int ag, int precision, const TtmDev &T)
+hipError_t launch_pass1(const KDesc &D, hipStream_t st, int precision)
 {
     if (precision == FMDX_PREC_MIXED)
     {
         unsigned g = (unsigned)(((long)D.ncell * D.cap + 255) / 256);
         hipLaunchKernelGGL(k_relpos, dim3(g), dim3(256), 0, st, D);
-        if (D.has_eam) hipLaunchKernelGGL(k_pairs32<1>, dim3(D.ncell), dim3(WG), 0, st, D, ag, T);
+        if (D.has_eam) hipLaunchKernelGGL(k_pairs32<1>, dim3(D.ncell), dim3(WG), pp_bytes(D), st, D);
         else launch_fill(D.cell_emb, D.ncell, 0.0, st);
     }
     else
     {
-        if (D.has_eam) hipLaunchKernelGGL(k_pairs64<1>, dim3(D.ncell), dim3(WG), 0, st, D, ag, T);
+        if (D.has_eam) hipLaunchKernelGGL(k_pairs64<1>, dim3(D.ncell), dim3(WG), pp_bytes(D), st, D);
         else launch_fill(D.cell_emb, D.ncell, 0.0, st);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_pass0(const KDesc &D, hipStream_t st, int ag, int precision, const TtmDev &T)
+hipError_t launch_pass0(const KDesc &D, hipStream_t st, int precision)
 {
-    if (precision == FMDX_PREC_MIXED) hipLaunchKernelGGL(k_pairs32<0>, dim3(D.ncell), dim3(WG), 0, st, D, ag, T);
-    else hipLaunchKernelGGL(k_pairs64<0>, dim3(D.ncell), dim3(WG), 0, st, D, ag, T);
+    if (precision == FMDX_PREC_MIXED) hipLaunchKernelGGL(k_pairs32<0>, dim3(D.ncell), dim3(WG), pp_bytes(D), st, D);
+    else hipLaunchKernelGGL(k_pairs64<0>, dim3(D.ncell), dim3(WG), pp_bytes(D), st, D);
     return hipGetLastError();
 }
 
-hipError_t launch_forces(const KDesc &D, hipStream_t st, int ag, int precision, const TtmDev &T)
+hipError_t launch_forces(const KDesc &D, hipStream_t st, int precision)
 {
-    hipError_t e = launch_pass1(D, st, ag, precision, T);
+    hipError_t e = launch_pass1(D, st, precision);
     if (e != hipSuccess) return e;
-    e = launch_pass0(D, st, ag, precision, T);
+    e = launch_pass0(D, st, precision);
     if (e != hipSuccess) return e;
     return launch_reduce_pe(D, st);
+}
+
+/* every system of a batch: relative positions (mixed), pass 1, pass 0, energies */
+hipError_t launchb_forces(const BatchPlan &P, hipStream_t st, int precision)
+{
+    BatchMap ms = {P.descs, P.slot_items}, mc = {P.descs, P.cell_items};
+    const size_t ppb = sizeof(PairParam) * P.max_nkinds * P.max_nkinds;
+    if (precision == FMDX_PREC_MIXED)
+    {
+        hipLaunchKernelGGL(kb_relpos, dim3(P.n_slot), dim3(256), 0, st, ms);
+        hipLaunchKernelGGL(kb_pairs32<1>, dim3(P.n_cell), dim3(WG), ppb, st, mc);
+        hipLaunchKernelGGL(kb_pairs32<0>, dim3(P.n_cell), dim3(WG), ppb, st, mc);
+    }
+    else
+    {
+        hipLaunchKernelGGL(kb_pairs64<1>, dim3(P.n_cell), dim3(WG), ppb, st, mc);
+        hipLaunchKernelGGL(kb_pairs64<0>, dim3(P.n_cell), dim3(WG), ppb, st, mc);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launchb_reduce_pe(P, st);
 }

@@ -22,15 +22,23 @@
  * The order key encodes the migration history: an atom that crossed the box
  * in dimension d was appended during phase d of the migration (lower face
  * before upper face), after every atom that reached the same cell earlier.
+ *
+ * Every kernel body takes (descriptor, logical block).  k_* wrappers run one
+ * system; kb_* wrappers run many systems in one launch from a (system,
+ * block) work list.
  */
 #include "fmdx_internal.h"
 
 #define TPB 256
 
-struct Hd
-{
-    double v[MAXK];
-};
+#define DEF_KERNELS(NAME)                                                             \
+    __global__ __launch_bounds__(TPB) void k_##NAME(KDesc D) { b_##NAME(D, blockIdx.x); } \
+    __global__ __launch_bounds__(TPB) void kb_##NAME(BatchMap M)                       \
+    {                                                                                  \
+        const int2 it = M.items[blockIdx.x];                                           \
+        const KDesc D = M.descs[it.x];                                                 \
+        b_##NAME(D, it.y);                                                             \
+    }
 
 static __device__ __forceinline__ void cell_coords(const KDesc &D, int c, int &cx, int &cy, int &cz)
 {
@@ -41,7 +49,7 @@ static __device__ __forceinline__ void cell_coords(const KDesc &D, int c, int &c
 }
 
 /* destination of one atom after it moved; misc.c:281-300 and md-ghost.c:827-869 */
-static __device__ void bin_flag(const KDesc &D, int c, int slot, double x, double y, double z)
+static __device__ void bin_flag(const KDesc &D, int c, long slot, double x, double y, double z)
 {
 #pragma clang fp contract(off)
     int cx, cy, cz;
@@ -95,29 +103,37 @@ static __device__ void bin_flag(const KDesc &D, int c, int slot, double x, doubl
     if (code == 0 && c < dest) atomicAdd(&D.nearly[dest], 1);
 }
 
+/* slot of thread threadIdx.x in logical block blk; false when it holds no atom */
+static __device__ __forceinline__ bool slot_of(const KDesc &D, int blk, long &slot, int &c)
+{
+    slot = (long)blk * TPB + threadIdx.x;
+    if (slot >= (long)D.ncell * D.cap) return false;
+    c = (int)(slot / D.cap);
+    int k = (int)(slot - (long)c * D.cap);
+    return k < D.cnt[c];
+}
+
 /* VelocityVerlet_startStep (integrators.c:40-75) fused with the binning flag */
-__global__ __launch_bounds__(TPB) void k_vv_start(KDesc D, int ag, double dt, Hd hd, int thermostat,
-                                                   double tau, double T0)
+static __device__ void b_vv_start(const KDesc &D, int blk)
 {
 #pragma clang fp contract(off)
-    if (blockIdx.x == 0 && threadIdx.x == 0) D.counters[0] = 0;
-    long slot = (long)blockIdx.x * TPB + threadIdx.x;
-    if (slot >= (long)D.ncell * D.cap) return;
-    int c = (int)(slot / D.cap);
-    int k = (int)(slot - (long)c * D.cap);
-    if (k >= D.cnt[c]) return;
+    if (blk == 0 && threadIdx.x == 0) D.counters[0] = 0;
+    long slot;
+    int c;
+    if (!slot_of(D, blk, slot, c)) return;
     double x = D.x[slot], y = D.y[slot], z = D.z[slot];
-    if (ag < 0 || D.grp[slot] == ag)
+    if (D.ag < 0 || D.grp[slot] == D.ag)
     {
+        const double dt = D.dt;
         double lambda = 1.0;
-        if (thermostat)
+        if (D.thermostat)
         {
             double T = D.thermo[TH_T];
-            lambda = sqrt(1 + dt / tau * (T0 / T - 1));
+            lambda = sqrt(1 + dt / D.tau * (D.T0 / T - 1));
         }
-        double h = hd.v[D.kind[slot]];
+        double h = D.hdtm[D.kind[slot]];
         double vx = D.vx[slot], vy = D.vy[slot], vz = D.vz[slot];
-        if (thermostat)
+        if (D.thermostat)
         {
             vx *= lambda;
             vy *= lambda;
@@ -132,22 +148,22 @@ __global__ __launch_bounds__(TPB) void k_vv_start(KDesc D, int ag, double dt, Hd
         D.vx[slot] = vx; D.vy[slot] = vy; D.vz[slot] = vz;
         D.x[slot] = x; D.y[slot] = y; D.z[slot] = z;
     }
-    bin_flag(D, c, (int)slot, x, y, z);
+    bin_flag(D, c, slot, x, y, z);
 }
+DEF_KERNELS(vv_start)
 
 /* SymplecticEuler_takeOneStep, first half (integrators.c:134-147) + binning flag */
-__global__ __launch_bounds__(TPB) void k_se_step(KDesc D, int ag, double dt)
+static __device__ void b_se_step(const KDesc &D, int blk)
 {
 #pragma clang fp contract(off)
-    if (blockIdx.x == 0 && threadIdx.x == 0) D.counters[0] = 0;
-    long slot = (long)blockIdx.x * TPB + threadIdx.x;
-    if (slot >= (long)D.ncell * D.cap) return;
-    int c = (int)(slot / D.cap);
-    int k = (int)(slot - (long)c * D.cap);
-    if (k >= D.cnt[c]) return;
+    if (blk == 0 && threadIdx.x == 0) D.counters[0] = 0;
+    long slot;
+    int c;
+    if (!slot_of(D, blk, slot, c)) return;
     double x = D.x[slot], y = D.y[slot], z = D.z[slot];
-    if (ag < 0 || D.grp[slot] == ag)
+    if (D.ag < 0 || D.grp[slot] == D.ag)
     {
+        const double dt = D.dt;
         double m = D.mass[D.kind[slot]];
         double vx = D.vx[slot], vy = D.vy[slot], vz = D.vz[slot];
         vx += dt * D.fx[slot] / m;
@@ -159,97 +175,93 @@ __global__ __launch_bounds__(TPB) void k_se_step(KDesc D, int ag, double dt)
         D.vx[slot] = vx; D.vy[slot] = vy; D.vz[slot] = vz;
         D.x[slot] = x; D.y[slot] = y; D.z[slot] = z;
     }
-    bin_flag(D, c, (int)slot, x, y, z);
+    bin_flag(D, c, slot, x, y, z);
 }
+DEF_KERNELS(se_step)
 
 /* binning flag without motion (_fmd_refreshGrid called on its own) */
-__global__ __launch_bounds__(TPB) void k_flag_only(KDesc D)
+static __device__ void b_flag_only(const KDesc &D, int blk)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) D.counters[0] = 0;
-    long slot = (long)blockIdx.x * TPB + threadIdx.x;
-    if (slot >= (long)D.ncell * D.cap) return;
-    int c = (int)(slot / D.cap);
-    int k = (int)(slot - (long)c * D.cap);
-    if (k >= D.cnt[c]) return;
-    bin_flag(D, c, (int)slot, D.x[slot], D.y[slot], D.z[slot]);
+    if (blk == 0 && threadIdx.x == 0) D.counters[0] = 0;
+    long slot;
+    int c;
+    if (!slot_of(D, blk, slot, c)) return;
+    bin_flag(D, c, slot, D.x[slot], D.y[slot], D.z[slot]);
 }
+DEF_KERNELS(flag_only)
 
-/* per-block deterministic tree sum of NQ values into part[blockIdx.x*8 + q] */
-template <int NQ>
-static __device__ void block_partials(const KDesc &D, double (&q)[NQ])
+/* per-block deterministic tree sum of 5 values into part[blk*8 + q] */
+static __device__ void block_partials(const KDesc &D, int blk, double (&q)[5])
 {
-    __shared__ double sh[NQ][TPB];
-    for (int i = 0; i < NQ; i++) sh[i][threadIdx.x] = q[i];
+    __shared__ double sh[5][TPB];
+    for (int i = 0; i < 5; i++) sh[i][threadIdx.x] = q[i];
     __syncthreads();
     for (int s = TPB / 2; s > 0; s >>= 1)
     {
         if (threadIdx.x < s)
-            for (int i = 0; i < NQ; i++) sh[i][threadIdx.x] += sh[i][threadIdx.x + s];
+            for (int i = 0; i < 5; i++) sh[i][threadIdx.x] += sh[i][threadIdx.x + s];
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int i = 0; i < NQ; i++) D.part[(long)blockIdx.x * 8 + i] = sh[i][0];
+        for (int i = 0; i < 5; i++) D.part[(long)blk * 8 + i] = sh[i][0];
 }
 
 /* VelocityVerlet_finishStep, integrators.c:94-126: second kick + group sums */
-__global__ __launch_bounds__(TPB) void k_vv_finish(KDesc D, int ag, Hd hd)
+static __device__ void b_vv_finish(const KDesc &D, int blk)
 {
 #pragma clang fp contract(off)
-    long slot = (long)blockIdx.x * TPB + threadIdx.x;
     double q[5] = {0, 0, 0, 0, 0};
-    if (slot < (long)D.ncell * D.cap)
+    long slot;
+    int c;
+    if (slot_of(D, blk, slot, c) && (D.ag < 0 || D.grp[slot] == D.ag))
     {
-        int c = (int)(slot / D.cap);
-        int k = (int)(slot - (long)c * D.cap);
-        if (k < D.cnt[c] && (ag < 0 || D.grp[slot] == ag))
-        {
-            int kd = D.kind[slot];
-            double h = hd.v[kd], m = D.mass[kd];
-            double vx = D.vx[slot] + h * D.fx[slot];
-            double vy = D.vy[slot] + h * D.fy[slot];
-            double vz = D.vz[slot] + h * D.fz[slot];
-            D.vx[slot] = vx; D.vy[slot] = vy; D.vz[slot] = vz;
-            q[0] = m * (vx * vx + vy * vy + vz * vz);
-            q[1] = m * vx;
-            q[2] = m * vy;
-            q[3] = m * vz;
-            q[4] = 1.0;
-        }
+        int kd = D.kind[slot];
+        double h = D.hdtm[kd], m = D.mass[kd];
+        double vx = D.vx[slot] + h * D.fx[slot];
+        double vy = D.vy[slot] + h * D.fy[slot];
+        double vz = D.vz[slot] + h * D.fz[slot];
+        D.vx[slot] = vx; D.vy[slot] = vy; D.vz[slot] = vz;
+        q[0] = m * (vx * vx + vy * vy + vz * vz);
+        q[1] = m * vx;
+        q[2] = m * vy;
+        q[3] = m * vz;
+        q[4] = 1.0;
     }
-    block_partials<5>(D, q);
+    block_partials(D, blk, q);
 }
+DEF_KERNELS(vv_finish)
 
 /* group sums without a kick: matter.c:31-68 and integrators.c:154-176 */
-__global__ __launch_bounds__(TPB) void k_thermo(KDesc D, int ag)
+static __device__ void b_thermo(const KDesc &D, int blk)
 {
 #pragma clang fp contract(off)
-    long slot = (long)blockIdx.x * TPB + threadIdx.x;
     double q[5] = {0, 0, 0, 0, 0};
-    if (slot < (long)D.ncell * D.cap)
+    long slot;
+    int c;
+    if (slot_of(D, blk, slot, c) && (D.ag < 0 || D.grp[slot] == D.ag))
     {
-        int c = (int)(slot / D.cap);
-        int k = (int)(slot - (long)c * D.cap);
-        if (k < D.cnt[c] && (ag < 0 || D.grp[slot] == ag))
-        {
-            double m = D.mass[D.kind[slot]];
-            double vx = D.vx[slot], vy = D.vy[slot], vz = D.vz[slot];
-            q[0] = m * (vx * vx + vy * vy + vz * vz);
-            q[1] = m * vx;
-            q[2] = m * vy;
-            q[3] = m * vz;
-            q[4] = 1.0;
-        }
+        double m = D.mass[D.kind[slot]];
+        double vx = D.vx[slot], vy = D.vy[slot], vz = D.vz[slot];
+        q[0] = m * (vx * vx + vy * vy + vz * vz);
+        q[1] = m * vx;
+        q[2] = m * vy;
+        q[3] = m * vz;
+        q[4] = 1.0;
     }
-    block_partials<5>(D, q);
+    block_partials(D, blk, q);
 }
+DEF_KERNELS(thermo)
+
+static __device__ __forceinline__ int nblk(const KDesc &D) { return (int)(((long)D.ncell * D.cap + TPB - 1) / TPB); }
 
 /* fixed-order reduction of the per-block partials: updateGroupProperties, integrators.c:78-92 */
-__global__ __launch_bounds__(1024) void k_reduce_thermo(KDesc D, int nblocks)
+static __device__ void b_reduce_thermo(const KDesc &D)
 {
 #pragma clang fp contract(off)
     __shared__ double sh[5][1024];
+    const int nb = nblk(D);
     double q[5] = {0, 0, 0, 0, 0};
-    for (int b = threadIdx.x; b < nblocks; b += 1024)
+    for (int b = threadIdx.x; b < nb; b += 1024)
         for (int i = 0; i < 5; i++) q[i] += D.part[(long)b * 8 + i];
     for (int i = 0; i < 5; i++) sh[i][threadIdx.x] = q[i];
     __syncthreads();
@@ -272,9 +284,15 @@ __global__ __launch_bounds__(1024) void k_reduce_thermo(KDesc D, int nblocks)
         D.thermo[TH_N] = n;
     }
 }
+__global__ __launch_bounds__(1024) void k_reduce_thermo(KDesc D) { b_reduce_thermo(D); }
+__global__ __launch_bounds__(1024) void kb_reduce_thermo(const KDesc *descs)
+{
+    const KDesc D = descs[blockIdx.x];
+    b_reduce_thermo(D);
+}
 
 /* fixed-order sum of the per-cell pair and embedding energies */
-__global__ __launch_bounds__(1024) void k_reduce_pe(KDesc D)
+static __device__ void b_reduce_pe(const KDesc &D)
 {
     __shared__ double sh[2][1024];
     double pe = 0, emb = 0;
@@ -301,6 +319,12 @@ __global__ __launch_bounds__(1024) void k_reduce_pe(KDesc D)
         D.thermo[TH_EMB] = sh[1][0];
     }
 }
+__global__ __launch_bounds__(1024) void k_reduce_pe(KDesc D) { b_reduce_pe(D); }
+__global__ __launch_bounds__(1024) void kb_reduce_pe(const KDesc *descs)
+{
+    const KDesc D = descs[blockIdx.x];
+    b_reduce_pe(D);
+}
 
 /* ----------------------------------------------------------- binning */
 
@@ -319,10 +343,10 @@ static __device__ __forceinline__ void copy_staged(const KDesc &D, int mid, long
 }
 
 /* replay of the swap-remove scan (misc.c:279-303) to rank and stage leavers */
-__global__ __launch_bounds__(TPB) void k_bin_rank(KDesc D)
+static __device__ void b_bin_rank(const KDesc &D, int blk)
 {
 #pragma clang fp contract(off)
-    int c = blockIdx.x * TPB + threadIdx.x;
+    int c = blk * TPB + threadIdx.x;
     if (c >= D.ncell) return;
     if (D.nleave[c] == 0) return;
     const int nc = D.cnt[c];
@@ -373,11 +397,12 @@ __global__ __launch_bounds__(TPB) void k_bin_rank(KDesc D)
         }
     }
 }
+DEF_KERNELS(bin_rank)
 
 /* place stayers, early and late arrivals of every affected cell */
-__global__ __launch_bounds__(TPB) void k_bin_apply(KDesc D)
+static __device__ void b_bin_apply(const KDesc &D, int blk)
 {
-    int c = blockIdx.x * TPB + threadIdx.x;
+    int c = blk * TPB + threadIdx.x;
     if (c >= D.ncell) return;
     const int nl = D.nleave[c], na = D.narr[c];
     if (nl == 0 && na == 0) return;
@@ -439,15 +464,14 @@ __global__ __launch_bounds__(TPB) void k_bin_apply(KDesc D)
     D.nearly[c] = 0;
     D.fill[c] = 0;
 }
+DEF_KERNELS(bin_apply)
 
 __global__ __launch_bounds__(TPB) void k_add_velocity(KDesc D, int group, double vx, double vy, double vz)
 {
 #pragma clang fp contract(off)
-    long slot = (long)blockIdx.x * TPB + threadIdx.x;
-    if (slot >= (long)D.ncell * D.cap) return;
-    int c = (int)(slot / D.cap);
-    int k = (int)(slot - (long)c * D.cap);
-    if (k >= D.cnt[c]) return;
+    long slot;
+    int c;
+    if (!slot_of(D, blockIdx.x, slot, c)) return;
     if (group == -1 || D.grp[slot] == group)
     {
         D.vx[slot] += vx;
@@ -459,11 +483,11 @@ __global__ __launch_bounds__(TPB) void k_add_velocity(KDesc D, int group, double
 /* copy every cell into a layout with another slot capacity */
 __global__ __launch_bounds__(TPB) void k_relayout(KDesc A, KDesc B)
 {
-    long slot = (long)blockIdx.x * TPB + threadIdx.x;
-    if (slot >= (long)A.ncell * A.cap) return;
-    int c = (int)(slot / A.cap);
+    long slot;
+    int c;
+    if (!slot_of(A, blockIdx.x, slot, c)) return;
     int k = (int)(slot - (long)c * A.cap);
-    if (k >= A.cnt[c] || k >= B.cap) return;
+    if (k >= B.cap) return;
     long t = (long)c * B.cap + k;
     B.x[t] = A.x[slot]; B.y[t] = A.y[slot]; B.z[t] = A.z[slot];
     B.vx[t] = A.vx[slot]; B.vy[t] = A.vy[slot]; B.vz[t] = A.vz[slot];
@@ -472,90 +496,14 @@ __global__ __launch_bounds__(TPB) void k_relayout(KDesc A, KDesc B)
     B.grp[t] = A.grp[slot]; B.aid[t] = A.aid[slot]; B.kind[t] = A.kind[slot];
 }
 
-/* ------------------------------------------------------------ launchers */
-
-static inline unsigned nblk(const KDesc &D) { return (unsigned)(((long)D.ncell * D.cap + TPB - 1) / TPB); }
-
-static Hd make_hd(const double *hdtm)
-{
-    Hd h;
-    for (int i = 0; i < MAXK; i++) h.v[i] = hdtm ? hdtm[i] : 0.0;
-    return h;
-}
-
-hipError_t launch_bin(const KDesc &D, hipStream_t st)
-{
-    unsigned g = (D.ncell + TPB - 1) / TPB;
-    hipLaunchKernelGGL(k_bin_rank, dim3(g), dim3(TPB), 0, st, D);
-    hipLaunchKernelGGL(k_bin_apply, dim3(g), dim3(TPB), 0, st, D);
-    return hipGetLastError();
-}
-
-hipError_t launch_vv_start(const KDesc &D, hipStream_t st, int ag, double dt, const double *hdtm,
-                           int thermostat, double tau, double T0)
-{
-    hipLaunchKernelGGL(k_vv_start, dim3(nblk(D)), dim3(TPB), 0, st, D, ag, dt, make_hd(hdtm), thermostat, tau, T0);
-    return launch_bin(D, st);
-}
-
-hipError_t launch_se_step(const KDesc &D, hipStream_t st, int ag, double dt)
-{
-    hipLaunchKernelGGL(k_se_step, dim3(nblk(D)), dim3(TPB), 0, st, D, ag, dt);
-    hipError_t e = launch_bin(D, st);
-    if (e != hipSuccess) return e;
-    return launch_thermo(D, st, ag);
-}
-
-hipError_t launch_flag_only(const KDesc &D, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_flag_only, dim3(nblk(D)), dim3(TPB), 0, st, D);
-    return launch_bin(D, st);
-}
-
-hipError_t launch_vv_finish(const KDesc &D, hipStream_t st, int ag, const double *hdtm)
-{
-    unsigned g = nblk(D);
-    hipLaunchKernelGGL(k_vv_finish, dim3(g), dim3(TPB), 0, st, D, ag, make_hd(hdtm));
-    hipLaunchKernelGGL(k_reduce_thermo, dim3(1), dim3(1024), 0, st, D, (int)g);
-    return hipGetLastError();
-}
-
-hipError_t launch_thermo(const KDesc &D, hipStream_t st, int ag)
-{
-    unsigned g = nblk(D);
-    hipLaunchKernelGGL(k_thermo, dim3(g), dim3(TPB), 0, st, D, ag);
-    hipLaunchKernelGGL(k_reduce_thermo, dim3(1), dim3(1024), 0, st, D, (int)g);
-    return hipGetLastError();
-}
-
-hipError_t launch_reduce_pe(const KDesc &D, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_reduce_pe, dim3(1), dim3(1024), 0, st, D);
-    return hipGetLastError();
-}
-
-hipError_t launch_add_velocity(const KDesc &D, hipStream_t st, int group, double vx, double vy, double vz)
-{
-    hipLaunchKernelGGL(k_add_velocity, dim3(nblk(D)), dim3(TPB), 0, st, D, group, vx, vy, vz);
-    return hipGetLastError();
-}
-
-hipError_t launch_relayout(const KDesc &A, const KDesc &B, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_relayout, dim3(nblk(A)), dim3(TPB), 0, st, A, B);
-    return hipGetLastError();
-}
-
 /* compact cell-ordered arrays -> slots (transfer_from_globalgrid_to_rank0_grid, matter.c:329-363) */
 __global__ __launch_bounds__(TPB) void k_scatter(KDesc D, const int *cell_start, const double *x, const double *v,
                                                   const int *grp, const unsigned *aid, const uint8_t *kind)
 {
-    long slot = (long)blockIdx.x * TPB + threadIdx.x;
-    if (slot >= (long)D.ncell * D.cap) return;
-    int c = (int)(slot / D.cap);
-    int k = (int)(slot - (long)c * D.cap);
-    if (k >= D.cnt[c]) return;
-    long a = (long)cell_start[c] + k;
+    long slot;
+    int c;
+    if (!slot_of(D, blockIdx.x, slot, c)) return;
+    long a = (long)cell_start[c] + (slot - (long)c * D.cap);
     D.x[slot] = x[3 * a]; D.y[slot] = x[3 * a + 1]; D.z[slot] = x[3 * a + 2];
     D.vx[slot] = v[3 * a]; D.vy[slot] = v[3 * a + 1]; D.vz[slot] = v[3 * a + 2];
     D.fx[slot] = 0.0; D.fy[slot] = 0.0; D.fz[slot] = 0.0; D.aux[slot] = 0.0;
@@ -566,12 +514,10 @@ __global__ __launch_bounds__(TPB) void k_scatter(KDesc D, const int *cell_start,
 __global__ __launch_bounds__(TPB) void k_gather(KDesc D, const int *cell_start, double *x, double *v, double *f,
                                                  double *aux, int *grp, unsigned *aid, uint8_t *kind)
 {
-    long slot = (long)blockIdx.x * TPB + threadIdx.x;
-    if (slot >= (long)D.ncell * D.cap) return;
-    int c = (int)(slot / D.cap);
-    int k = (int)(slot - (long)c * D.cap);
-    if (k >= D.cnt[c]) return;
-    long a = (long)cell_start[c] + k;
+    long slot;
+    int c;
+    if (!slot_of(D, blockIdx.x, slot, c)) return;
+    long a = (long)cell_start[c] + (slot - (long)c * D.cap);
     x[3 * a] = D.x[slot]; x[3 * a + 1] = D.y[slot]; x[3 * a + 2] = D.z[slot];
     v[3 * a] = D.vx[slot]; v[3 * a + 1] = D.vy[slot]; v[3 * a + 2] = D.vz[slot];
     f[3 * a] = D.fx[slot]; f[3 * a + 1] = D.fy[slot]; f[3 * a + 2] = D.fz[slot];
@@ -585,17 +531,81 @@ __global__ __launch_bounds__(TPB) void k_fill(double *p, long n, double v)
     if (i < n) p[i] = v;
 }
 
+/* ------------------------------------------------------------ launchers */
+
+static inline unsigned nblk_h(const KDesc &D) { return (unsigned)(((long)D.ncell * D.cap + TPB - 1) / TPB); }
+static inline unsigned ncblk_h(const KDesc &D) { return (unsigned)((D.ncell + TPB - 1) / TPB); }
+
+hipError_t launch_bin(const KDesc &D, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_bin_rank, dim3(ncblk_h(D)), dim3(TPB), 0, st, D);
+    hipLaunchKernelGGL(k_bin_apply, dim3(ncblk_h(D)), dim3(TPB), 0, st, D);
+    return hipGetLastError();
+}
+
+hipError_t launch_vv_start(const KDesc &D, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_vv_start, dim3(nblk_h(D)), dim3(TPB), 0, st, D);
+    return launch_bin(D, st);
+}
+
+hipError_t launch_se_step(const KDesc &D, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_se_step, dim3(nblk_h(D)), dim3(TPB), 0, st, D);
+    hipError_t e = launch_bin(D, st);
+    if (e != hipSuccess) return e;
+    return launch_thermo(D, st);
+}
+
+hipError_t launch_flag_only(const KDesc &D, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_flag_only, dim3(nblk_h(D)), dim3(TPB), 0, st, D);
+    return launch_bin(D, st);
+}
+
+hipError_t launch_vv_finish(const KDesc &D, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_vv_finish, dim3(nblk_h(D)), dim3(TPB), 0, st, D);
+    hipLaunchKernelGGL(k_reduce_thermo, dim3(1), dim3(1024), 0, st, D);
+    return hipGetLastError();
+}
+
+hipError_t launch_thermo(const KDesc &D, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_thermo, dim3(nblk_h(D)), dim3(TPB), 0, st, D);
+    hipLaunchKernelGGL(k_reduce_thermo, dim3(1), dim3(1024), 0, st, D);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_pe(const KDesc &D, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reduce_pe, dim3(1), dim3(1024), 0, st, D);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_velocity(const KDesc &D, hipStream_t st, int group, double vx, double vy, double vz)
+{
+    hipLaunchKernelGGL(k_add_velocity, dim3(nblk_h(D)), dim3(TPB), 0, st, D, group, vx, vy, vz);
+    return hipGetLastError();
+}
+
+hipError_t launch_relayout(const KDesc &A, const KDesc &B, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_relayout, dim3(nblk_h(A)), dim3(TPB), 0, st, A, B);
+    return hipGetLastError();
+}
+
 hipError_t launch_scatter(const KDesc &D, hipStream_t st, const int *cell_start, const double *x, const double *v,
                           const int *grp, const unsigned *aid, const uint8_t *kind)
 {
-    hipLaunchKernelGGL(k_scatter, dim3(nblk(D)), dim3(TPB), 0, st, D, cell_start, x, v, grp, aid, kind);
+    hipLaunchKernelGGL(k_scatter, dim3(nblk_h(D)), dim3(TPB), 0, st, D, cell_start, x, v, grp, aid, kind);
     return hipGetLastError();
 }
 
 hipError_t launch_gather(const KDesc &D, hipStream_t st, const int *cell_start, double *x, double *v, double *f,
                          double *aux, int *grp, unsigned *aid, uint8_t *kind)
 {
-    hipLaunchKernelGGL(k_gather, dim3(nblk(D)), dim3(TPB), 0, st, D, cell_start, x, v, f, aux, grp, aid, kind);
+    hipLaunchKernelGGL(k_gather, dim3(nblk_h(D)), dim3(TPB), 0, st, D, cell_start, x, v, f, aux, grp, aid, kind);
     return hipGetLastError();
 }
 
@@ -603,5 +613,38 @@ hipError_t launch_fill(double *p, long n, double v, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+/* ------------------------------------------------------ batched launchers */
+
+hipError_t launchb_vv_start(const BatchPlan &P, hipStream_t st)
+{
+    BatchMap ms = {P.descs, P.slot_items}, mc = {P.descs, P.cellblk_items};
+    hipLaunchKernelGGL(kb_vv_start, dim3(P.n_slot), dim3(TPB), 0, st, ms);
+    hipLaunchKernelGGL(kb_bin_rank, dim3(P.n_cellblk), dim3(TPB), 0, st, mc);
+    hipLaunchKernelGGL(kb_bin_apply, dim3(P.n_cellblk), dim3(TPB), 0, st, mc);
+    return hipGetLastError();
+}
+
+hipError_t launchb_vv_finish(const BatchPlan &P, hipStream_t st)
+{
+    BatchMap ms = {P.descs, P.slot_items};
+    hipLaunchKernelGGL(kb_vv_finish, dim3(P.n_slot), dim3(TPB), 0, st, ms);
+    hipLaunchKernelGGL(kb_reduce_thermo, dim3(P.nsys), dim3(1024), 0, st, P.descs);
+    return hipGetLastError();
+}
+
+hipError_t launchb_thermo(const BatchPlan &P, hipStream_t st)
+{
+    BatchMap ms = {P.descs, P.slot_items};
+    hipLaunchKernelGGL(kb_thermo, dim3(P.n_slot), dim3(TPB), 0, st, ms);
+    hipLaunchKernelGGL(kb_reduce_thermo, dim3(P.nsys), dim3(1024), 0, st, P.descs);
+    return hipGetLastError();
+}
+
+hipError_t launchb_reduce_pe(const BatchPlan &P, hipStream_t st)
+{
+    hipLaunchKernelGGL(kb_reduce_pe, dim3(P.nsys), dim3(1024), 0, st, P.descs);
     return hipGetLastError();
 }

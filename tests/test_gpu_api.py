@@ -1,0 +1,205 @@
+"""The public C API (libfmd.so) end to end on the GPU, against the oracle's
+time loops on identical inputs: integrate (velocity Verlet), equilibrate
+(Berendsen), EAM through the product's own setfl loader and tables, the
+1-D TTM through symplectic Euler, timer events and state files."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from fmd_amd import api
+from fmd_amd.workloads import maxwell_velocities
+from oracle.oracle import Oracle, TtmCfg, MASS_UNIT, KB
+from tests.pairsys import lattice_fcc, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def by_id(a):
+    o = np.argsort(a["atomid"])
+    return {k: v[o] for k, v in a.items() if k != "cell_count" and len(np.shape(v)) and len(v) == len(o)}
+
+
+def _lj_pair(rng, n=(8, 8, 8), T=120.0):
+    a = 5.26
+    x = lattice_fcc(n, a, rng, jitter=0.05)
+    v = maxwell_velocities(len(x), 39.948, T, rng)
+    return x, v, [n[0] * a, n[1] * a, n[2] * a]
+
+
+def test_integrate_lj_matches_oracle(rng):
+    x, v, L = _lj_pair(rng)
+    f = api.Fmd()
+    f.fmd_box_setSize(*L)
+    f.fmd_box_setPBC(True, True, True)
+    f.set_kinds(["Ar"], [39.948])
+    f.fmd_pot_lj_apply(0, 0, 3.405, 0.0104, 2.5 * 3.405)
+    f.load_atoms(x, v)
+    f.fmd_dync_integrate(api.GROUP_ALL, 0.03, 1e-3)
+    o = Oracle(L, [1, 1, 1], [39.948], ["Ar"])
+    o.set_lj(0, 0, 3.405, 0.0104, 2.5 * 3.405)
+    o.make_grid(); o.add_atoms(x, v); o.distribute()
+    steps, t = o.integrate_vv(0.0, 0.03, 1e-3)
+    assert steps == 30
+    assert f.fmd_dync_getTime() == t
+    ga, ra = f.atoms(), o.atoms()
+    assert np.array_equal(ga["atomid"], ra["atomid"])
+    assert rel_err(ga["x"], ra["x"], scale=L[0]) < 1e-12
+    e = o.energies()
+    assert f.fmd_matt_getTotalEnergy() == pytest.approx(e["pe"] + e["ke"], rel=1e-10)
+    assert f.fmd_matt_getTemperature() == pytest.approx(e["T"], rel=1e-10)
+    f.free()
+
+
+def test_equilibrate_eam_through_product_loader(rng, cu01):
+    a = 3.615
+    x = lattice_fcc((7, 7, 7), a, rng, jitter=0.03)
+    v = maxwell_velocities(len(x), 63.546, 500.0, rng)
+    L = [7 * a] * 3
+    f = api.Fmd()
+    f.fmd_box_setSize(*L)
+    f.fmd_box_setPBC(True, True, True)
+    f.set_kinds(["Cu"], [63.546])
+    pot = f.fmd_pot_eam_alloy_load(cu01.encode())
+    assert pot
+    assert f.fmd_pot_eam_getCutoffRadius(pot) == pytest.approx(5.506786, rel=1e-12)
+    f.fmd_pot_apply(0, 0, pot)
+    f.load_atoms(x, v)
+    f.fmd_dync_equilibrate(api.GROUP_ALL, 0.02, 2e-3, 0.05, 300.0)
+    assert f.fmd_dync_getTime() == 0.0   # equilibrate restores the clock (integrators.c:310-324)
+    o = Oracle(L, [1, 1, 1], [63.546], ["Cu"])
+    o.load_eam(cu01); o.apply_eam(0, 0)
+    o.make_grid(); o.add_atoms(x, v); o.distribute()
+    steps, _ = o.integrate_vv(0.0, 0.02, 2e-3, thermostat=1, tau=0.05, T0=300.0)
+    assert steps == 10
+    ga, ra = f.atoms(), o.atoms()
+    assert np.array_equal(ga["atomid"], ra["atomid"])
+    assert rel_err(ga["x"], ra["x"], scale=L[0]) < 1e-12
+    assert rel_err(ga["f"], ra["f"]) < 1e-9
+    e = o.energies()
+    assert f.fmd_matt_getTemperature() == pytest.approx(e["T"], rel=1e-10)
+    assert f.fmd_ext_getPotentialEnergy() == pytest.approx(e["pe"], rel=1e-10)
+    f.free()
+
+
+def test_timer_events_and_state_roundtrip(rng, tmp_path):
+    x, v, L = _lj_pair(rng, n=(6, 6, 6))
+    f = api.Fmd()
+    f.fmd_box_setSize(*L)
+    f.fmd_box_setPBC(True, True, True)
+    f.set_kinds(["Ar"], [39.948])
+    f.fmd_pot_lj_apply(0, 0, 3.405, 0.0104, 2.5 * 3.405)
+    f.load_atoms(x, v)
+    ticks = []
+
+    @api.EventHandler
+    def handler(md, ev, usp, params):
+        if ev == api.EVENT_TIMER_TICK:
+            t = C.cast(params, C.POINTER(api.TimerTick)).contents.timer
+            ticks.append((t, api.lib().fmd_dync_getTime(md), api.lib().fmd_matt_getTotalEnergy(md)))
+
+    f.fmd_setEventHandler(None, handler)
+    t0 = f.fmd_timer_makeSimple(0.0, 0.005, -1.0)
+    t1 = f.fmd_timer_makeSimple(0.0, 0.01, 0.02)
+    f.fmd_dync_integrate(api.GROUP_ALL, 0.03, 1e-3)
+    n0 = [t for t in ticks if t[0] == t0]
+    n1 = [t for t in ticks if t[0] == t1]
+    assert len(n0) == 7 and len(n1) == 3      # t = 0, 5, ..., 30 fs and t = 0, 10, 20 fs
+    # the LJ pair energy is truncated without a shift (lj.h:39-60), so the
+    # total energy jumps as pairs cross the cutoff: compare each tick with the
+    # oracle stepped through the same loop (integrators.c:40-75) instead
+    o = Oracle(L, [1, 1, 1], [39.948], ["Ar"])
+    o.set_lj(0, 0, 3.405, 0.0104, 2.5 * 3.405)
+    o.make_grid(); o.add_atoms(x, v); o.distribute()
+    o.forces(); o.thermo()
+    e = o.energies()
+    ref = {0.0: e["pe"] + e["ke"]}
+    t = 0.0
+    while t < 0.03:
+        o.vv_start(1e-3); o.forces(); o.vv_finish(1e-3)
+        t += 1e-3
+        e = o.energies()
+        ref[t] = e["pe"] + e["ke"]
+    for _, tt, en in n0 + n1:
+        assert tt in ref
+        assert en == pytest.approx(ref[tt], rel=1e-10)
+    path = str(tmp_path / "s.stt")
+    f.fmd_io_saveState(path.encode())
+    g = api.Fmd()
+    g.fmd_box_setSize(*L)
+    g.fmd_box_setPBC(True, True, True)
+    g.set_kinds(["Ar"], [39.948])
+    g.fmd_pot_lj_apply(0, 0, 3.405, 0.0104, 2.5 * 3.405)
+    g.fmd_io_loadState(path.encode(), True)
+    a1, a2 = f.atoms(), g.atoms()
+    assert np.array_equal(a1["x"], a2["x"]) and np.array_equal(a1["v"], a2["v"])
+    assert g.fmd_dync_getTime() == f.fmd_dync_getTime()
+    f.free(); g.free()
+
+
+def _ttm_cfg_for_oracle(gamma, K, G, ratio, laser, ncz):
+    import math
+    Lp = laser["AbsorptionDepth"] * 1e10
+    I0 = math.sqrt(4 * math.log(2) / math.pi) * laser["fluence"] / laser["duration"] * 6.2415091259e-14
+    sig = laser["duration"] / math.sqrt(8 * math.log(2)) * 1e12
+    c = TtmCfg()
+    c.type, c.ncz, c.ratio = 1, ncz, ratio
+    c.C_gamma, c.K, c.G = gamma * 6.2415091259e-12, K * 6.2415091259e-4, G * 6.2415091259e-24
+    c.act_frac, c.start, c.stop = 0.1, 0.0, -1.0
+    c.laser_t0 = laser["t0"] * 1e12
+    c.laser_tdiff = 4 * math.sqrt(2) * sig
+    c.laser_2sig2_inv = -0.5 / (sig * sig)
+    c.laser_absdepth_inv = -1.0 / Lp
+    c.laser_factor = I0 * (1 - laser["reflectance"]) / Lp
+    return c
+
+
+def test_ttm_type1_matches_oracle(rng, cu01):
+    # examples/03_ttm1.c geometry scaled down: Cu slab, open z faces, 1 x 1 x 20 turi
+    lp = 3.6316
+    L = [6 * lp, 6 * lp, 40 * lp]
+    x = lattice_fcc((6, 6, 30), lp, rng, jitter=0.02, origin=(0.0, 0.0, 5.0 * lp))
+    v = maxwell_velocities(len(x), 63.546, 300.0, rng)
+    laser = dict(fluence=4e4, reflectance=0.85, duration=100e-15, t0=5e-15, AbsorptionDepth=14e-9)
+    f = api.Fmd()
+    f.fmd_box_setSize(*L)
+    f.fmd_box_setPBC(True, True, False)
+    f.set_kinds(["Cu"], [63.546])
+    pot = f.fmd_pot_eam_alloy_load(cu01.encode())
+    f.fmd_pot_apply(0, 0, pot)
+    f.load_atoms(x, v)
+    f.fmd_ext_distribute()
+    turi = f.fmd_turi_add(api.TURI_TTM_TYPE1, 1, 1, 20, 0.0, -1.0)
+    fTe = f.fmd_field_find(turi, api.FIELD_TTM_TE)
+    fTl = f.fmd_field_find(turi, api.FIELD_TEMPERATURE)
+    fn = f.fmd_field_find(turi, api.FIELD_NUMBER)
+    assert (fn, fTl, fTe) == (0, 3, 4)
+    f._fmd_ttm_setHeatCapacity_linear(turi, api.HeatCapLinear(96.6))
+    f._fmd_ttm_setHeatConductivity_constant1(turi, 400.0)
+    f._fmd_ttm_setCouplingFactor_constant1(turi, 1e17)
+    f.fmd_ttm_setElectronTemperature(turi, 300.0)
+    f.fmd_ttm_setTimestepRatio(turi, 20)
+    f.fmd_ttm_setCellActivationFraction(turi, 0.1)
+    f._fmd_ttm_setLaserSource_gaussian(turi, api.LaserGaussian(laser["fluence"], laser["reflectance"],
+                                                                laser["t0"], laser["duration"],
+                                                                laser["AbsorptionDepth"]))
+    f.fmd_dync_integrate(api.GROUP_ALL, 0.01, 1e-3)
+    o = Oracle(L, [1, 1, 0], [63.546], ["Cu"])
+    o.load_eam(cu01); o.apply_eam(0, 0)
+    o.make_grid(); o.add_atoms(x, v); o.distribute()
+    o.thermo()
+    o.ttm_init(_ttm_cfg_for_oracle(96.6, 400.0, 1e17, 20, laser, 20), 300.0)
+    steps, t = o.integrate_se(0.0, 0.01, 1e-3)
+    assert steps == 10
+    r = o.ttm_arrays(22)
+    te = f.field_array(turi, fTe)
+    n = f.field_array(turi, fn, np.uint32)
+    assert np.array_equal(n, r["num"][1:21])
+    act = r["Te"][1:21] >= 0
+    assert np.array_equal(te < 0, ~act)
+    assert rel_err(te[act], r["Te"][1:21][act]) < 1e-10
+    ga, ra = f.atoms(), o.atoms()
+    assert np.array_equal(ga["atomid"], ra["atomid"])
+    assert rel_err(ga["x"], ra["x"], scale=L[2]) < 1e-12
+    f.free()
