@@ -25,8 +25,8 @@ struct fmdx_system
     PairParam hp[MAXK * MAXK];
     PairParam *dpairs;
     /* EAM */
-    double4 *Fc, *rhoc, *phic;
-    float4 *Fc32, *rhoc32, *phic32;
+    double4 *Fc, *rhoc, *phic, *rp;
+    float4 *Fc32, *rhoc32, *phic32, *rp32;
     /* TTM */
     int ttm_on;
     TtmState T;
@@ -157,6 +157,7 @@ extern "C" int fmdx_system_create(fmdx_system_t **out, const double l[3], const 
     r |= dalloc(&D.nearly, ncell); r |= dalloc(&D.fill, ncell);
     r |= dalloc(&D.counters, 8);
     r |= dalloc(&D.cell_pe, ncell); r |= dalloc(&D.cell_emb, ncell);
+    r |= dalloc(&D.hbits, (size_t)ncell * HBITS_WORDS * 64); r |= dalloc(&D.htrips, ncell);
     r |= dalloc(&D.thermo, 16);
     r |= dalloc(&s->dpairs, MAXK * MAXK);
     if (r) { fmdx_system_destroy(s); return FMDX_ENOMEM; }
@@ -168,6 +169,7 @@ extern "C" int fmdx_system_create(fmdx_system_t **out, const double l[3], const 
     hipMemsetAsync(D.counters, 0, sizeof(unsigned) * 8, s->st);
     hipMemsetAsync(D.cell_pe, 0, sizeof(double) * ncell, s->st);
     hipMemsetAsync(D.cell_emb, 0, sizeof(double) * ncell, s->st);
+    hipMemsetAsync(D.htrips, 0xff, sizeof(int) * ncell, s->st);   /* -1: no recorded hits */
     hipMemsetAsync(D.thermo, 0, sizeof(double) * 16, s->st);
     memset(s->hp, 0, sizeof s->hp);
     D.pairs = s->dpairs;
@@ -187,9 +189,11 @@ extern "C" void fmdx_system_destroy(fmdx_system_t *s)
     free_slots(D);
     dfree(D.cnt); dfree(D.nleave); dfree(D.narr); dfree(D.nearly); dfree(D.fill);
     dfree(D.counters); dfree(D.cell_pe); dfree(D.cell_emb); dfree(D.thermo);
+    dfree(D.hbits); dfree(D.htrips);
     dfree(s->dpairs);
     dfree(s->Fc); dfree(s->rhoc); dfree(s->phic);
     dfree(s->Fc32); dfree(s->rhoc32); dfree(s->phic32);
+    dfree(s->rp); dfree(s->rp32);
     for (int i = 0; i < s->nttm_mem; i++) hipFree(s->ttm_mem[i]);
     for (int i = 0; i < s->nev; i++) hipEventDestroy(s->ev[i]);
     for (int i = 0; i < NPROF * 3; i++)
@@ -293,8 +297,16 @@ extern "C" int fmdx_set_eam(fmdx_system_t *s, int nelem, int nrho, double drho, 
             b[i] = make_float4((float)a[i].x, (float)a[i].y, (float)a[i].z, (float)a[i].w);
     };
     cvt(hF, fF); cvt(hR, fR); cvt(hP, fP);
+    /* element 0: rho and phi coefficients of one interval side by side, so that a
+       single-element pair fetches one 64-byte (FP64) or 32-byte (FP32) record */
+    std::vector<double4> hRP(2 * (size_t)m2);
+    for (long k = 0; k < m2; k++) { hRP[2 * k] = hR[k]; hRP[2 * k + 1] = hP[k]; }
+    std::vector<float4> fRP(hRP.size());
+    cvt(hRP, fRP);
     dfree(s->Fc); dfree(s->rhoc); dfree(s->phic); dfree(s->Fc32); dfree(s->rhoc32); dfree(s->phic32);
+    dfree(s->rp); dfree(s->rp32);
     int r = 0;
+    r |= dalloc(&s->rp, hRP.size()); r |= dalloc(&s->rp32, fRP.size());
     r |= dalloc(&s->Fc, hF.size()); r |= dalloc(&s->rhoc, hR.size()); r |= dalloc(&s->phic, hP.size());
     r |= dalloc(&s->Fc32, fF.size()); r |= dalloc(&s->rhoc32, fR.size()); r |= dalloc(&s->phic32, fP.size());
     if (r) return FMDX_ENOMEM;
@@ -304,11 +316,14 @@ extern "C" int fmdx_set_eam(fmdx_system_t *s, int nelem, int nrho, double drho, 
     CK(hipMemcpy(s->Fc32, fF.data(), fF.size() * sizeof(float4), hipMemcpyHostToDevice));
     CK(hipMemcpy(s->rhoc32, fR.data(), fR.size() * sizeof(float4), hipMemcpyHostToDevice));
     CK(hipMemcpy(s->phic32, fP.data(), fP.size() * sizeof(float4), hipMemcpyHostToDevice));
+    CK(hipMemcpy(s->rp, hRP.data(), hRP.size() * sizeof(double4), hipMemcpyHostToDevice));
+    CK(hipMemcpy(s->rp32, fRP.data(), fRP.size() * sizeof(float4), hipMemcpyHostToDevice));
     EamDev &E = s->D.eam;
     E.ne = nelem; E.nrho = nrho; E.nr2 = nr2;
     E.drho = drho; E.inv_drho = 1.0 / drho; E.dr2 = dr2; E.inv_dr2 = 1.0 / dr2; E.rc2 = rc2;
     E.Fc = s->Fc; E.rhoc = s->rhoc; E.phic = s->phic;
     E.Fc32 = s->Fc32; E.rhoc32 = s->rhoc32; E.phic32 = s->phic32;
+    E.rp = s->rp; E.rp32 = s->rp32;
     for (int k = 0; k < s->D.nkinds; k++) s->D.elem[k] = elem_of_kind ? elem_of_kind[k] : -1;
     return FMDX_OK;
 }

@@ -23,6 +23,7 @@
 #define MAXK FMDX_MAX_KINDS
 #define MAXE FMDX_MAX_ELEMENTS
 #define NPART_MAX 4096         /* blocks reduced by the single-block reductions */
+#define HBITS_WORDS 4          /* 64-bit hit words per lane kept between the EAM passes */
 
 /* mover destination codes */
 #define MV_STAY (-1)
@@ -47,6 +48,8 @@ struct EamDev
     const double4 *rhoc;   /* [ne][nr2-1]  */
     const double4 *phic;   /* [ne*ne][nr2-1] */
     const float4 *Fc32, *rhoc32, *phic32;
+    const double4 *rp;     /* [2*(nr2-1)] element 0: rho, phi coefficients per interval */
+    const float4 *rp32;
 };
 
 struct TtmDev
@@ -93,6 +96,9 @@ struct KDesc
     int elem[MAXK];
     const PairParam *pairs;   /* [MAXK*MAXK] */
     EamDev eam;
+    /* pair-candidate hits found by the density pass, replayed by the force pass */
+    unsigned long long *hbits;   /* [ncell][HBITS_WORDS][64] one bit per (lane, scan trip) */
+    int *htrips;                 /* [ncell] hit words recorded, -1 = none (the force pass rescans) */
     /* reductions */
     double *cell_pe;       /* [ncell] pair energy per home cell */
     double *cell_emb;      /* [ncell] embedding energy per home cell */

@@ -1,21 +1,29 @@
 /*
  * k_force.hip -- 27-neighbour-cell pair kernels for LJ, Morse and two-pass EAM.
  *
- * One 64-lane wavefront owns one home cell.  Its lanes are split into
- * (home atom a, stripe s): with n home atoms each atom gets S = 64/n lanes,
- * and lane (a,s) tests every S-th atom of the 27 neighbour cells.  The scan
- * is wave-uniform (same trip count on every lane); a lane that finds a
- * partner inside the cutoff only records it in an LDS hit buffer, and the
- * spline / exp work runs in a separate flush loop over those buffers so that
- * it is never executed on behalf of a lane without a hit.  Partial results
- * of the S stripes are combined in a fixed order, so every atom's sum is
- * deterministic.
+ * One 64-lane wavefront (one workgroup) owns one home cell.
+ *  1. Staging.  The atoms of the 27 neighbour cells are copied into an LDS
+ *     candidate tile with coalesced SoA loads.  Periodic images are shifted
+ *     into the home cell's frame on the way in, as the reference's ghost
+ *     copies are (md-ghost.c:85-107).  Candidates outside the active group
+ *     and the padding of the last trip get NaN coordinates, so that they
+ *     never match.  A list longer than one tile is processed tile by tile.
+ *  2. Scan.  Lanes are split into (home atom a, stripe s): with n home atoms
+ *     each atom gets S = 64/n lanes and lane (a,s) tests candidates s, s+S,
+ *     s+2S, ... of the tile.  The trip count is wave-uniform, and every 64
+ *     trips leave one 64-bit word of hits per lane.  The spline / exp work
+ *     then runs over the set bits of that word, so it is never executed on
+ *     behalf of a lane without a hit.
+ *  3. Replay.  The EAM density pass stores its hit words; the force pass of
+ *     the same step stages the same tiles and walks those words instead of
+ *     scanning again.  Cells whose words do not fit are rescanned.
+ * The partial sums of the S stripes are combined in a fixed order, so every
+ * atom's sum is deterministic.
  *
  * Full shell: each atom accumulates only its own force / density, which
  * removes the reference's Newton-3 writes into neighbour cells (the OpenMP
  * race of lj.c:37-87, eam.c:38-142) and the ghost fold-back
- * (md-ghost.c:788-825).  Periodic images are formed by shifting the home
- * position by the box length of the wrapped neighbour cell.
+ * (md-ghost.c:788-825).
  *
  * Pair set and formulas:
  *   LJ      lj.h:26-58        F += 12*(4eps)/r^2 * ((s/r)^12 - (s/r)^6/2) r,   U = 4eps((s/r)^12-(s/r)^6)
@@ -25,148 +33,249 @@
  * with every tabulated function a natural cubic spline on the reference's
  * uniform r^2 (or rho) grid, stored as per-interval polynomial coefficients.
  * The pair energy of a pair is accumulated half on each side.
+ *
+ * Two precisions share this code: R = double (positions, arithmetic and sums
+ * in FP64) and R = float (FP32 positions relative to the home cell origin,
+ * FP32 pair arithmetic, Kahan-compensated FP32 sums, FP64 results).
  */
 #include "fmdx_internal.h"
 
 #define WG 64
-#define HB 32
+#define NOKIND 0xFF    /* padding or a candidate outside the active group */
 
-static __device__ __forceinline__ unsigned long long wave_any(bool p)
+/* pair-category specialisations chosen on the host */
+enum
 {
-    return __ballot(p);
-}
-
-/* spline value and derivative (w.r.t. x) on interval k with b = x/h - k */
-static __device__ __forceinline__ double sp_val(const double4 &q, double b)
-{
-    return q.x + b * (q.y + b * (q.z + b * q.w));
-}
-static __device__ __forceinline__ double sp_der(const double4 &q, double b, double inv_h)
-{
-    return (q.y + b * (2.0 * q.z + 3.0 * b * q.w)) * inv_h;
-}
-static __device__ __forceinline__ float sp_val32(const float4 &q, float b)
-{
-    return q.x + b * (q.y + b * (q.z + b * q.w));
-}
-static __device__ __forceinline__ float sp_der32(const float4 &q, float b, float inv_h)
-{
-    return (q.y + b * (2.0f * q.z + 3.0f * b * q.w)) * inv_h;
-}
-
-struct Acc
-{
-    double fx, fy, fz, e;
+    CAT_GEN = 0,     /* per-pair dispatch through the pair table */
+    CAT_EAM1 = 1,    /* EAM everywhere, one element: interleaved rho/phi records */
+    CAT_LJ = 2,      /* one kind, LJ */
+    CAT_MORSE = 3    /* one kind, Morse */
 };
 
-/* per-wave LDS scratch: the hit buffer during the scan, the stripe sums afterwards */
-union ShScratch
+template <typename R> struct Prec;
+template <> struct Prec<double>
 {
-    uint16_t hb[HB][WG];
-    double red[4][WG];
+    enum { CT = 256 };   /* candidates per LDS tile */
+    typedef double4 Q;
+    static __device__ __forceinline__ const double4 *rho(const EamDev &E) { return E.rhoc; }
+    static __device__ __forceinline__ const double4 *phi(const EamDev &E) { return E.phic; }
+    static __device__ __forceinline__ const double4 *rp(const EamDev &E) { return E.rp; }
+    static __device__ __forceinline__ const double *px(const KDesc &D) { return D.x; }
+    static __device__ __forceinline__ const double *py(const KDesc &D) { return D.y; }
+    static __device__ __forceinline__ const double *pz(const KDesc &D) { return D.z; }
+};
+template <> struct Prec<float>
+{
+    enum { CT = 512 };
+    typedef float4 Q;
+    static __device__ __forceinline__ const float4 *rho(const EamDev &E) { return E.rhoc32; }
+    static __device__ __forceinline__ const float4 *phi(const EamDev &E) { return E.phic32; }
+    static __device__ __forceinline__ const float4 *rp(const EamDev &E) { return E.rp32; }
+    static __device__ __forceinline__ const float *px(const KDesc &D) { return D.xr; }
+    static __device__ __forceinline__ const float *py(const KDesc &D) { return D.yr; }
+    static __device__ __forceinline__ const float *pz(const KDesc &D) { return D.zr; }
 };
 
-/* Kahan-compensated FP32 accumulator */
-struct KahanF
+/* LDS candidate tile (positions in the home frame, F' of the candidate, kind);
+   N covers the tile plus the padding of the last scan trip */
+template <typename R, bool FP> struct Tile;
+template <bool FP> struct Tile<double, FP>
+{
+    enum { N = Prec<double>::CT + WG };
+    double2 xy[N];
+    double z[N];
+    double fp[FP ? N : 1];
+    uint8_t k[N];
+    __device__ __forceinline__ void put(int m, double x, double y, double zz, double f, int kk)
+    {
+        xy[m] = make_double2(x, y);
+        z[m] = zz;
+        if (FP) fp[m] = f;
+        k[m] = (uint8_t)kk;
+    }
+    __device__ __forceinline__ void pos(int m, double &x, double &y, double &zz) const
+    {
+        const double2 a = xy[m];
+        x = a.x; y = a.y; zz = z[m];
+    }
+    __device__ __forceinline__ void load(int m, double &x, double &y, double &zz, double &f) const
+    {
+        pos(m, x, y, zz);
+        f = FP ? fp[m] : 0.0;
+    }
+};
+template <bool FP> struct Tile<float, FP>
+{
+    enum { N = Prec<float>::CT + WG };
+    float4 p[N];
+    uint8_t k[N];
+    __device__ __forceinline__ void put(int m, float x, float y, float zz, float f, int kk)
+    {
+        p[m] = make_float4(x, y, zz, f);
+        k[m] = (uint8_t)kk;
+    }
+    __device__ __forceinline__ void pos(int m, float &x, float &y, float &zz) const
+    {
+        const float4 a = p[m];
+        x = a.x; y = a.y; zz = a.z;
+    }
+    __device__ __forceinline__ void load(int m, float &x, float &y, float &zz, float &f) const
+    {
+        const float4 a = p[m];
+        x = a.x; y = a.y; zz = a.z; f = a.w;
+    }
+};
+
+/* accumulators: plain FP64, or Kahan-compensated FP32 */
+struct AccD
+{
+    double v;
+    __device__ __forceinline__ void zero() { v = 0.0; }
+    __device__ __forceinline__ void add(double x) { v += x; }
+    __device__ __forceinline__ double get() const { return v; }
+};
+struct AccF
 {
     float s, c;
-    __device__ void add(float v)
+    __device__ __forceinline__ void zero() { s = 0.0f; c = 0.0f; }
+    __device__ __forceinline__ void add(float v)
     {
         float y = v - c;
         float t = s + y;
         c = (t - s) - y;
         s = t;
     }
+    __device__ __forceinline__ double get() const { return (double)s - (double)c; }
 };
+template <typename R> struct AccOf;
+template <> struct AccOf<double> { typedef AccD T; };
+template <> struct AccOf<float> { typedef AccF T; };
 
-/* ---------------------------------------------------------------- FP64 */
-
-template <int PASS>
-static __device__ __forceinline__ void hit64(const KDesc &D, const PairParam *pp, int nk, int ki, double Fpi,
-                                             double dx, double dy, double dz, long js, Acc &acc)
+/* spline value and derivative (w.r.t. x) on interval k with b = x/h - k */
+template <typename R, typename Q>
+static __device__ __forceinline__ R sp_val(const Q &q, R b)
 {
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    const int kj = D.kind[js];
-    const PairParam &p = pp[ki * nk + kj];
-    if (!(r2 < p.rc2)) return;
+    return q.x + b * (q.y + b * (q.z + b * q.w));
+}
+template <typename R, typename Q>
+static __device__ __forceinline__ R sp_der(const Q &q, R b, R inv_h)
+{
+    return (q.y + b * ((R)2 * q.z + (R)3 * b * q.w)) * inv_h;
+}
+
+static __device__ __forceinline__ double fexp(double x) { return exp(x); }
+static __device__ __forceinline__ float fexp(float x) { return expf(x); }
+static __device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
+static __device__ __forceinline__ float fsqrt(float x) { return sqrtf(x); }
+
+/* one candidate pair inside the search radius; acc = {fx, fy, fz, energy or density} */
+template <typename R, int PASS, int CAT, typename A>
+static __device__ __forceinline__ void pair_hit(const KDesc &D, const PairParam &p, R Fpi, R Fpj, R dx, R dy,
+                                                R dz, A *acc)
+{
+    typedef typename Prec<R>::Q Q;
+    const R r2 = dx * dx + dy * dy + dz * dz;
+    if (!(r2 < (R)p.rc2)) return;
+    const int cat = CAT == CAT_GEN ? p.cat
+                  : CAT == CAT_EAM1 ? FMDX_POT_EAM
+                  : CAT == CAT_LJ ? FMDX_POT_LJ : FMDX_POT_MORSE;
+    const EamDev &E = D.eam;
     if (PASS == 1)
     {
-        if (p.cat != FMDX_POT_EAM) return;
-        const EamDev &E = D.eam;
-        double t = r2 * E.inv_dr2;
+        if (cat != FMDX_POT_EAM) return;
+        const R t = r2 * (R)E.inv_dr2;
         int k = (int)t;
         if (k > E.nr2 - 2) k = E.nr2 - 2;
-        double b = t - k;
-        acc.e += sp_val(E.rhoc[(long)p.jloc * (E.nr2 - 1) + k], b);
+        const R b = t - (R)k;
+        const Q q = CAT == CAT_EAM1 ? Prec<R>::rp(E)[2 * (long)k]
+                                    : Prec<R>::rho(E)[(long)p.jloc * (E.nr2 - 1) + k];
+        acc[3].add(sp_val<R, Q>(q, b));
         return;
     }
-    if (p.cat == FMDX_POT_EAM)
+    if (cat == FMDX_POT_EAM)
     {
-        const EamDev &E = D.eam;
-        double t = r2 * E.inv_dr2;
+        const R inv = (R)E.inv_dr2;
+        const R t = r2 * inv;
         int k = (int)t;
         if (k > E.nr2 - 2) k = E.nr2 - 2;
-        double b = t - k;
-        const long m = E.nr2 - 1;
-        double rho_ip = sp_der(E.rhoc[(long)p.iloc * m + k], b, E.inv_dr2);
-        double rho_jp = (p.jloc == p.iloc) ? rho_ip : sp_der(E.rhoc[(long)p.jloc * m + k], b, E.inv_dr2);
-        const double4 q = E.phic[(long)(p.iloc * E.ne + p.jloc) * m + k];
-        double phid = sp_der(q, b, E.inv_dr2);
-        double Fpj = D.aux[js];
-        double mag = 2.0 * (Fpi * rho_jp + Fpj * rho_ip + phid);
-        acc.fx -= mag * dx;
-        acc.fy -= mag * dy;
-        acc.fz -= mag * dz;
-        acc.e += 0.5 * sp_val(q, b);
+        const R b = t - (R)k;
+        R rho_ip, rho_jp;
+        Q q;
+        if (CAT == CAT_EAM1)
+        {
+            const Q *rec = Prec<R>::rp(E) + 2 * (long)k;
+            rho_ip = sp_der<R, Q>(rec[0], b, inv);
+            rho_jp = rho_ip;
+            q = rec[1];
+        }
+        else
+        {
+            const long m = E.nr2 - 1;
+            const Q *rt = Prec<R>::rho(E);
+            rho_ip = sp_der<R, Q>(rt[(long)p.iloc * m + k], b, inv);
+            rho_jp = (p.jloc == p.iloc) ? rho_ip : sp_der<R, Q>(rt[(long)p.jloc * m + k], b, inv);
+            q = Prec<R>::phi(E)[(long)(p.iloc * E.ne + p.jloc) * m + k];
+        }
+        const R phid = sp_der<R, Q>(q, b, inv);
+        const R mag = (R)2 * (Fpi * rho_jp + Fpj * rho_ip + phid);
+        acc[0].add(-mag * dx);
+        acc[1].add(-mag * dy);
+        acc[2].add(-mag * dz);
+        acc[3].add((R)0.5 * sp_val<R, Q>(q, b));
     }
-    else if (p.cat == FMDX_POT_LJ)
+    else if (cat == FMDX_POT_LJ)
     {
-        double inv_r2 = 1.0 / r2;
-        double inv_rs2 = p.p1 * inv_r2;
-        double inv_rs6 = inv_rs2 * inv_rs2 * inv_rs2;
-        double inv_rs12 = inv_rs6 * inv_rs6;
-        double factor = 12.0 * p.p0 * inv_r2 * (inv_rs12 - 0.5 * inv_rs6);
-        acc.fx += factor * dx;
-        acc.fy += factor * dy;
-        acc.fz += factor * dz;
-        acc.e += 0.5 * p.p0 * (inv_rs12 - inv_rs6);
+        const R inv_r2 = (R)1 / r2;
+        const R inv_rs2 = (R)p.p1 * inv_r2;
+        const R inv_rs6 = inv_rs2 * inv_rs2 * inv_rs2;
+        const R inv_rs12 = inv_rs6 * inv_rs6;
+        const R factor = (R)12 * (R)p.p0 * inv_r2 * (inv_rs12 - (R)0.5 * inv_rs6);
+        acc[0].add(factor * dx);
+        acc[1].add(factor * dy);
+        acc[2].add(factor * dz);
+        acc[3].add((R)0.5 * (R)p.p0 * (inv_rs12 - inv_rs6));
     }
-    else if (p.cat == FMDX_POT_MORSE)
+    else if (cat == FMDX_POT_MORSE)
     {
-        double r = sqrt(r2);
-        double e1 = exp(-p.p1 * (r - p.p2));
-        double e2 = e1 * e1;
-        double factor = 2.0 * p.p1 * p.p0 / r * (e2 - e1);
-        acc.fx += factor * dx;
-        acc.fy += factor * dy;
-        acc.fz += factor * dz;
-        acc.e += 0.5 * p.p0 * (e2 - 2.0 * e1);
+        const R r = fsqrt(r2);
+        const R e1 = fexp(-(R)p.p1 * (r - (R)p.p2));
+        const R e2 = e1 * e1;
+        const R factor = (R)2 * (R)p.p1 * (R)p.p0 / r * (e2 - e1);
+        acc[0].add(factor * dx);
+        acc[1].add(factor * dy);
+        acc[2].add(factor * dz);
+        acc[3].add((R)0.5 * (R)p.p0 * (e2 - (R)2 * e1));
     }
 }
 
-template <int PASS>
-static __device__ void b_pairs64(const KDesc &D, const int c)
+template <typename R, int PASS, int CAT>
+static __device__ void b_cell(const KDesc &D, const int c)
 {
-    const int ag = D.ag;
-    const TtmDev &T = D.ttm;
+    typedef typename AccOf<R>::T Acc;
+    typedef Prec<R> P;
+    enum { CT = P::CT };
+    constexpr bool WITH_FP = PASS == 0 && (CAT == CAT_GEN || CAT == CAT_EAM1);
     if (PASS == 1 && !D.has_eam)
     {
-        if (threadIdx.x == 0) D.cell_emb[c] = 0.0;
+        if (threadIdx.x == 0) { D.cell_emb[c] = 0.0; D.htrips[c] = -1; }
         return;
     }
-    __shared__ int nb_base[27], nb_cnt[27];
-    __shared__ double nb_sx[27], nb_sy[27], nb_sz[27];
-    __shared__ ShScratch sm;
+    __shared__ long nb_base[27];
+    __shared__ int nb_pre[28];
+    __shared__ R nb_sx[27], nb_sy[27], nb_sz[27];
+    __shared__ Tile<R, WITH_FP> T;
     extern __shared__ PairParam pp[];   /* nkinds^2 entries, sized at launch */
-    uint16_t(&hb)[HB][WG] = sm.hb;
-    double(&red)[4][WG] = sm.red;
+    /* the stripe sums reuse the tile once the scan is over */
+    double(*red)[WG] = reinterpret_cast<double(*)[WG]>(&T);
+    static_assert(sizeof(Tile<R, WITH_FP>) >= 4 * WG * sizeof(double), "tile too small for the stripe sums");
 
     const int lane = threadIdx.x;
     const int nkk = D.nkinds * D.nkinds;
     for (int i = lane; i < nkk; i += WG) pp[i] = D.pairs[i];
+    const int cz = c % D.nc2, ct = c / D.nc2, cy = ct % D.nc1, cx = ct / D.nc1;
+    int my_cnt = 0;
     if (lane < 27)
     {
-        int cz = c % D.nc2, t = c / D.nc2, cy = t % D.nc1, cx = t / D.nc1;
         int jx = cx + lane / 9 - 1, jy = cy + (lane / 3) % 3 - 1, jz = cz + lane % 3 - 1;
         double sx = 0, sy = 0, sz = 0;
         bool ok = true;
@@ -176,350 +285,156 @@ static __device__ void b_pairs64(const KDesc &D, const int c)
         else if (jy >= D.nc1) { if (D.pbc1) { jy -= D.nc1; sy = D.L1; } else ok = false; }
         if (jz < 0) { if (D.pbc2) { jz += D.nc2; sz = -D.L2; } else ok = false; }
         else if (jz >= D.nc2) { if (D.pbc2) { jz -= D.nc2; sz = D.L2; } else ok = false; }
-        int jc = ok ? (jx * D.nc1 + jy) * D.nc2 + jz : 0;
+        const long jc = ok ? ((long)jx * D.nc1 + jy) * D.nc2 + jz : 0;
         nb_base[lane] = jc * D.cap;
-        nb_cnt[lane] = ok ? D.cnt[jc] : 0;
-        nb_sx[lane] = sx; nb_sy[lane] = sy; nb_sz[lane] = sz;
+        my_cnt = ok ? D.cnt[jc] : 0;
+        if (sizeof(R) == sizeof(double))
+        {
+            /* image shift of the neighbour cell: staged x = x_j + shift */
+            nb_sx[lane] = (R)sx; nb_sy[lane] = (R)sy; nb_sz[lane] = (R)sz;
+        }
+        else
+        {
+            /* origin of the image of cell jc relative to the home cell origin (FP64, then rounded) */
+            nb_sx[lane] = (R)((jx * D.h0 + sx) - cx * D.h0);
+            nb_sy[lane] = (R)((jy * D.h1 + sy) - cy * D.h1);
+            nb_sz[lane] = (R)((jz * D.h2 + sz) - cz * D.h2);
+        }
     }
+    /* inclusive prefix of the neighbour counts over lanes 0..26 */
+    int incl = my_cnt;
+    for (int o = 1; o < 32; o <<= 1)
+    {
+        const int v = __shfl_up(incl, o);
+        if (lane >= o) incl += v;
+    }
+    if (lane < 27) nb_pre[lane + 1] = incl;
+    if (lane == 0) nb_pre[0] = 0;
+    const int ncand = __shfl(incl, 26);
     __syncthreads();
+
     const int nh = D.cnt[c];
-    double cell_acc = 0.0;
-    const double rcut2 = PASS == 1 ? D.rceam2 : D.rcmax2;
-
-    for (int h0 = 0; h0 < nh; h0 += WG)
-    {
-        const int nhc = min(WG, nh - h0);
-        const int S = WG / nhc;
-        const int a = lane % nhc, s = lane / nhc;
-        const long is = (long)c * D.cap + h0 + a;
-        double xi = 0, yi = 0, zi = 0, Fpi = 0;
-        int ki = 0;
-        bool act = false;
-        if (s < S)
-        {
-            xi = D.x[is]; yi = D.y[is]; zi = D.z[is];
-            ki = D.kind[is];
-            act = (ag < 0 || D.grp[is] == ag);
-            if (PASS == 1 && D.elem[ki] < 0) act = false;
-            if (PASS == 0 && D.has_eam) Fpi = D.aux[is];
-        }
-        Acc acc = {0, 0, 0, 0};
-        int fill = 0;
-        for (int k = 0; k < 27; k++)
-        {
-            const int nk = nb_cnt[k];
-            if (nk == 0) continue;
-            const long base = nb_base[k];
-            const double xs = xi - nb_sx[k], ys = yi - nb_sy[k], zs = zi - nb_sz[k];
-            const int trips = (nk + S - 1) / S;
-            for (int t = 0; t < trips; t++)
-            {
-                const int j = s + t * S;
-                bool hit = false;
-                if (act && j < nk && !(k == 13 && j == h0 + a))
-                {
-                    const long js = base + j;
-                    double dx = xs - D.x[js], dy = ys - D.y[js], dz = zs - D.z[js];
-                    double r2 = dx * dx + dy * dy + dz * dz;
-                    hit = r2 < rcut2;
-                    if (hit && ag >= 0 && D.grp[js] != ag) hit = false;
-                }
-                if (hit) hb[fill++][lane] = (uint16_t)((k << 8) | j);
-                if (wave_any(fill == HB))
-                {
-                    for (int h = 0; wave_any(h < fill); h++)
-                    {
-                        if (h < fill)
-                        {
-                            unsigned e = hb[h][lane];
-                            int kk = e >> 8, jj = e & 255;
-                            long js = nb_base[kk] + jj;
-                            double dx = (xi - nb_sx[kk]) - D.x[js];
-                            double dy = (yi - nb_sy[kk]) - D.y[js];
-                            double dz = (zi - nb_sz[kk]) - D.z[js];
-                            hit64<PASS>(D, pp, D.nkinds, ki, Fpi, dx, dy, dz, js, acc);
-                        }
-                    }
-                    fill = 0;
-                }
-            }
-        }
-        for (int h = 0; wave_any(h < fill); h++)
-        {
-            if (h < fill)
-            {
-                unsigned e = hb[h][lane];
-                int kk = e >> 8, jj = e & 255;
-                long js = nb_base[kk] + jj;
-                double dx = (xi - nb_sx[kk]) - D.x[js];
-                double dy = (yi - nb_sy[kk]) - D.y[js];
-                double dz = (zi - nb_sz[kk]) - D.z[js];
-                hit64<PASS>(D, pp, D.nkinds, ki, Fpi, dx, dy, dz, js, acc);
-            }
-        }
-        __syncthreads();   /* the hit buffer and the reduction scratch share LDS */
-        red[0][lane] = acc.fx; red[1][lane] = acc.fy; red[2][lane] = acc.fz; red[3][lane] = acc.e;
-        __syncthreads();
-        double ecell = 0.0;
-        if (lane < nhc)
-        {
-            double sfx = 0, sfy = 0, sfz = 0, se = 0;
-            for (int q = 0; q < S; q++)
-            {
-                int l = lane + q * nhc;
-                sfx += red[0][l]; sfy += red[1][l]; sfz += red[2][l]; se += red[3][l];
-            }
-            if (PASS == 1)
-            {
-                double Fp = 0.0;
-                if (act)
-                {
-                    const EamDev &E = D.eam;
-                    int el = D.elem[ki];
-                    double t = se * E.inv_drho;
-                    int k = (int)t;
-                    if (k >= E.nrho - 1 || k < 0 || !(se == se))
-                    {
-                        set_err(D, FMDX_DEV_ERR_RHO_RANGE);
-                        k = k < 0 ? 0 : E.nrho - 2;
-                    }
-                    double b = t - k;
-                    const double4 q = E.Fc[(long)el * (E.nrho - 1) + k];
-                    Fp = sp_der(q, b, E.inv_drho);
-                    ecell = sp_val(q, b);
-                }
-                D.aux[is] = Fp;
-            }
-            else
-            {
-                if (act && T.on)
-                {
-                    int iz = (int)(zi / T.tcellh_z) + 1;
-                    if (iz >= 0 && iz < T.ncz + 2)
-                        sfz += T.xi[iz] * D.mass[ki] * (D.vz[is] - T.vcm[3 * iz + 2]);
-                }
-                D.fx[is] = act ? sfx : 0.0;
-                D.fy[is] = act ? sfy : 0.0;
-                D.fz[is] = act ? sfz : 0.0;
-                ecell = se;
-            }
-        }
-        __syncthreads();
-        red[0][lane] = ecell;
-        __syncthreads();
-        if (lane == 0)
-            for (int l = 0; l < nhc; l++) cell_acc += red[0][l];
-        __syncthreads();
-    }
-    if (lane == 0)
-    {
-        if (PASS == 1) D.cell_emb[c] = cell_acc;
-        else D.cell_pe[c] = cell_acc;
-    }
-}
-
-template <int PASS>
-__global__ __launch_bounds__(WG) void k_pairs64(KDesc D) { b_pairs64<PASS>(D, blockIdx.x); }
-
-template <int PASS>
-__global__ __launch_bounds__(WG) void kb_pairs64(BatchMap M)
-{
-    const int2 it = M.items[blockIdx.x];
-    const KDesc D = M.descs[it.x];
-    b_pairs64<PASS>(D, it.y);
-}
-
-/* ------------------------------------------------------------- mixed */
-
-/* FP32 pair arithmetic on displacements taken from the FP64 positions,
-   Kahan-compensated FP32 accumulation, FP64 results */
-template <int PASS>
-static __device__ __forceinline__ void hit32(const KDesc &D, const PairParam *pp, int nk, int ki, float Fpi,
-                                             float dx, float dy, float dz, long js,
-                                             KahanF &ax, KahanF &ay, KahanF &az, KahanF &ae)
-{
-    const float r2 = dx * dx + dy * dy + dz * dz;
-    const int kj = D.kind[js];
-    const PairParam &p = pp[ki * nk + kj];
-    if (!(r2 < (float)p.rc2)) return;
-    const EamDev &E = D.eam;
-    if (PASS == 1)
-    {
-        if (p.cat != FMDX_POT_EAM) return;
-        float t = r2 * (float)E.inv_dr2;
-        int k = (int)t;
-        if (k > E.nr2 - 2) k = E.nr2 - 2;
-        float b = t - (float)k;
-        ae.add(sp_val32(E.rhoc32[(long)p.jloc * (E.nr2 - 1) + k], b));
-        return;
-    }
-    if (p.cat == FMDX_POT_EAM)
-    {
-        float inv = (float)E.inv_dr2;
-        float t = r2 * inv;
-        int k = (int)t;
-        if (k > E.nr2 - 2) k = E.nr2 - 2;
-        float b = t - (float)k;
-        const long m = E.nr2 - 1;
-        float rho_ip = sp_der32(E.rhoc32[(long)p.iloc * m + k], b, inv);
-        float rho_jp = (p.jloc == p.iloc) ? rho_ip : sp_der32(E.rhoc32[(long)p.jloc * m + k], b, inv);
-        const float4 q = E.phic32[(long)(p.iloc * E.ne + p.jloc) * m + k];
-        float phid = sp_der32(q, b, inv);
-        float Fpj = (float)D.aux[js];
-        float mag = 2.0f * (Fpi * rho_jp + Fpj * rho_ip + phid);
-        ax.add(-mag * dx);
-        ay.add(-mag * dy);
-        az.add(-mag * dz);
-        ae.add(0.5f * sp_val32(q, b));
-    }
-    else if (p.cat == FMDX_POT_LJ)
-    {
-        float inv_r2 = 1.0f / r2;
-        float inv_rs2 = (float)p.p1 * inv_r2;
-        float inv_rs6 = inv_rs2 * inv_rs2 * inv_rs2;
-        float inv_rs12 = inv_rs6 * inv_rs6;
-        float factor = 12.0f * (float)p.p0 * inv_r2 * (inv_rs12 - 0.5f * inv_rs6);
-        ax.add(factor * dx);
-        ay.add(factor * dy);
-        az.add(factor * dz);
-        ae.add(0.5f * (float)p.p0 * (inv_rs12 - inv_rs6));
-    }
-    else if (p.cat == FMDX_POT_MORSE)
-    {
-        float r = sqrtf(r2);
-        float e1 = expf(-(float)p.p1 * (r - (float)p.p2));
-        float e2 = e1 * e1;
-        float factor = 2.0f * (float)p.p1 * (float)p.p0 / r * (e2 - e1);
-        ax.add(factor * dx);
-        ay.add(factor * dy);
-        az.add(factor * dz);
-        ae.add(0.5f * (float)p.p0 * (e2 - 2.0f * e1));
-    }
-}
-
-template <int PASS>
-static __device__ void b_pairs32(const KDesc &D, const int c)
-{
     const int ag = D.ag;
-    const TtmDev &T = D.ttm;
-    if (PASS == 1 && !D.has_eam)
-    {
-        if (threadIdx.x == 0) D.cell_emb[c] = 0.0;
-        return;
-    }
-    __shared__ int nb_base[27], nb_cnt[27];
-    __shared__ float nb_ox[27], nb_oy[27], nb_oz[27];   /* neighbour-cell origin minus home origin */
-    __shared__ ShScratch sm;
-    extern __shared__ PairParam pp[];   /* nkinds^2 entries, sized at launch */
-    uint16_t(&hb)[HB][WG] = sm.hb;
-    double(&red)[4][WG] = sm.red;
-
-    const int lane = threadIdx.x;
-    const int nkk = D.nkinds * D.nkinds;
-    for (int i = lane; i < nkk; i += WG) pp[i] = D.pairs[i];
-    const int cz = c % D.nc2, ct = c / D.nc2, cy = ct % D.nc1, cx = ct / D.nc1;
-    if (lane < 27)
-    {
-        int dxn = lane / 9 - 1, dyn = (lane / 3) % 3 - 1, dzn = lane % 3 - 1;
-        int jx = cx + dxn, jy = cy + dyn, jz = cz + dzn;
-        double sx = 0, sy = 0, sz = 0;
-        bool ok = true;
-        if (jx < 0) { if (D.pbc0) { jx += D.nc0; sx = -D.L0; } else ok = false; }
-        else if (jx >= D.nc0) { if (D.pbc0) { jx -= D.nc0; sx = D.L0; } else ok = false; }
-        if (jy < 0) { if (D.pbc1) { jy += D.nc1; sy = -D.L1; } else ok = false; }
-        else if (jy >= D.nc1) { if (D.pbc1) { jy -= D.nc1; sy = D.L1; } else ok = false; }
-        if (jz < 0) { if (D.pbc2) { jz += D.nc2; sz = -D.L2; } else ok = false; }
-        else if (jz >= D.nc2) { if (D.pbc2) { jz -= D.nc2; sz = D.L2; } else ok = false; }
-        int jc = ok ? (jx * D.nc1 + jy) * D.nc2 + jz : 0;
-        nb_base[lane] = jc * D.cap;
-        nb_cnt[lane] = ok ? D.cnt[jc] : 0;
-        /* origin of the image of cell jc relative to the home cell origin (FP64, then rounded) */
-        nb_ox[lane] = (float)((jx * D.h0 + sx) - cx * D.h0);
-        nb_oy[lane] = (float)((jy * D.h1 + sy) - cy * D.h1);
-        nb_oz[lane] = (float)((jz * D.h2 + sz) - cz * D.h2);
-    }
-    __syncthreads();
-    const int nh = D.cnt[c];
+    const R rcut2 = (R)D.rcmax2;
+    const R qnan = (R)__builtin_nan("");
+    const long hbase = (long)c * D.cap;
+    const bool one_block = nh > 0 && nh <= WG;
+    int hw = -1;
+    if (PASS == 0 && D.has_eam) hw = __builtin_amdgcn_readfirstlane(D.htrips[c]);
+    if (PASS == 1 && lane == 0 && !one_block) D.htrips[c] = -1;
+    const bool record = PASS == 1 && one_block;
+    const bool replay = PASS == 0 && one_block && hw >= 0;
+    PairParam p1;
+    if (CAT != CAT_GEN) p1 = pp[0];
     double cell_acc = 0.0;
-    const float rcut2 = (float)(PASS == 1 ? D.rceam2 : D.rcmax2);
-    const double ox = cx * D.h0, oy = cy * D.h1, oz = cz * D.h2;
 
     for (int h0 = 0; h0 < nh; h0 += WG)
     {
         const int nhc = min(WG, nh - h0);
         const int S = WG / nhc;
         const int a = lane % nhc, s = lane / nhc;
-        const long is = (long)c * D.cap + h0 + a;
-        float xi = 0, yi = 0, zi = 0, Fpi = 0;
+        const long is = hbase + h0 + a;
+        /* a lane without an active home atom scans with NaN coordinates: no hits */
+        R xi = qnan, yi = qnan, zi = qnan, Fpi = 0;
         int ki = 0;
         bool act = false;
         if (s < S)
         {
-            xi = D.xr[is]; yi = D.yr[is]; zi = D.zr[is];
             ki = D.kind[is];
             act = (ag < 0 || D.grp[is] == ag);
-            if (PASS == 1 && D.elem[ki] < 0) act = false;
-            if (PASS == 0 && D.has_eam) Fpi = (float)D.aux[is];
+            if (act) { xi = P::px(D)[is]; yi = P::py(D)[is]; zi = P::pz(D)[is]; }
+            if (PASS == 0 && D.has_eam) Fpi = (R)D.aux[is];
         }
-        KahanF ax = {0, 0}, ay = {0, 0}, az = {0, 0}, ae = {0, 0};
-        int fill = 0;
-        for (int k = 0; k < 27; k++)
+        const PairParam *ppi = pp + ki * D.nkinds;
+        const int selfm = nb_pre[13] + h0 + a;   /* candidate index of the home atom itself */
+        Acc acc[4];
+        for (int q = 0; q < 4; q++) acc[q].zero();
+        int wi = 0;   /* hit words so far (wave-uniform) */
+        for (int cs = 0; cs < ncand; cs += CT)
         {
-            const int nk = nb_cnt[k];
-            if (nk == 0) continue;
-            const long base = nb_base[k];
-            const float xs = xi - nb_ox[k], ys = yi - nb_oy[k], zs = zi - nb_oz[k];
-            const int trips = (nk + S - 1) / S;
-            for (int t = 0; t < trips; t++)
+            const int nt = min((int)CT, ncand - cs);
+            const int trips = (nt + S - 1) / S;
+            const int npad = trips * S;
+            __syncthreads();   /* the previous tile is no longer read */
+            for (int m = lane; m < npad; m += WG)
             {
-                const int j = s + t * S;
-                bool hit = false;
-                if (act && j < nk && !(k == 13 && j == h0 + a))
+                if (m < nt)
                 {
-                    const long js = base + j;
-                    float dx = xs - D.xr[js], dy = ys - D.yr[js], dz = zs - D.zr[js];
-                    float r2 = dx * dx + dy * dy + dz * dz;
-                    hit = r2 < rcut2;
-                    if (hit && ag >= 0 && D.grp[js] != ag) hit = false;
-                }
-                if (hit) hb[fill++][lane] = (uint16_t)((k << 8) | j);
-                if (wave_any(fill == HB))
-                {
-                    for (int h = 0; wave_any(h < fill); h++)
+                    const int g = cs + m;
+                    int lo = 0, hi = 27;
+                    while (hi - lo > 1)
                     {
-                        if (h < fill)
-                        {
-                            unsigned e = hb[h][lane];
-                            int kk = e >> 8, jj = e & 255;
-                            long js = nb_base[kk] + jj;
-                            float dx = (xi - nb_ox[kk]) - D.xr[js];
-                            float dy = (yi - nb_oy[kk]) - D.yr[js];
-                            float dz = (zi - nb_oz[kk]) - D.zr[js];
-                            hit32<PASS>(D, pp, D.nkinds, ki, Fpi, dx, dy, dz, js, ax, ay, az, ae);
-                        }
+                        const int mid = (lo + hi) >> 1;
+                        if (nb_pre[mid] <= g) lo = mid;
+                        else hi = mid;
                     }
-                    fill = 0;
+                    const long js = nb_base[lo] + (g - nb_pre[lo]);
+                    int kj = D.kind[js];
+                    R x = P::px(D)[js] + nb_sx[lo];
+                    R y = P::py(D)[js] + nb_sy[lo];
+                    R z = P::pz(D)[js] + nb_sz[lo];
+                    if (ag >= 0 && D.grp[js] != ag) { kj = NOKIND; x = y = z = qnan; }
+                    T.put(m, x, y, z, WITH_FP ? (R)D.aux[js] : (R)0, kj);
                 }
+                else
+                    T.put(m, qnan, qnan, qnan, (R)0, NOKIND);
             }
-        }
-        for (int h = 0; wave_any(h < fill); h++)
-        {
-            if (h < fill)
+            __syncthreads();
+            const int selft = selfm - cs;
+            for (int g0 = 0; g0 < trips; g0 += 64)
             {
-                unsigned e = hb[h][lane];
-                int kk = e >> 8, jj = e & 255;
-                long js = nb_base[kk] + jj;
-                float dx = (xi - nb_ox[kk]) - D.xr[js];
-                float dy = (yi - nb_oy[kk]) - D.yr[js];
-                float dz = (zi - nb_oz[kk]) - D.zr[js];
-                hit32<PASS>(D, pp, D.nkinds, ki, Fpi, dx, dy, dz, js, ax, ay, az, ae);
+                const int ng = min(64, trips - g0);
+                unsigned long long word;
+                if (replay)
+                    word = wi < hw ? D.hbits[((long)c * HBITS_WORDS + wi) * WG + lane] : 0ull;
+                else
+                {
+                    unsigned lo = 0, hi = 0;
+                    int m = s + g0 * S;
+                    const int n0 = min(ng, 32);
+                    for (int t = 0; t < n0; t++, m += S)
+                    {
+                        R x, y, z;
+                        T.pos(m, x, y, z);
+                        const R dx = xi - x, dy = yi - y, dz = zi - z;
+                        const R r2 = dx * dx + dy * dy + dz * dz;
+                        if (r2 < rcut2 && m != selft) lo |= 1u << t;
+                    }
+                    for (int t = 32; t < ng; t++, m += S)
+                    {
+                        R x, y, z;
+                        T.pos(m, x, y, z);
+                        const R dx = xi - x, dy = yi - y, dz = zi - z;
+                        const R r2 = dx * dx + dy * dy + dz * dz;
+                        if (r2 < rcut2 && m != selft) hi |= 1u << (t - 32);
+                    }
+                    word = ((unsigned long long)hi << 32) | lo;
+                    if (record && wi < HBITS_WORDS) D.hbits[((long)c * HBITS_WORDS + wi) * WG + lane] = word;
+                }
+                while (__ballot(word != 0ull))
+                {
+                    if (word)
+                    {
+                        const int t = __builtin_ctzll(word);
+                        word &= word - 1ull;
+                        const int m = s + (g0 + t) * S;
+                        R x, y, z, fj;
+                        T.load(m, x, y, z, fj);
+                        if (CAT == CAT_GEN)
+                            pair_hit<R, PASS, CAT>(D, ppi[T.k[m]], Fpi, fj, xi - x, yi - y, zi - z, acc);
+                        else
+                            pair_hit<R, PASS, CAT>(D, p1, Fpi, fj, xi - x, yi - y, zi - z, acc);
+                    }
+                }
+                wi++;
             }
         }
-        __syncthreads();   /* the hit buffer and the reduction scratch share LDS */
-        red[0][lane] = (double)ax.s - (double)ax.c;
-        red[1][lane] = (double)ay.s - (double)ay.c;
-        red[2][lane] = (double)az.s - (double)az.c;
-        red[3][lane] = (double)ae.s - (double)ae.c;
+        if (record && lane == 0) D.htrips[c] = wi <= HBITS_WORDS ? wi : -1;
+        __syncthreads();   /* the tile is reused for the stripe sums */
+        red[0][lane] = acc[0].get();
+        red[1][lane] = acc[1].get();
+        red[2][lane] = acc[2].get();
+        red[3][lane] = acc[3].get();
         __syncthreads();
         double ecell = 0.0;
         if (lane < nhc)
@@ -527,38 +442,39 @@ static __device__ void b_pairs32(const KDesc &D, const int c)
             double sfx = 0, sfy = 0, sfz = 0, se = 0;
             for (int q = 0; q < S; q++)
             {
-                int l = lane + q * nhc;
+                const int l = lane + q * nhc;
                 sfx += red[0][l]; sfy += red[1][l]; sfz += red[2][l]; se += red[3][l];
             }
             if (PASS == 1)
             {
                 double Fp = 0.0;
-                if (act)
+                const int el = D.elem[ki];
+                if (act && el >= 0)
                 {
                     const EamDev &E = D.eam;
-                    int el = D.elem[ki];
-                    double t = se * E.inv_drho;
+                    const double t = se * E.inv_drho;
                     int k = (int)t;
                     if (k >= E.nrho - 1 || k < 0 || !(se == se))
                     {
                         set_err(D, FMDX_DEV_ERR_RHO_RANGE);
                         k = k < 0 ? 0 : E.nrho - 2;
                     }
-                    double b = t - k;
+                    const double b = t - k;
                     const double4 q = E.Fc[(long)el * (E.nrho - 1) + k];
-                    Fp = sp_der(q, b, E.inv_drho);
-                    ecell = sp_val(q, b);
+                    Fp = sp_der<double, double4>(q, b, E.inv_drho);
+                    ecell = sp_val<double, double4>(q, b);
                 }
                 D.aux[is] = Fp;
             }
             else
             {
-                if (act && T.on)
+                const TtmDev &TT = D.ttm;
+                if (act && TT.on)
                 {
-                    double zabs = (double)zi + oz;
-                    int iz = (int)(zabs / T.tcellh_z) + 1;
-                    if (iz >= 0 && iz < T.ncz + 2)
-                        sfz += T.xi[iz] * D.mass[ki] * (D.vz[is] - T.vcm[3 * iz + 2]);
+                    const double zabs = sizeof(R) == sizeof(double) ? (double)zi : (double)zi + cz * D.h2;
+                    const int iz = (int)(zabs / TT.tcellh_z) + 1;
+                    if (iz >= 0 && iz < TT.ncz + 2)
+                        sfz += TT.xi[iz] * D.mass[ki] * (D.vz[is] - TT.vcm[3 * iz + 2]);
                 }
                 D.fx[is] = act ? sfx : 0.0;
                 D.fy[is] = act ? sfy : 0.0;
@@ -566,13 +482,11 @@ static __device__ void b_pairs32(const KDesc &D, const int c)
                 ecell = se;
             }
         }
-        (void)ox; (void)oy;
         __syncthreads();
         red[0][lane] = ecell;
         __syncthreads();
         if (lane == 0)
             for (int l = 0; l < nhc; l++) cell_acc += red[0][l];
-        __syncthreads();
     }
     if (lane == 0)
     {
@@ -581,15 +495,24 @@ static __device__ void b_pairs32(const KDesc &D, const int c)
     }
 }
 
-template <int PASS>
-__global__ __launch_bounds__(WG) void k_pairs32(KDesc D) { b_pairs32<PASS>(D, blockIdx.x); }
+/* Workgroups are dispatched to the 8 XCDs round-robin; give each XCD a
+   contiguous range of cells so that neighbour cells share its L2. */
+static __device__ __forceinline__ int xcd_cell(int b, int n)
+{
+    const int q = n >> 3, r = n & 7;
+    const int x = b & 7, i = b >> 3;
+    return x * q + min(x, r) + i;
+}
 
-template <int PASS>
-__global__ __launch_bounds__(WG) void kb_pairs32(BatchMap M)
+template <typename R, int PASS, int CAT>
+__global__ __launch_bounds__(WG) void k_cell(KDesc D) { b_cell<R, PASS, CAT>(D, xcd_cell(blockIdx.x, gridDim.x)); }
+
+template <typename R, int PASS>
+__global__ __launch_bounds__(WG) void kb_cell(BatchMap M)
 {
     const int2 it = M.items[blockIdx.x];
-    const KDesc D = M.descs[it.x];
-    b_pairs32<PASS>(D, it.y);
+    const KDesc &D = M.descs[it.x];
+    b_cell<R, PASS, CAT_GEN>(D, it.y);
 }
 
 /* FP32 positions relative to the origin of the home cell, from the FP64 positions */
@@ -609,11 +532,33 @@ __global__ __launch_bounds__(256) void k_relpos(KDesc D) { b_relpos(D, blockIdx.
 __global__ __launch_bounds__(256) void kb_relpos(BatchMap M)
 {
     const int2 it = M.items[blockIdx.x];
-    const KDesc D = M.descs[it.x];
+    const KDesc &D = M.descs[it.x];
     b_relpos(D, it.y);
 }
 
 static inline size_t pp_bytes(const KDesc &D) { return sizeof(PairParam) * D.nkinds * D.nkinds; }
+
+static int pick_cat(const KDesc &D)
+{
+    if (D.single_cat == FMDX_POT_EAM && D.eam.ne == 1 && D.eam.rp) return CAT_EAM1;
+    if (D.nkinds == 1 && D.single_cat == FMDX_POT_LJ) return CAT_LJ;
+    if (D.nkinds == 1 && D.single_cat == FMDX_POT_MORSE) return CAT_MORSE;
+    return CAT_GEN;
+}
+
+template <typename R, int PASS>
+static void launch_cell(const KDesc &D, hipStream_t st)
+{
+    const dim3 g(D.ncell), b(WG);
+    const size_t lds = pp_bytes(D);
+    switch (pick_cat(D))
+    {
+        case CAT_EAM1: hipLaunchKernelGGL((k_cell<R, PASS, CAT_EAM1>), g, b, lds, st, D); break;
+        case CAT_LJ: hipLaunchKernelGGL((k_cell<R, PASS, CAT_LJ>), g, b, lds, st, D); break;
+        case CAT_MORSE: hipLaunchKernelGGL((k_cell<R, PASS, CAT_MORSE>), g, b, lds, st, D); break;
+        default: hipLaunchKernelGGL((k_cell<R, PASS, CAT_GEN>), g, b, lds, st, D); break;
+    }
+}
 
 /* pass 1 (EAM density + F'(rho)) and pass 0 (forces + energy), launched separately so that
    callers can time each kernel with events on the stream */
@@ -623,21 +568,21 @@ hipError_t launch_pass1(const KDesc &D, hipStream_t st, int precision)
     {
         unsigned g = (unsigned)(((long)D.ncell * D.cap + 255) / 256);
         hipLaunchKernelGGL(k_relpos, dim3(g), dim3(256), 0, st, D);
-        if (D.has_eam) hipLaunchKernelGGL(k_pairs32<1>, dim3(D.ncell), dim3(WG), pp_bytes(D), st, D);
-        else launch_fill(D.cell_emb, D.ncell, 0.0, st);
+        if (D.has_eam) launch_cell<float, 1>(D, st);
+        else (void)launch_fill(D.cell_emb, D.ncell, 0.0, st);
     }
     else
     {
-        if (D.has_eam) hipLaunchKernelGGL(k_pairs64<1>, dim3(D.ncell), dim3(WG), pp_bytes(D), st, D);
-        else launch_fill(D.cell_emb, D.ncell, 0.0, st);
+        if (D.has_eam) launch_cell<double, 1>(D, st);
+        else (void)launch_fill(D.cell_emb, D.ncell, 0.0, st);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_pass0(const KDesc &D, hipStream_t st, int precision)
 {
-    if (precision == FMDX_PREC_MIXED) hipLaunchKernelGGL(k_pairs32<0>, dim3(D.ncell), dim3(WG), pp_bytes(D), st, D);
-    else hipLaunchKernelGGL(k_pairs64<0>, dim3(D.ncell), dim3(WG), pp_bytes(D), st, D);
+    if (precision == FMDX_PREC_MIXED) launch_cell<float, 0>(D, st);
+    else launch_cell<double, 0>(D, st);
     return hipGetLastError();
 }
 
@@ -658,13 +603,13 @@ hipError_t launchb_forces(const BatchPlan &P, hipStream_t st, int precision)
     if (precision == FMDX_PREC_MIXED)
     {
         hipLaunchKernelGGL(kb_relpos, dim3(P.n_slot), dim3(256), 0, st, ms);
-        hipLaunchKernelGGL(kb_pairs32<1>, dim3(P.n_cell), dim3(WG), ppb, st, mc);
-        hipLaunchKernelGGL(kb_pairs32<0>, dim3(P.n_cell), dim3(WG), ppb, st, mc);
+        hipLaunchKernelGGL((kb_cell<float, 1>), dim3(P.n_cell), dim3(WG), ppb, st, mc);
+        hipLaunchKernelGGL((kb_cell<float, 0>), dim3(P.n_cell), dim3(WG), ppb, st, mc);
     }
     else
     {
-        hipLaunchKernelGGL(kb_pairs64<1>, dim3(P.n_cell), dim3(WG), ppb, st, mc);
-        hipLaunchKernelGGL(kb_pairs64<0>, dim3(P.n_cell), dim3(WG), ppb, st, mc);
+        hipLaunchKernelGGL((kb_cell<double, 1>), dim3(P.n_cell), dim3(WG), ppb, st, mc);
+        hipLaunchKernelGGL((kb_cell<double, 0>), dim3(P.n_cell), dim3(WG), ppb, st, mc);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

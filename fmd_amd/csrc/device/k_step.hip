@@ -36,7 +36,7 @@
     __global__ __launch_bounds__(TPB) void kb_##NAME(BatchMap M)                       \
     {                                                                                  \
         const int2 it = M.items[blockIdx.x];                                           \
-        const KDesc D = M.descs[it.x];                                                 \
+        const KDesc &D = M.descs[it.x];                                                 \
         b_##NAME(D, it.y);                                                             \
     }
 
@@ -287,7 +287,7 @@ static __device__ void b_reduce_thermo(const KDesc &D)
 __global__ __launch_bounds__(1024) void k_reduce_thermo(KDesc D) { b_reduce_thermo(D); }
 __global__ __launch_bounds__(1024) void kb_reduce_thermo(const KDesc *descs)
 {
-    const KDesc D = descs[blockIdx.x];
+    const KDesc &D = descs[blockIdx.x];
     b_reduce_thermo(D);
 }
 
@@ -322,7 +322,7 @@ static __device__ void b_reduce_pe(const KDesc &D)
 __global__ __launch_bounds__(1024) void k_reduce_pe(KDesc D) { b_reduce_pe(D); }
 __global__ __launch_bounds__(1024) void kb_reduce_pe(const KDesc *descs)
 {
-    const KDesc D = descs[blockIdx.x];
+    const KDesc &D = descs[blockIdx.x];
     b_reduce_pe(D);
 }
 
