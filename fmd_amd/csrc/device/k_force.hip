@@ -79,29 +79,29 @@ template <> struct Prec<float>
 /* LDS candidate tile (positions in the home frame, F' of the candidate, kind);
    N covers the tile plus the padding of the last scan trip */
 template <typename R, bool FP> struct Tile;
+struct Rec3 { double x, y, z; };
+template <bool FP> struct RecOf { typedef double4 T; };
+template <> struct RecOf<false> { typedef Rec3 T; };
 template <bool FP> struct Tile<double, FP>
 {
     enum { N = Prec<double>::CT + WG };
-    double2 xy[N];
-    double z[N];
-    double fp[FP ? N : 1];
+    typename RecOf<FP>::T p[N];   /* x, y, z (, F') */
     uint8_t k[N];
     __device__ __forceinline__ void put(int m, double x, double y, double zz, double f, int kk)
     {
-        xy[m] = make_double2(x, y);
-        z[m] = zz;
-        if (FP) fp[m] = f;
+        p[m].x = x; p[m].y = y; p[m].z = zz;
+        if constexpr (FP) p[m].w = f;
         k[m] = (uint8_t)kk;
     }
     __device__ __forceinline__ void pos(int m, double &x, double &y, double &zz) const
     {
-        const double2 a = xy[m];
-        x = a.x; y = a.y; zz = z[m];
+        x = p[m].x; y = p[m].y; zz = p[m].z;
     }
     __device__ __forceinline__ void load(int m, double &x, double &y, double &zz, double &f) const
     {
-        pos(m, x, y, zz);
-        f = FP ? fp[m] : 0.0;
+        x = p[m].x; y = p[m].y; zz = p[m].z;
+        if constexpr (FP) f = p[m].w;
+        else f = 0.0;
     }
 };
 template <bool FP> struct Tile<float, FP>
@@ -248,6 +248,69 @@ static __device__ __forceinline__ void pair_hit(const KDesc &D, const PairParam 
     }
 }
 
+/* Single-element EAM: the set bits of a hit word, two per iteration and
+   without branches, so that the coefficient gathers of both pairs are in
+   flight together.  A slot without a pair (or outside the pair cutoff)
+   evaluates interval 0 and contributes an exact zero. */
+template <typename R, int PASS, typename TileT, typename A>
+static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T, unsigned long long word, int s,
+                                                  int g0, int S, R xi, R yi, R zi, R Fpi, R rc2, A *acc)
+{
+    typedef typename Prec<R>::Q Q;
+    const EamDev &E = D.eam;
+    const R inv = (R)E.inv_dr2;
+    const int kmax = E.nr2 - 2;
+    const Q *rp = Prec<R>::rp(E);
+    while (__ballot(word != 0ull))
+    {
+        const bool v1 = word != 0ull;
+        const int t1 = v1 ? __builtin_ctzll(word) : 0;
+        word &= word - 1ull;
+        const bool v2 = word != 0ull;
+        const int t2 = v2 ? __builtin_ctzll(word) : 0;
+        word &= word - 1ull;
+        const int m1 = s + (g0 + t1) * S, m2 = s + (g0 + t2) * S;
+        R x1, y1, z1, f1, x2, y2, z2, f2;
+        T.load(m1, x1, y1, z1, f1);
+        T.load(m2, x2, y2, z2, f2);
+        const R dx1 = xi - x1, dy1 = yi - y1, dz1 = zi - z1;
+        const R dx2 = xi - x2, dy2 = yi - y2, dz2 = zi - z2;
+        R r21 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
+        R r22 = dx2 * dx2 + dy2 * dy2 + dz2 * dz2;
+        const bool u1 = v1 && r21 < rc2, u2 = v2 && r22 < rc2;
+        r21 = u1 ? r21 : (R)0;
+        r22 = u2 ? r22 : (R)0;
+        const R tt1 = r21 * inv, tt2 = r22 * inv;
+        const int k1 = min((int)tt1, kmax), k2 = min((int)tt2, kmax);
+        const R b1 = tt1 - (R)k1, b2 = tt2 - (R)k2;
+        if (PASS == 1)
+        {
+            const Q q1 = rp[2 * (long)k1], q2 = rp[2 * (long)k2];
+            const R c1 = sp_val<R, Q>(q1, b1), c2 = sp_val<R, Q>(q2, b2);
+            acc[3].add(u1 ? c1 : (R)0);
+            acc[3].add(u2 ? c2 : (R)0);
+        }
+        else
+        {
+            const Q qr1 = rp[2 * (long)k1], qp1 = rp[2 * (long)k1 + 1];
+            const Q qr2 = rp[2 * (long)k2], qp2 = rp[2 * (long)k2 + 1];
+            const R rho1 = sp_der<R, Q>(qr1, b1, inv), rho2 = sp_der<R, Q>(qr2, b2, inv);
+            const R ph1 = sp_der<R, Q>(qp1, b1, inv), ph2 = sp_der<R, Q>(qp2, b2, inv);
+            const R mag1 = (R)2 * (Fpi * rho1 + f1 * rho1 + ph1);
+            const R mag2 = (R)2 * (Fpi * rho2 + f2 * rho2 + ph2);
+            const R e1 = (R)0.5 * sp_val<R, Q>(qp1, b1), e2 = (R)0.5 * sp_val<R, Q>(qp2, b2);
+            acc[0].add(u1 ? -mag1 * dx1 : (R)0);
+            acc[1].add(u1 ? -mag1 * dy1 : (R)0);
+            acc[2].add(u1 ? -mag1 * dz1 : (R)0);
+            acc[3].add(u1 ? e1 : (R)0);
+            acc[0].add(u2 ? -mag2 * dx2 : (R)0);
+            acc[1].add(u2 ? -mag2 * dy2 : (R)0);
+            acc[2].add(u2 ? -mag2 * dz2 : (R)0);
+            acc[3].add(u2 ? e2 : (R)0);
+        }
+    }
+}
+
 template <typename R, int PASS, int CAT>
 static __device__ void b_cell(const KDesc &D, const int c)
 {
@@ -310,10 +373,10 @@ static __device__ void b_cell(const KDesc &D, const int c)
     }
     if (lane < 27) nb_pre[lane + 1] = incl;
     if (lane == 0) nb_pre[0] = 0;
-    const int ncand = __shfl(incl, 26);
+    const int ncand = __builtin_amdgcn_readfirstlane(__shfl(incl, 26));
     __syncthreads();
 
-    const int nh = D.cnt[c];
+    const int nh = __builtin_amdgcn_readfirstlane(D.cnt[c]);
     const int ag = D.ag;
     const R rcut2 = (R)D.rcmax2;
     const R qnan = (R)__builtin_nan("");
@@ -389,43 +452,52 @@ static __device__ void b_cell(const KDesc &D, const int c)
                     word = wi < hw ? D.hbits[((long)c * HBITS_WORDS + wi) * WG + lane] : 0ull;
                 else
                 {
+                    /* bits are shifted in from the last trip down, so trip t lands on bit t */
                     unsigned lo = 0, hi = 0;
-                    int m = s + g0 * S;
                     const int n0 = min(ng, 32);
-                    for (int t = 0; t < n0; t++, m += S)
+                    int m = s + (g0 + ng - 1) * S;
+#pragma unroll 4
+                    for (int t = ng - 1; t >= 32; t--, m -= S)
                     {
                         R x, y, z;
                         T.pos(m, x, y, z);
                         const R dx = xi - x, dy = yi - y, dz = zi - z;
                         const R r2 = dx * dx + dy * dy + dz * dz;
-                        if (r2 < rcut2 && m != selft) lo |= 1u << t;
+                        hi = (hi << 1) | (r2 < rcut2 ? 1u : 0u);
                     }
-                    for (int t = 32; t < ng; t++, m += S)
+#pragma unroll 4
+                    for (int t = n0 - 1; t >= 0; t--, m -= S)
                     {
                         R x, y, z;
                         T.pos(m, x, y, z);
                         const R dx = xi - x, dy = yi - y, dz = zi - z;
                         const R r2 = dx * dx + dy * dy + dz * dz;
-                        if (r2 < rcut2 && m != selft) hi |= 1u << (t - 32);
+                        lo = (lo << 1) | (r2 < rcut2 ? 1u : 0u);
                     }
                     word = ((unsigned long long)hi << 32) | lo;
+                    /* the home atom meets itself on one trip of one stripe: clear that bit */
+                    const int ds = selft - s - g0 * S;
+                    if (ds >= 0 && ds < ng * S && ds % S == 0) word &= ~(1ull << (ds / S));
                     if (record && wi < HBITS_WORDS) D.hbits[((long)c * HBITS_WORDS + wi) * WG + lane] = word;
                 }
-                while (__ballot(word != 0ull))
-                {
-                    if (word)
+                if constexpr (CAT == CAT_EAM1)
+                    flush_eam1<R, PASS>(D, T, word, s, g0, S, xi, yi, zi, Fpi, (R)p1.rc2, acc);
+                else
+                    while (__ballot(word != 0ull))
                     {
-                        const int t = __builtin_ctzll(word);
-                        word &= word - 1ull;
-                        const int m = s + (g0 + t) * S;
-                        R x, y, z, fj;
-                        T.load(m, x, y, z, fj);
-                        if (CAT == CAT_GEN)
-                            pair_hit<R, PASS, CAT>(D, ppi[T.k[m]], Fpi, fj, xi - x, yi - y, zi - z, acc);
-                        else
-                            pair_hit<R, PASS, CAT>(D, p1, Fpi, fj, xi - x, yi - y, zi - z, acc);
+                        if (word)
+                        {
+                            const int t = __builtin_ctzll(word);
+                            word &= word - 1ull;
+                            const int m = s + (g0 + t) * S;
+                            R x, y, z, fj;
+                            T.load(m, x, y, z, fj);
+                            if (CAT == CAT_GEN)
+                                pair_hit<R, PASS, CAT>(D, ppi[T.k[m]], Fpi, fj, xi - x, yi - y, zi - z, acc);
+                            else
+                                pair_hit<R, PASS, CAT>(D, p1, Fpi, fj, xi - x, yi - y, zi - z, acc);
+                        }
                     }
-                }
                 wi++;
             }
         }
