@@ -114,7 +114,7 @@ static int alloc_slots(KDesc &D, int cap, unsigned nmov)
     r |= dalloc(&D.mvx, nmov); r |= dalloc(&D.mvy, nmov); r |= dalloc(&D.mvz, nmov);
     r |= dalloc(&D.mvvx, nmov); r |= dalloc(&D.mvvy, nmov); r |= dalloc(&D.mvvz, nmov);
     r |= dalloc(&D.mvgrp, nmov); r |= dalloc(&D.mvaid, nmov); r |= dalloc(&D.mvkind, nmov);
-    size_t nblk = (ns + 255) / 256 + 1;
+    size_t nblk = ((size_t)D.ncell + CPB - 1) / CPB + 1;
     r |= dalloc(&D.part, nblk * 8);
     if (r) { free_slots(D); return FMDX_ENOMEM; }
     return FMDX_OK;
@@ -158,6 +158,7 @@ extern "C" int fmdx_system_create(fmdx_system_t **out, const double l[3], const 
     r |= dalloc(&D.counters, 8);
     r |= dalloc(&D.cell_pe, ncell); r |= dalloc(&D.cell_emb, ncell);
     r |= dalloc(&D.hbits, (size_t)ncell * HBITS_WORDS * 64); r |= dalloc(&D.htrips, ncell);
+    r |= dalloc(&D.epart, 2 * NPART_MAX);
     r |= dalloc(&D.thermo, 16);
     r |= dalloc(&s->dpairs, MAXK * MAXK);
     if (r) { fmdx_system_destroy(s); return FMDX_ENOMEM; }
@@ -189,7 +190,7 @@ extern "C" void fmdx_system_destroy(fmdx_system_t *s)
     free_slots(D);
     dfree(D.cnt); dfree(D.nleave); dfree(D.narr); dfree(D.nearly); dfree(D.fill);
     dfree(D.counters); dfree(D.cell_pe); dfree(D.cell_emb); dfree(D.thermo);
-    dfree(D.hbits); dfree(D.htrips);
+    dfree(D.hbits); dfree(D.htrips); dfree(D.epart);
     dfree(s->dpairs);
     dfree(s->Fc); dfree(s->rhoc); dfree(s->phic);
     dfree(s->Fc32); dfree(s->rhoc32); dfree(s->phic32);

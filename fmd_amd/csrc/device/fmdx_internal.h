@@ -102,7 +102,8 @@ struct KDesc
     /* reductions */
     double *cell_pe;       /* [ncell] pair energy per home cell */
     double *cell_emb;      /* [ncell] embedding energy per home cell */
-    double *part;          /* [NPART_MAX*8] per-block thermo partials */
+    double *part;          /* [blocks*8] per-block thermo partials */
+    double *epart;         /* [NPART_MAX*2] per-block energy partials */
     double *thermo;        /* fmdx_thermo_t mirror, doubles first */
     /* step parameters, set by the host before a launch */
     int ag;                /* active group, -1 = every atom */
@@ -143,6 +144,23 @@ enum
 static __device__ __forceinline__ void set_err(const KDesc &D, unsigned bits)
 {
     atomicOr(&D.counters[3], bits);
+}
+
+/* The O(N) kernels map a block of TPB = 256 threads onto CPB = 16 consecutive
+   cells, 16 lanes per cell; lane l visits the occupied slots l, l+16, ... of
+   its cell, so only occupied slots are touched and each cell's slots are read
+   as one contiguous segment. */
+#define TPB 256
+#define CPB 16
+#define LPC (TPB / CPB)
+
+template <class F>
+static __device__ __forceinline__ void each_slot(const KDesc &D, int blk, F f)
+{
+    const int c = blk * CPB + (int)threadIdx.x / LPC;
+    if (c >= D.ncell) return;
+    const int n = D.cnt[c];
+    for (int k = (int)threadIdx.x % LPC; k < n; k += LPC) f((long)c * D.cap + k, c);
 }
 
 /* work lists of a batch of systems (device arrays) */

@@ -590,15 +590,12 @@ __global__ __launch_bounds__(WG) void kb_cell(BatchMap M)
 /* FP32 positions relative to the origin of the home cell, from the FP64 positions */
 static __device__ void b_relpos(const KDesc &D, int blk)
 {
-    long slot = (long)blk * 256 + threadIdx.x;
-    if (slot >= (long)D.ncell * D.cap) return;
-    int c = (int)(slot / D.cap);
-    int k = (int)(slot - (long)c * D.cap);
-    if (k >= D.cnt[c]) return;
-    int cz = c % D.nc2, t = c / D.nc2, cy = t % D.nc1, cx = t / D.nc1;
-    D.xr[slot] = (float)(D.x[slot] - cx * D.h0);
-    D.yr[slot] = (float)(D.y[slot] - cy * D.h1);
-    D.zr[slot] = (float)(D.z[slot] - cz * D.h2);
+    each_slot(D, blk, [&](long slot, int c) {
+        const int cz = c % D.nc2, t = c / D.nc2, cy = t % D.nc1, cx = t / D.nc1;
+        D.xr[slot] = (float)(D.x[slot] - cx * D.h0);
+        D.yr[slot] = (float)(D.y[slot] - cy * D.h1);
+        D.zr[slot] = (float)(D.z[slot] - cz * D.h2);
+    });
 }
 __global__ __launch_bounds__(256) void k_relpos(KDesc D) { b_relpos(D, blockIdx.x); }
 __global__ __launch_bounds__(256) void kb_relpos(BatchMap M)
@@ -638,8 +635,7 @@ hipError_t launch_pass1(const KDesc &D, hipStream_t st, int precision)
 {
     if (precision == FMDX_PREC_MIXED)
     {
-        unsigned g = (unsigned)(((long)D.ncell * D.cap + 255) / 256);
-        hipLaunchKernelGGL(k_relpos, dim3(g), dim3(256), 0, st, D);
+        hipLaunchKernelGGL(k_relpos, dim3((D.ncell + CPB - 1) / CPB), dim3(TPB), 0, st, D);
         if (D.has_eam) launch_cell<float, 1>(D, st);
         else (void)launch_fill(D.cell_emb, D.ncell, 0.0, st);
     }

@@ -29,7 +29,6 @@
  */
 #include "fmdx_internal.h"
 
-#define TPB 256
 
 #define DEF_KERNELS(NAME)                                                             \
     __global__ __launch_bounds__(TPB) void k_##NAME(KDesc D) { b_##NAME(D, blockIdx.x); } \
@@ -103,52 +102,37 @@ static __device__ void bin_flag(const KDesc &D, int c, long slot, double x, doub
     if (code == 0 && c < dest) atomicAdd(&D.nearly[dest], 1);
 }
 
-/* slot of thread threadIdx.x in logical block blk; false when it holds no atom */
-static __device__ __forceinline__ bool slot_of(const KDesc &D, int blk, long &slot, int &c)
-{
-    slot = (long)blk * TPB + threadIdx.x;
-    if (slot >= (long)D.ncell * D.cap) return false;
-    c = (int)(slot / D.cap);
-    int k = (int)(slot - (long)c * D.cap);
-    return k < D.cnt[c];
-}
-
 /* VelocityVerlet_startStep (integrators.c:40-75) fused with the binning flag */
 static __device__ void b_vv_start(const KDesc &D, int blk)
 {
 #pragma clang fp contract(off)
     if (blk == 0 && threadIdx.x == 0) D.counters[0] = 0;
-    long slot;
-    int c;
-    if (!slot_of(D, blk, slot, c)) return;
-    double x = D.x[slot], y = D.y[slot], z = D.z[slot];
-    if (D.ag < 0 || D.grp[slot] == D.ag)
-    {
-        const double dt = D.dt;
-        double lambda = 1.0;
-        if (D.thermostat)
+    double lambda = 1.0;
+    if (D.thermostat) lambda = sqrt(1 + D.dt / D.tau * (D.T0 / D.thermo[TH_T] - 1));
+    each_slot(D, blk, [&](long slot, int c) {
+        double x = D.x[slot], y = D.y[slot], z = D.z[slot];
+        if (D.ag < 0 || D.grp[slot] == D.ag)
         {
-            double T = D.thermo[TH_T];
-            lambda = sqrt(1 + dt / D.tau * (D.T0 / T - 1));
+            const double dt = D.dt;
+            const double h = D.hdtm[D.kind[slot]];
+            double vx = D.vx[slot], vy = D.vy[slot], vz = D.vz[slot];
+            if (D.thermostat)
+            {
+                vx *= lambda;
+                vy *= lambda;
+                vz *= lambda;
+            }
+            vx += h * D.fx[slot];
+            x += dt * vx;
+            vy += h * D.fy[slot];
+            y += dt * vy;
+            vz += h * D.fz[slot];
+            z += dt * vz;
+            D.vx[slot] = vx; D.vy[slot] = vy; D.vz[slot] = vz;
+            D.x[slot] = x; D.y[slot] = y; D.z[slot] = z;
         }
-        double h = D.hdtm[D.kind[slot]];
-        double vx = D.vx[slot], vy = D.vy[slot], vz = D.vz[slot];
-        if (D.thermostat)
-        {
-            vx *= lambda;
-            vy *= lambda;
-            vz *= lambda;
-        }
-        vx += h * D.fx[slot];
-        x += dt * vx;
-        vy += h * D.fy[slot];
-        y += dt * vy;
-        vz += h * D.fz[slot];
-        z += dt * vz;
-        D.vx[slot] = vx; D.vy[slot] = vy; D.vz[slot] = vz;
-        D.x[slot] = x; D.y[slot] = y; D.z[slot] = z;
-    }
-    bin_flag(D, c, slot, x, y, z);
+        bin_flag(D, c, slot, x, y, z);
+    });
 }
 DEF_KERNELS(vv_start)
 
@@ -157,25 +141,24 @@ static __device__ void b_se_step(const KDesc &D, int blk)
 {
 #pragma clang fp contract(off)
     if (blk == 0 && threadIdx.x == 0) D.counters[0] = 0;
-    long slot;
-    int c;
-    if (!slot_of(D, blk, slot, c)) return;
-    double x = D.x[slot], y = D.y[slot], z = D.z[slot];
-    if (D.ag < 0 || D.grp[slot] == D.ag)
-    {
-        const double dt = D.dt;
-        double m = D.mass[D.kind[slot]];
-        double vx = D.vx[slot], vy = D.vy[slot], vz = D.vz[slot];
-        vx += dt * D.fx[slot] / m;
-        x += dt * vx;
-        vy += dt * D.fy[slot] / m;
-        y += dt * vy;
-        vz += dt * D.fz[slot] / m;
-        z += dt * vz;
-        D.vx[slot] = vx; D.vy[slot] = vy; D.vz[slot] = vz;
-        D.x[slot] = x; D.y[slot] = y; D.z[slot] = z;
-    }
-    bin_flag(D, c, slot, x, y, z);
+    each_slot(D, blk, [&](long slot, int c) {
+        double x = D.x[slot], y = D.y[slot], z = D.z[slot];
+        if (D.ag < 0 || D.grp[slot] == D.ag)
+        {
+            const double dt = D.dt;
+            const double m = D.mass[D.kind[slot]];
+            double vx = D.vx[slot], vy = D.vy[slot], vz = D.vz[slot];
+            vx += dt * D.fx[slot] / m;
+            x += dt * vx;
+            vy += dt * D.fy[slot] / m;
+            y += dt * vy;
+            vz += dt * D.fz[slot] / m;
+            z += dt * vz;
+            D.vx[slot] = vx; D.vy[slot] = vy; D.vz[slot] = vz;
+            D.x[slot] = x; D.y[slot] = y; D.z[slot] = z;
+        }
+        bin_flag(D, c, slot, x, y, z);
+    });
 }
 DEF_KERNELS(se_step)
 
@@ -183,10 +166,7 @@ DEF_KERNELS(se_step)
 static __device__ void b_flag_only(const KDesc &D, int blk)
 {
     if (blk == 0 && threadIdx.x == 0) D.counters[0] = 0;
-    long slot;
-    int c;
-    if (!slot_of(D, blk, slot, c)) return;
-    bin_flag(D, c, slot, D.x[slot], D.y[slot], D.z[slot]);
+    each_slot(D, blk, [&](long slot, int c) { bin_flag(D, c, slot, D.x[slot], D.y[slot], D.z[slot]); });
 }
 DEF_KERNELS(flag_only)
 
@@ -211,22 +191,23 @@ static __device__ void b_vv_finish(const KDesc &D, int blk)
 {
 #pragma clang fp contract(off)
     double q[5] = {0, 0, 0, 0, 0};
-    long slot;
-    int c;
-    if (slot_of(D, blk, slot, c) && (D.ag < 0 || D.grp[slot] == D.ag))
-    {
-        int kd = D.kind[slot];
-        double h = D.hdtm[kd], m = D.mass[kd];
-        double vx = D.vx[slot] + h * D.fx[slot];
-        double vy = D.vy[slot] + h * D.fy[slot];
-        double vz = D.vz[slot] + h * D.fz[slot];
-        D.vx[slot] = vx; D.vy[slot] = vy; D.vz[slot] = vz;
-        q[0] = m * (vx * vx + vy * vy + vz * vz);
-        q[1] = m * vx;
-        q[2] = m * vy;
-        q[3] = m * vz;
-        q[4] = 1.0;
-    }
+    each_slot(D, blk, [&](long slot, int c) {
+        (void)c;
+        if (D.ag < 0 || D.grp[slot] == D.ag)
+        {
+            const int kd = D.kind[slot];
+            const double h = D.hdtm[kd], m = D.mass[kd];
+            const double vx = D.vx[slot] + h * D.fx[slot];
+            const double vy = D.vy[slot] + h * D.fy[slot];
+            const double vz = D.vz[slot] + h * D.fz[slot];
+            D.vx[slot] = vx; D.vy[slot] = vy; D.vz[slot] = vz;
+            q[0] += m * (vx * vx + vy * vy + vz * vz);
+            q[1] += m * vx;
+            q[2] += m * vy;
+            q[3] += m * vz;
+            q[4] += 1.0;
+        }
+    });
     block_partials(D, blk, q);
 }
 DEF_KERNELS(vv_finish)
@@ -236,23 +217,24 @@ static __device__ void b_thermo(const KDesc &D, int blk)
 {
 #pragma clang fp contract(off)
     double q[5] = {0, 0, 0, 0, 0};
-    long slot;
-    int c;
-    if (slot_of(D, blk, slot, c) && (D.ag < 0 || D.grp[slot] == D.ag))
-    {
-        double m = D.mass[D.kind[slot]];
-        double vx = D.vx[slot], vy = D.vy[slot], vz = D.vz[slot];
-        q[0] = m * (vx * vx + vy * vy + vz * vz);
-        q[1] = m * vx;
-        q[2] = m * vy;
-        q[3] = m * vz;
-        q[4] = 1.0;
-    }
+    each_slot(D, blk, [&](long slot, int c) {
+        (void)c;
+        if (D.ag < 0 || D.grp[slot] == D.ag)
+        {
+            const double m = D.mass[D.kind[slot]];
+            const double vx = D.vx[slot], vy = D.vy[slot], vz = D.vz[slot];
+            q[0] += m * (vx * vx + vy * vy + vz * vz);
+            q[1] += m * vx;
+            q[2] += m * vy;
+            q[3] += m * vz;
+            q[4] += 1.0;
+        }
+    });
     block_partials(D, blk, q);
 }
 DEF_KERNELS(thermo)
 
-static __device__ __forceinline__ int nblk(const KDesc &D) { return (int)(((long)D.ncell * D.cap + TPB - 1) / TPB); }
+static __device__ __forceinline__ int nblk(const KDesc &D) { return (D.ncell + CPB - 1) / CPB; }
 
 /* fixed-order reduction of the per-block partials: updateGroupProperties, integrators.c:78-92 */
 static __device__ void b_reduce_thermo(const KDesc &D)
@@ -324,6 +306,72 @@ __global__ __launch_bounds__(1024) void kb_reduce_pe(const KDesc *descs)
 {
     const KDesc &D = descs[blockIdx.x];
     b_reduce_pe(D);
+}
+
+/* two-level fixed-order sum of the per-cell energies for one large system:
+   block b sums a contiguous run of cells, one block sums the block totals */
+static inline int pe_blocks(int ncell)
+{
+    const int nb = (ncell + 1023) / 1024;
+    return nb < 1 ? 1 : (nb > NPART_MAX ? NPART_MAX : nb);
+}
+
+__global__ __launch_bounds__(TPB) void k_pe_part(KDesc D)
+{
+    __shared__ double sh[2][TPB];
+    const int per = (D.ncell + gridDim.x - 1) / gridDim.x;
+    const int c0 = blockIdx.x * per, c1 = min(D.ncell, c0 + per);
+    double pe = 0, emb = 0;
+    for (int c = c0 + threadIdx.x; c < c1; c += TPB)
+    {
+        pe += D.cell_pe[c];
+        emb += D.cell_emb[c];
+    }
+    sh[0][threadIdx.x] = pe;
+    sh[1][threadIdx.x] = emb;
+    __syncthreads();
+    for (int s = TPB / 2; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s)
+        {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        D.epart[2 * blockIdx.x] = sh[0][0];
+        D.epart[2 * blockIdx.x + 1] = sh[1][0];
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_pe_final(KDesc D, int nb)
+{
+    __shared__ double sh[2][1024];
+    double pe = 0, emb = 0;
+    for (int b = threadIdx.x; b < nb; b += 1024)
+    {
+        pe += D.epart[2 * b];
+        emb += D.epart[2 * b + 1];
+    }
+    sh[0][threadIdx.x] = pe;
+    sh[1][threadIdx.x] = emb;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s)
+        {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        D.thermo[TH_PE] = sh[0][0] + sh[1][0];
+        D.thermo[TH_EMB] = sh[1][0];
+    }
 }
 
 /* ----------------------------------------------------------- binning */
@@ -469,60 +517,57 @@ DEF_KERNELS(bin_apply)
 __global__ __launch_bounds__(TPB) void k_add_velocity(KDesc D, int group, double vx, double vy, double vz)
 {
 #pragma clang fp contract(off)
-    long slot;
-    int c;
-    if (!slot_of(D, blockIdx.x, slot, c)) return;
-    if (group == -1 || D.grp[slot] == group)
-    {
-        D.vx[slot] += vx;
-        D.vy[slot] += vy;
-        D.vz[slot] += vz;
-    }
+    each_slot(D, blockIdx.x, [&](long slot, int c) {
+        (void)c;
+        if (group == -1 || D.grp[slot] == group)
+        {
+            D.vx[slot] += vx;
+            D.vy[slot] += vy;
+            D.vz[slot] += vz;
+        }
+    });
 }
 
 /* copy every cell into a layout with another slot capacity */
 __global__ __launch_bounds__(TPB) void k_relayout(KDesc A, KDesc B)
 {
-    long slot;
-    int c;
-    if (!slot_of(A, blockIdx.x, slot, c)) return;
-    int k = (int)(slot - (long)c * A.cap);
-    if (k >= B.cap) return;
-    long t = (long)c * B.cap + k;
-    B.x[t] = A.x[slot]; B.y[t] = A.y[slot]; B.z[t] = A.z[slot];
-    B.vx[t] = A.vx[slot]; B.vy[t] = A.vy[slot]; B.vz[t] = A.vz[slot];
-    B.fx[t] = A.fx[slot]; B.fy[t] = A.fy[slot]; B.fz[t] = A.fz[slot];
-    B.aux[t] = A.aux[slot];
-    B.grp[t] = A.grp[slot]; B.aid[t] = A.aid[slot]; B.kind[t] = A.kind[slot];
+    each_slot(A, blockIdx.x, [&](long slot, int c) {
+        const int k = (int)(slot - (long)c * A.cap);
+        if (k >= B.cap) return;
+        const long t = (long)c * B.cap + k;
+        B.x[t] = A.x[slot]; B.y[t] = A.y[slot]; B.z[t] = A.z[slot];
+        B.vx[t] = A.vx[slot]; B.vy[t] = A.vy[slot]; B.vz[t] = A.vz[slot];
+        B.fx[t] = A.fx[slot]; B.fy[t] = A.fy[slot]; B.fz[t] = A.fz[slot];
+        B.aux[t] = A.aux[slot];
+        B.grp[t] = A.grp[slot]; B.aid[t] = A.aid[slot]; B.kind[t] = A.kind[slot];
+    });
 }
 
 /* compact cell-ordered arrays -> slots (transfer_from_globalgrid_to_rank0_grid, matter.c:329-363) */
 __global__ __launch_bounds__(TPB) void k_scatter(KDesc D, const int *cell_start, const double *x, const double *v,
                                                   const int *grp, const unsigned *aid, const uint8_t *kind)
 {
-    long slot;
-    int c;
-    if (!slot_of(D, blockIdx.x, slot, c)) return;
-    long a = (long)cell_start[c] + (slot - (long)c * D.cap);
-    D.x[slot] = x[3 * a]; D.y[slot] = x[3 * a + 1]; D.z[slot] = x[3 * a + 2];
-    D.vx[slot] = v[3 * a]; D.vy[slot] = v[3 * a + 1]; D.vz[slot] = v[3 * a + 2];
-    D.fx[slot] = 0.0; D.fy[slot] = 0.0; D.fz[slot] = 0.0; D.aux[slot] = 0.0;
-    D.grp[slot] = grp[a]; D.aid[slot] = aid[a]; D.kind[slot] = kind[a];
+    each_slot(D, blockIdx.x, [&](long slot, int c) {
+        const long a = (long)cell_start[c] + (slot - (long)c * D.cap);
+        D.x[slot] = x[3 * a]; D.y[slot] = x[3 * a + 1]; D.z[slot] = x[3 * a + 2];
+        D.vx[slot] = v[3 * a]; D.vy[slot] = v[3 * a + 1]; D.vz[slot] = v[3 * a + 2];
+        D.fx[slot] = 0.0; D.fy[slot] = 0.0; D.fz[slot] = 0.0; D.aux[slot] = 0.0;
+        D.grp[slot] = grp[a]; D.aid[slot] = aid[a]; D.kind[slot] = kind[a];
+    });
 }
 
 /* slots -> compact cell-ordered arrays */
 __global__ __launch_bounds__(TPB) void k_gather(KDesc D, const int *cell_start, double *x, double *v, double *f,
                                                  double *aux, int *grp, unsigned *aid, uint8_t *kind)
 {
-    long slot;
-    int c;
-    if (!slot_of(D, blockIdx.x, slot, c)) return;
-    long a = (long)cell_start[c] + (slot - (long)c * D.cap);
-    x[3 * a] = D.x[slot]; x[3 * a + 1] = D.y[slot]; x[3 * a + 2] = D.z[slot];
-    v[3 * a] = D.vx[slot]; v[3 * a + 1] = D.vy[slot]; v[3 * a + 2] = D.vz[slot];
-    f[3 * a] = D.fx[slot]; f[3 * a + 1] = D.fy[slot]; f[3 * a + 2] = D.fz[slot];
-    aux[a] = D.aux[slot];
-    grp[a] = D.grp[slot]; aid[a] = D.aid[slot]; kind[a] = D.kind[slot];
+    each_slot(D, blockIdx.x, [&](long slot, int c) {
+        const long a = (long)cell_start[c] + (slot - (long)c * D.cap);
+        x[3 * a] = D.x[slot]; x[3 * a + 1] = D.y[slot]; x[3 * a + 2] = D.z[slot];
+        v[3 * a] = D.vx[slot]; v[3 * a + 1] = D.vy[slot]; v[3 * a + 2] = D.vz[slot];
+        f[3 * a] = D.fx[slot]; f[3 * a + 1] = D.fy[slot]; f[3 * a + 2] = D.fz[slot];
+        aux[a] = D.aux[slot];
+        grp[a] = D.grp[slot]; aid[a] = D.aid[slot]; kind[a] = D.kind[slot];
+    });
 }
 
 __global__ __launch_bounds__(TPB) void k_fill(double *p, long n, double v)
@@ -533,7 +578,7 @@ __global__ __launch_bounds__(TPB) void k_fill(double *p, long n, double v)
 
 /* ------------------------------------------------------------ launchers */
 
-static inline unsigned nblk_h(const KDesc &D) { return (unsigned)(((long)D.ncell * D.cap + TPB - 1) / TPB); }
+static inline unsigned nblk_h(const KDesc &D) { return (unsigned)((D.ncell + CPB - 1) / CPB); }
 static inline unsigned ncblk_h(const KDesc &D) { return (unsigned)((D.ncell + TPB - 1) / TPB); }
 
 hipError_t launch_bin(const KDesc &D, hipStream_t st)
@@ -579,7 +624,9 @@ hipError_t launch_thermo(const KDesc &D, hipStream_t st)
 
 hipError_t launch_reduce_pe(const KDesc &D, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_reduce_pe, dim3(1), dim3(1024), 0, st, D);
+    const int nb = pe_blocks(D.ncell);
+    hipLaunchKernelGGL(k_pe_part, dim3(nb), dim3(TPB), 0, st, D);
+    hipLaunchKernelGGL(k_pe_final, dim3(1), dim3(1024), 0, st, D, nb);
     return hipGetLastError();
 }
 
