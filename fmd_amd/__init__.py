@@ -106,6 +106,17 @@ def load_fmdx():
         lib.fmdx_device_count.argtypes = [C.POINTER(C.c_int)]
         lib.fmdx_strerror.argtypes = [C.c_int]
         lib.fmdx_strerror.restype = C.c_char_p
+        lib.fmdx_batch_create.argtypes = [C.POINTER(P), C.c_int]
+        lib.fmdx_batch_destroy.argtypes = [P]
+        lib.fmdx_batch_destroy.restype = None
+        lib.fmdx_batch_add.argtypes = [P, P, C.c_double, C.c_int, C.c_double, C.c_double]
+        lib.fmdx_batch_remove.argtypes = [P, P]
+        lib.fmdx_batch_size.argtypes = [P, C.POINTER(C.c_int)]
+        lib.fmdx_batch_use_graph.argtypes = [P, C.c_int]
+        lib.fmdx_batch_set_check_interval.argtypes = [P, C.c_int]
+        lib.fmdx_batch_step.argtypes = [P, C.c_int]
+        lib.fmdx_batch_status.argtypes = [P, C.POINTER(Thermo)]
+        lib.fmdx_batch_sync.argtypes = [P]
         _fmdx = lib
     return _fmdx
 
@@ -270,3 +281,58 @@ class DeviceSystem:
         _ck(self.lib.fmdx_ttm_download(self.h, _ptr(Te), _ptr(Ti), _ptr(xi), _ptr(vcm), _ptr(num), _ptr(mass)),
             "fmdx_ttm_download")
         return dict(Te=Te, Ti=Ti, xi=xi, vcm=vcm.reshape(-1, 3), num=num, mass=mass)
+
+
+def _handle(s):
+    """a system handle: DeviceSystem, or a raw fmdx_system_t pointer"""
+    return s.h if isinstance(s, DeviceSystem) else C.c_void_p(s if isinstance(s, int) else s.value)
+
+
+class Batch:
+    """Independent systems advanced together by the batch engine of libfmdx.so."""
+
+    def __init__(self, precision=FMDX_PREC_MIXED):
+        self.lib = load_fmdx()
+        self.h = C.c_void_p()
+        _ck(self.lib.fmdx_batch_create(C.byref(self.h), precision), "fmdx_batch_create")
+        self.members = []
+
+    def close(self):
+        if self.h:
+            self.lib.fmdx_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, s, dt, thermostat=0, tau=1.0, T0=0.0):
+        _ck(self.lib.fmdx_batch_add(self.h, _handle(s), dt, thermostat, tau, T0), "fmdx_batch_add")
+        self.members.append(s)
+
+    def remove(self, s):
+        _ck(self.lib.fmdx_batch_remove(self.h, _handle(s)), "fmdx_batch_remove")
+        self.members.remove(s)
+
+    def size(self):
+        n = C.c_int(0)
+        _ck(self.lib.fmdx_batch_size(self.h, C.byref(n)), "fmdx_batch_size")
+        return n.value
+
+    def use_graph(self, on):
+        _ck(self.lib.fmdx_batch_use_graph(self.h, 1 if on else 0), "fmdx_batch_use_graph")
+
+    def set_check_interval(self, steps):
+        _ck(self.lib.fmdx_batch_set_check_interval(self.h, steps), "fmdx_batch_set_check_interval")
+
+    def step(self, n=1):
+        _ck(self.lib.fmdx_batch_step(self.h, n), "fmdx_batch_step")
+
+    def status(self):
+        n = self.size()
+        out = (Thermo * max(n, 1))()
+        _ck(self.lib.fmdx_batch_status(self.h, out), "fmdx_batch_status")
+        return list(out)[:n]
+

@@ -9,59 +9,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <vector>
-#include "fmdx_internal.h"
-
-#define NPROF 64   /* force calls whose kernels are timed in profile mode */
-
-struct fmdx_system
-{
-    int dev;
-    hipStream_t st;
-    int precision;
-    KDesc D;
-    unsigned natoms_uploaded;
-    int uploaded;
-    double mass[MAXK];
-    PairParam hp[MAXK * MAXK];
-    PairParam *dpairs;
-    /* EAM */
-    double4 *Fc, *rhoc, *phic, *rp;
-    float4 *Fc32, *rhoc32, *phic32, *rp32;
-    /* TTM */
-    int ttm_on;
-    TtmState T;
-    void *ttm_mem[16];
-    int nttm_mem;
-    /* timing */
-    hipEvent_t ev[16];
-    int nev;
-    int profile;
-    hipEvent_t pev[NPROF * 3];
-    unsigned prof_n;
-};
-
-#define CK(call)                                   \
-    do                                             \
-    {                                              \
-        hipError_t e_ = (call);                    \
-        if (e_ != hipSuccess) return FMDX_EHIP;    \
-    } while (0)
-
-template <class T>
-static int dalloc(T **p, size_t n)
-{
-    *p = nullptr;
-    if (n == 0) n = 1;
-    if (hipMalloc((void **)p, n * sizeof(T)) != hipSuccess) return FMDX_ENOMEM;
-    return FMDX_OK;
-}
-
-template <class T>
-static void dfree(T *&p)
-{
-    if (p) hipFree((void *)p);
-    p = nullptr;
-}
+#include "fmdx_sys.h"
 
 extern "C" const char *fmdx_strerror(int code)
 {
@@ -510,17 +458,6 @@ extern "C" int fmdx_update_forces(fmdx_system_t *s, int active_group, int ttm_fr
     else
         CK(launch_forces(s->D, s->st, s->precision));
     return FMDX_OK;
-}
-
-/* md->timestep * 0.5 / mass, integrators.c:67, evaluated on the host with the same operations */
-static void set_step(KDesc &D, const double *mass, int ag, double dt, int thermostat, double tau, double T0)
-{
-    D.ag = ag;
-    D.dt = dt;
-    D.thermostat = thermostat;
-    D.tau = tau;
-    D.T0 = T0;
-    for (int k = 0; k < MAXK; k++) D.hdtm[k] = dt * 0.5 / mass[k];
 }
 
 extern "C" int fmdx_vv_start(fmdx_system_t *s, int active_group, double dt, int thermostat, double tau, double T0)

@@ -1,0 +1,381 @@
+/*
+ * fmdx_batch.hip -- many independent systems advanced together.
+ *
+ * The reference runs one system per fmd_t.  A batch keeps the device state of
+ * every member system where it already lives (each member is an uploaded
+ * fmdx_system_t) and replaces the per-system launches by one launch per
+ * kernel over a work list of (system, logical block) items:
+ *   slot items     (system, group of 16 cells)   integrators, relative positions
+ *   cell-block     (system, block of 256 cells)  exact-order re-binning
+ *   cell items     (system, cell)                pair kernels, one list per
+ *                                                pair category and pass
+ *   energy items   (system, partial block)       fixed-order energy sums
+ * Each item runs exactly the code and summation order of the single-system
+ * path, so a member's trajectory does not depend on what else is in the
+ * batch.  The lists are ordered so that the round-robin workgroup dispatch
+ * gives each XCD a contiguous run of items (neighbour cells share its L2).
+ * A whole velocity-Verlet step is captured once in a HIP graph and replayed,
+ * which leaves one host launch per step.
+ *
+ * Between steps the host reads every member's status word with one copy
+ * (kb_collect) to grow slot capacities before they can overflow and to
+ * report members whose device error word is set.
+ */
+#include <algorithm>
+#include <vector>
+#include "fmdx_sys.h"
+
+struct fmdx_batch
+{
+    int dev;
+    hipStream_t st;
+    int precision;
+    std::vector<fmdx_system_t *> sys;
+    KDesc *descs;
+    int descs_cap;
+    int2 *items;
+    size_t items_cap;
+    unsigned *dstatus;
+    double *dthermo;
+    int status_cap;
+    std::vector<unsigned> hstatus;
+    std::vector<double> hthermo;
+    BatchPlan plan;
+    int dirty;
+    int use_graph;
+    hipGraph_t graph;
+    hipGraphExec_t gexec;
+    int graph_ok;
+    int check_every;
+};
+
+/* item b of a list of n runs on XCD b % 8: give XCD x the contiguous run of
+   list positions x*q + min(x, r) ... (q = n / 8, r = n % 8) */
+static void xcd_order(std::vector<int2> &v)
+{
+    const size_t n = v.size();
+    if (n < 16) return;
+    std::vector<int2> o(n);
+    const size_t q = n / 8, r = n % 8;
+    for (size_t b = 0; b < n; b++)
+    {
+        const size_t x = b % 8, i = b / 8;
+        o[b] = v[x * q + std::min(x, r) + i];
+    }
+    v.swap(o);
+}
+
+static void graph_drop(fmdx_batch_t *b)
+{
+    if (b->graph_ok)
+    {
+        hipGraphExecDestroy(b->gexec);
+        hipGraphDestroy(b->graph);
+    }
+    b->graph_ok = 0;
+}
+
+static int upload_descs(fmdx_batch_t *b)
+{
+    const int n = (int)b->sys.size();
+    std::vector<KDesc> h(n > 0 ? n : 1);
+    for (int i = 0; i < n; i++) h[i] = b->sys[i]->D;
+    if (n) CK(hipMemcpyAsync(b->descs, h.data(), sizeof(KDesc) * n, hipMemcpyHostToDevice, b->st));
+    return FMDX_OK;
+}
+
+static int rebuild(fmdx_batch_t *b)
+{
+    const int n = (int)b->sys.size();
+    graph_drop(b);
+    if (n > b->descs_cap)
+    {
+        dfree(b->descs);
+        int cap = std::max(n, 2 * b->descs_cap);
+        if (dalloc(&b->descs, cap)) return FMDX_ENOMEM;
+        b->descs_cap = cap;
+    }
+    if (n > b->status_cap)
+    {
+        dfree(b->dstatus);
+        dfree(b->dthermo);
+        int cap = std::max(n, 2 * b->status_cap);
+        if (dalloc(&b->dstatus, (size_t)cap * 8) || dalloc(&b->dthermo, (size_t)cap * 16)) return FMDX_ENOMEM;
+        b->status_cap = cap;
+    }
+    std::vector<int2> slot, cblk, pe, c0[4], c1[4];
+    int maxk = 1;
+    for (int i = 0; i < n; i++)
+    {
+        const KDesc &D = b->sys[i]->D;
+        const int cat = force_category(D);
+        maxk = std::max(maxk, D.nkinds);
+        for (int g = 0; g < (D.ncell + CPB - 1) / CPB; g++) slot.push_back(make_int2(i, g));
+        for (int g = 0; g < (D.ncell + TPB - 1) / TPB; g++) cblk.push_back(make_int2(i, g));
+        for (int g = 0; g < pe_blocks(D.ncell); g++) pe.push_back(make_int2(i, g));
+        for (int c = 0; c < D.ncell; c++)
+        {
+            c0[cat].push_back(make_int2(i, c));
+            if (D.has_eam) c1[cat].push_back(make_int2(i, c));
+        }
+    }
+    std::vector<int2> *lists[11] = {&slot, &cblk, &pe, &c0[0], &c0[1], &c0[2], &c0[3], &c1[0], &c1[1], &c1[2], &c1[3]};
+    size_t total = 0;
+    for (auto *l : lists)
+    {
+        xcd_order(*l);
+        total += l->size();
+    }
+    if (total > b->items_cap)
+    {
+        dfree(b->items);
+        if (dalloc(&b->items, total)) return FMDX_ENOMEM;
+        b->items_cap = total;
+    }
+    std::vector<int2> all;
+    all.reserve(total);
+    const int2 *ptr[11];
+    for (int k = 0; k < 11; k++)
+    {
+        ptr[k] = b->items + all.size();
+        all.insert(all.end(), lists[k]->begin(), lists[k]->end());
+    }
+    if (total) CK(hipMemcpyAsync(b->items, all.data(), sizeof(int2) * total, hipMemcpyHostToDevice, b->st));
+    int r = upload_descs(b);
+    if (r) return r;
+    BatchPlan &P = b->plan;
+    P.descs = b->descs;
+    P.nsys = n;
+    P.slot_items = ptr[0]; P.n_slot = (int)slot.size();
+    P.cellblk_items = ptr[1]; P.n_cellblk = (int)cblk.size();
+    P.pe_items = ptr[2]; P.n_pe = (int)pe.size();
+    for (int c = 0; c < 4; c++)
+    {
+        P.cell0[c] = ptr[3 + c]; P.n_cell0[c] = (int)c0[c].size();
+        P.cell1[c] = ptr[7 + c]; P.n_cell1[c] = (int)c1[c].size();
+    }
+    P.max_nkinds = maxk;
+    b->hstatus.assign((size_t)std::max(n, 1) * 8, 0);
+    b->hthermo.assign((size_t)std::max(n, 1) * 16, 0.0);
+    CK(hipStreamSynchronize(b->st));
+    b->dirty = 0;
+    return FMDX_OK;
+}
+
+static hipError_t enqueue_step(fmdx_batch_t *b)
+{
+    hipError_t e = launchb_vv_start(b->plan, b->st);
+    if (e == hipSuccess) e = launchb_forces(b->plan, b->st, b->precision);
+    if (e == hipSuccess) e = launchb_vv_finish(b->plan, b->st);
+    return e;
+}
+
+static int capture(fmdx_batch_t *b)
+{
+    graph_drop(b);
+    CK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+    hipError_t e = enqueue_step(b);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(b->st, &g);
+    if (e != hipSuccess || e2 != hipSuccess)
+    {
+        if (g) hipGraphDestroy(g);
+        return FMDX_EHIP;
+    }
+    if (hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0) != hipSuccess)
+    {
+        hipGraphDestroy(g);
+        return FMDX_EHIP;
+    }
+    b->graph = g;
+    b->graph_ok = 1;
+    return FMDX_OK;
+}
+
+/* one copy of every member's status words and thermo record */
+static int collect(fmdx_batch_t *b)
+{
+    const int n = (int)b->sys.size();
+    if (!n) return FMDX_OK;
+    CK(launchb_collect(b->plan, b->st, b->dstatus, b->dthermo));
+    CK(hipMemcpyAsync(b->hstatus.data(), b->dstatus, sizeof(unsigned) * 8 * n, hipMemcpyDeviceToHost, b->st));
+    CK(hipMemcpyAsync(b->hthermo.data(), b->dthermo, sizeof(double) * 16 * n, hipMemcpyDeviceToHost, b->st));
+    CK(hipStreamSynchronize(b->st));
+    return FMDX_OK;
+}
+
+/* grow the slot capacity of members whose fullest cell nears it */
+static int grow(fmdx_batch_t *b)
+{
+    int changed = 0;
+    for (size_t i = 0; i < b->sys.size(); i++)
+    {
+        fmdx_system_t *s = b->sys[i];
+        const unsigned maxc = b->hstatus[i * 8 + 2];
+        const unsigned cap = (unsigned)s->D.cap;
+        if (cap >= 256 || maxc + 8 + maxc / 4 <= cap) continue;
+        unsigned nc = maxc + maxc / 2 + 8;
+        nc = (nc + 7) & ~7u;
+        if (nc > 256) nc = 256;
+        int r = fmdx_set_capacity(s, nc);
+        if (r) return r;
+        changed = 1;
+    }
+    if (changed)
+    {
+        /* new slot arrays: the captured graph reads descriptors from device
+           memory, so refreshing them is enough */
+        int r = upload_descs(b);
+        if (r) return r;
+        CK(hipStreamSynchronize(b->st));
+    }
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_batch_create(fmdx_batch_t **out, int precision)
+{
+    if (!out || (precision != FMDX_PREC_FP64 && precision != FMDX_PREC_MIXED)) return FMDX_EARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return FMDX_ENODEV;
+    fmdx_batch_t *b = new fmdx_batch_t();
+    CK(hipGetDevice(&b->dev));
+    if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess)
+    {
+        delete b;
+        return FMDX_EHIP;
+    }
+    b->precision = precision;
+    b->use_graph = 1;
+    b->check_every = 8;
+    b->dirty = 1;
+    *out = b;
+    return FMDX_OK;
+}
+
+extern "C" void fmdx_batch_destroy(fmdx_batch_t *b)
+{
+    if (!b) return;
+    hipStreamSynchronize(b->st);
+    graph_drop(b);
+    dfree(b->descs);
+    dfree(b->items);
+    dfree(b->dstatus);
+    dfree(b->dthermo);
+    hipStreamDestroy(b->st);
+    delete b;
+}
+
+/* Join a batch.  The member's forces and thermo record are brought up to date
+   on its own stream first, as the reference's integrate loop does before its
+   first step (integrators.c:232). */
+extern "C" int fmdx_batch_add(fmdx_batch_t *b, fmdx_system_t *s, double dt, int thermostat, double tau, double T0)
+{
+    if (!b || !s) return FMDX_EARG;
+    if (!s->uploaded) return FMDX_ESTATE;
+    if (s->ttm_on) return FMDX_EARG;   /* TTM members are not batched */
+    for (auto *t : b->sys)
+        if (t == s) return FMDX_EARG;
+    int r = fmdx_set_precision(s, b->precision);
+    if (r) return r;
+    r = fmdx_update_forces(s, -1, 0);
+    if (r) return r;
+    r = fmdx_thermo(s, -1);
+    if (r) return r;
+    CK(hipStreamSynchronize(s->st));
+    set_step(s->D, s->mass, -1, dt, thermostat, tau, T0);
+    b->sys.push_back(s);
+    b->dirty = 1;
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_batch_remove(fmdx_batch_t *b, fmdx_system_t *s)
+{
+    if (!b || !s) return FMDX_EARG;
+    auto it = std::find(b->sys.begin(), b->sys.end(), s);
+    if (it == b->sys.end()) return FMDX_EARG;
+    CK(hipStreamSynchronize(b->st));
+    b->sys.erase(it);
+    b->dirty = 1;
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_batch_size(fmdx_batch_t *b, int *n)
+{
+    if (!b || !n) return FMDX_EARG;
+    *n = (int)b->sys.size();
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_batch_use_graph(fmdx_batch_t *b, int on)
+{
+    if (!b) return FMDX_EARG;
+    b->use_graph = on ? 1 : 0;
+    if (!on) graph_drop(b);
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_batch_set_check_interval(fmdx_batch_t *b, int steps)
+{
+    if (!b || steps < 1) return FMDX_EARG;
+    b->check_every = steps;
+    return FMDX_OK;
+}
+
+/* Velocity-Verlet steps of every member (VelocityVerlet_takeOneStep,
+   integrators.c:181-211, with each member's own dt and thermostat). */
+extern "C" int fmdx_batch_step(fmdx_batch_t *b, int nsteps)
+{
+    if (!b || nsteps < 0) return FMDX_EARG;
+    int r;
+    if (b->dirty && (r = rebuild(b))) return r;
+    if (b->sys.empty()) return FMDX_OK;
+    if (b->use_graph && !b->graph_ok && (r = capture(b))) return r;
+    for (int k = 0; k < nsteps; k++)
+    {
+        if (b->use_graph) CK(hipGraphLaunch(b->gexec, b->st));
+        else CK(enqueue_step(b));
+        if ((k + 1) % b->check_every == 0 || k + 1 == nsteps)
+        {
+            if ((r = collect(b))) return r;
+            if ((r = grow(b))) return r;
+        }
+    }
+    return FMDX_OK;
+}
+
+/* thermo record and status of every member, in join order */
+extern "C" int fmdx_batch_status(fmdx_batch_t *b, fmdx_thermo_t *out)
+{
+    if (!b || !out) return FMDX_EARG;
+    int r;
+    if (b->dirty && (r = rebuild(b))) return r;
+    if ((r = collect(b))) return r;
+    for (size_t i = 0; i < b->sys.size(); i++)
+    {
+        const double *th = &b->hthermo[i * 16];
+        const unsigned *ct = &b->hstatus[i * 8];
+        fmdx_thermo_t &o = out[i];
+        o.pe = th[TH_PE];
+        o.embed = th[TH_EMB];
+        o.mv2 = th[TH_MV2];
+        o.ke = th[TH_KE];
+        o.temperature = th[TH_T];
+        o.momentum[0] = th[TH_PX];
+        o.momentum[1] = th[TH_PY];
+        o.momentum[2] = th[TH_PZ];
+        o.n_active = th[TH_N];
+        o.natoms = b->sys[i]->natoms_uploaded - ct[1];
+        o.max_cell_count = ct[2];
+        o.err = ct[3];
+        o.cap = (unsigned)b->sys[i]->D.cap;
+    }
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_batch_sync(fmdx_batch_t *b)
+{
+    if (!b) return FMDX_EARG;
+    CK(hipStreamSynchronize(b->st));
+    return FMDX_OK;
+}

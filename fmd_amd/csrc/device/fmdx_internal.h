@@ -163,17 +163,28 @@ static __device__ __forceinline__ void each_slot(const KDesc &D, int blk, F f)
     for (int k = (int)threadIdx.x % LPC; k < n; k += LPC) f((long)c * D.cap + k, c);
 }
 
-/* work lists of a batch of systems (device arrays) */
+/* blocks of the first level of the fixed-order energy sum of a system */
+static __host__ __device__ inline int pe_blocks(int ncell)
+{
+    const int nb = (ncell + 1023) / 1024;
+    return nb < 1 ? 1 : (nb > NPART_MAX ? NPART_MAX : nb);
+}
+
+/* work lists of a batch of systems (device arrays of (system, logical block)) */
 struct BatchPlan
 {
-    const KDesc *descs;      /* [nsys] */
+    const KDesc *descs;          /* [nsys] */
     int nsys;
-    const int2 *slot_items;  /* (system, block of 256 slots) */
+    const int2 *slot_items;      /* (system, group of CPB cells): O(N) kernels */
     int n_slot;
-    const int2 *cellblk_items; /* (system, block of 256 cells) */
+    const int2 *cellblk_items;   /* (system, block of TPB cells): binning */
     int n_cellblk;
-    const int2 *cell_items;  /* (system, cell), one wavefront each */
-    int n_cell;
+    const int2 *pe_items;        /* (system, energy partial block) */
+    int n_pe;
+    const int2 *cell0[4];        /* (system, cell) for the force pass, per pair category */
+    int n_cell0[4];
+    const int2 *cell1[4];        /* (system, cell) for the EAM density pass, per pair category */
+    int n_cell1[4];
     int max_nkinds;
 };
 
@@ -194,6 +205,7 @@ hipError_t launchb_vv_finish(const BatchPlan &P, hipStream_t st);
 hipError_t launchb_thermo(const BatchPlan &P, hipStream_t st);
 hipError_t launchb_reduce_pe(const BatchPlan &P, hipStream_t st);
 hipError_t launchb_forces(const BatchPlan &P, hipStream_t st, int precision);
+hipError_t launchb_collect(const BatchPlan &P, hipStream_t st, unsigned *status, double *thermo);
 hipError_t launch_add_velocity(const KDesc &D, hipStream_t st, int group, double vx, double vy, double vz);
 hipError_t launch_relayout(const KDesc &Dold, const KDesc &Dnew, hipStream_t st);
 hipError_t launch_scatter(const KDesc &D, hipStream_t st, const int *cell_start, const double *x, const double *v,

@@ -1,0 +1,77 @@
+/*
+ * fmdx_sys.h -- host-side state of one device system, shared by the C-ABI
+ * translation units (single-system entry points and the batch engine).
+ */
+#ifndef FMDX_SYS_H
+#define FMDX_SYS_H
+
+#include "fmdx_internal.h"
+
+#define NPROF 64   /* force calls whose kernels are timed in profile mode */
+
+struct fmdx_system
+{
+    int dev;
+    hipStream_t st;
+    int precision;
+    KDesc D;
+    unsigned natoms_uploaded;
+    int uploaded;
+    double mass[MAXK];
+    PairParam hp[MAXK * MAXK];
+    PairParam *dpairs;
+    /* EAM */
+    double4 *Fc, *rhoc, *phic, *rp;
+    float4 *Fc32, *rhoc32, *phic32, *rp32;
+    /* TTM */
+    int ttm_on;
+    TtmState T;
+    void *ttm_mem[16];
+    int nttm_mem;
+    /* timing */
+    hipEvent_t ev[16];
+    int nev;
+    int profile;
+    hipEvent_t pev[NPROF * 3];
+    unsigned prof_n;
+};
+
+#define CK(call)                                   \
+    do                                             \
+    {                                              \
+        hipError_t e_ = (call);                    \
+        if (e_ != hipSuccess) return FMDX_EHIP;    \
+    } while (0)
+
+template <class T>
+static inline int dalloc(T **p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    if (hipMalloc((void **)p, n * sizeof(T)) != hipSuccess) return FMDX_ENOMEM;
+    return FMDX_OK;
+}
+
+template <class T>
+static inline void dfree(T *&p)
+{
+    if (p) hipFree((void *)p);
+    p = nullptr;
+}
+
+/* step parameters of a system; md->timestep * 0.5 / mass (integrators.c:67)
+   is evaluated on the host with the same operations */
+static inline void set_step(KDesc &D, const double *mass, int ag, double dt, int thermostat, double tau, double T0)
+{
+    D.ag = ag;
+    D.dt = dt;
+    D.thermostat = thermostat;
+    D.tau = tau;
+    D.T0 = T0;
+    for (int k = 0; k < MAXK; k++) D.hdtm[k] = dt * 0.5 / mass[k];
+}
+
+/* pair-category specialisation of the force kernels for a system (k_force.hip) */
+int force_category(const KDesc &D);
+
+#endif /* FMDX_SYS_H */

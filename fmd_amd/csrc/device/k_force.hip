@@ -579,14 +579,13 @@ static __device__ __forceinline__ int xcd_cell(int b, int n)
 template <typename R, int PASS, int CAT>
 __global__ __launch_bounds__(WG) void k_cell(KDesc D) { b_cell<R, PASS, CAT>(D, xcd_cell(blockIdx.x, gridDim.x)); }
 
-template <typename R, int PASS>
+template <typename R, int PASS, int CAT>
 __global__ __launch_bounds__(WG) void kb_cell(BatchMap M)
 {
     const int2 it = M.items[blockIdx.x];
     const KDesc &D = M.descs[it.x];
-    b_cell<R, PASS, CAT_GEN>(D, it.y);
+    b_cell<R, PASS, CAT>(D, it.y);
 }
-
 /* FP32 positions relative to the origin of the home cell, from the FP64 positions */
 static __device__ void b_relpos(const KDesc &D, int blk)
 {
@@ -607,7 +606,7 @@ __global__ __launch_bounds__(256) void kb_relpos(BatchMap M)
 
 static inline size_t pp_bytes(const KDesc &D) { return sizeof(PairParam) * D.nkinds * D.nkinds; }
 
-static int pick_cat(const KDesc &D)
+int force_category(const KDesc &D)
 {
     if (D.single_cat == FMDX_POT_EAM && D.eam.ne == 1 && D.eam.rp) return CAT_EAM1;
     if (D.nkinds == 1 && D.single_cat == FMDX_POT_LJ) return CAT_LJ;
@@ -620,7 +619,7 @@ static void launch_cell(const KDesc &D, hipStream_t st)
 {
     const dim3 g(D.ncell), b(WG);
     const size_t lds = pp_bytes(D);
-    switch (pick_cat(D))
+    switch (force_category(D))
     {
         case CAT_EAM1: hipLaunchKernelGGL((k_cell<R, PASS, CAT_EAM1>), g, b, lds, st, D); break;
         case CAT_LJ: hipLaunchKernelGGL((k_cell<R, PASS, CAT_LJ>), g, b, lds, st, D); break;
@@ -663,21 +662,41 @@ hipError_t launch_forces(const KDesc &D, hipStream_t st, int precision)
     return launch_reduce_pe(D, st);
 }
 
-/* every system of a batch: relative positions (mixed), pass 1, pass 0, energies */
+template <typename R, int PASS>
+static void launchb_cells(const BatchPlan &P, hipStream_t st, const int2 *const *lists, const int *n)
+{
+    const size_t ppb = sizeof(PairParam) * P.max_nkinds * P.max_nkinds;
+    for (int cat = 0; cat < 4; cat++)
+    {
+        if (n[cat] == 0) continue;
+        BatchMap mc = {P.descs, lists[cat]};
+        const dim3 g(n[cat]), b(WG);
+        switch (cat)
+        {
+            case CAT_EAM1: hipLaunchKernelGGL((kb_cell<R, PASS, CAT_EAM1>), g, b, ppb, st, mc); break;
+            case CAT_LJ: hipLaunchKernelGGL((kb_cell<R, PASS, CAT_LJ>), g, b, ppb, st, mc); break;
+            case CAT_MORSE: hipLaunchKernelGGL((kb_cell<R, PASS, CAT_MORSE>), g, b, ppb, st, mc); break;
+            default: hipLaunchKernelGGL((kb_cell<R, PASS, CAT_GEN>), g, b, ppb, st, mc); break;
+        }
+    }
+}
+
+/* every system of a batch: relative positions (mixed), the density pass over
+   the EAM systems, the force pass, energies; one launch per pair category */
 hipError_t launchb_forces(const BatchPlan &P, hipStream_t st, int precision)
 {
-    BatchMap ms = {P.descs, P.slot_items}, mc = {P.descs, P.cell_items};
-    const size_t ppb = sizeof(PairParam) * P.max_nkinds * P.max_nkinds;
+    if (P.nsys == 0) return hipSuccess;
     if (precision == FMDX_PREC_MIXED)
     {
-        hipLaunchKernelGGL(kb_relpos, dim3(P.n_slot), dim3(256), 0, st, ms);
-        hipLaunchKernelGGL((kb_cell<float, 1>), dim3(P.n_cell), dim3(WG), ppb, st, mc);
-        hipLaunchKernelGGL((kb_cell<float, 0>), dim3(P.n_cell), dim3(WG), ppb, st, mc);
+        BatchMap ms = {P.descs, P.slot_items};
+        hipLaunchKernelGGL(kb_relpos, dim3(P.n_slot), dim3(TPB), 0, st, ms);
+        launchb_cells<float, 1>(P, st, P.cell1, P.n_cell1);
+        launchb_cells<float, 0>(P, st, P.cell0, P.n_cell0);
     }
     else
     {
-        hipLaunchKernelGGL((kb_cell<double, 1>), dim3(P.n_cell), dim3(WG), ppb, st, mc);
-        hipLaunchKernelGGL((kb_cell<double, 0>), dim3(P.n_cell), dim3(WG), ppb, st, mc);
+        launchb_cells<double, 1>(P, st, P.cell1, P.n_cell1);
+        launchb_cells<double, 0>(P, st, P.cell0, P.n_cell0);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

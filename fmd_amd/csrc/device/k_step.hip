@@ -273,54 +273,14 @@ __global__ __launch_bounds__(1024) void kb_reduce_thermo(const KDesc *descs)
     b_reduce_thermo(D);
 }
 
-/* fixed-order sum of the per-cell pair and embedding energies */
-static __device__ void b_reduce_pe(const KDesc &D)
-{
-    __shared__ double sh[2][1024];
-    double pe = 0, emb = 0;
-    for (int c = threadIdx.x; c < D.ncell; c += 1024)
-    {
-        pe += D.cell_pe[c];
-        emb += D.cell_emb[c];
-    }
-    sh[0][threadIdx.x] = pe;
-    sh[1][threadIdx.x] = emb;
-    __syncthreads();
-    for (int s = 512; s > 0; s >>= 1)
-    {
-        if (threadIdx.x < s)
-        {
-            sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
-            sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-    {
-        D.thermo[TH_PE] = sh[0][0] + sh[1][0];
-        D.thermo[TH_EMB] = sh[1][0];
-    }
-}
-__global__ __launch_bounds__(1024) void k_reduce_pe(KDesc D) { b_reduce_pe(D); }
-__global__ __launch_bounds__(1024) void kb_reduce_pe(const KDesc *descs)
-{
-    const KDesc &D = descs[blockIdx.x];
-    b_reduce_pe(D);
-}
 
-/* two-level fixed-order sum of the per-cell energies for one large system:
-   block b sums a contiguous run of cells, one block sums the block totals */
-static inline int pe_blocks(int ncell)
-{
-    const int nb = (ncell + 1023) / 1024;
-    return nb < 1 ? 1 : (nb > NPART_MAX ? NPART_MAX : nb);
-}
-
-__global__ __launch_bounds__(TPB) void k_pe_part(KDesc D)
+/* two-level fixed-order sum of the per-cell energies of a system: block b of
+   nb sums a contiguous run of cells, one block sums the nb block totals */
+static __device__ void b_pe_part(const KDesc &D, int blk, int nb)
 {
     __shared__ double sh[2][TPB];
-    const int per = (D.ncell + gridDim.x - 1) / gridDim.x;
-    const int c0 = blockIdx.x * per, c1 = min(D.ncell, c0 + per);
+    const int per = (D.ncell + nb - 1) / nb;
+    const int c0 = blk * per, c1 = min(D.ncell, c0 + per);
     double pe = 0, emb = 0;
     for (int c = c0 + threadIdx.x; c < c1; c += TPB)
     {
@@ -341,12 +301,19 @@ __global__ __launch_bounds__(TPB) void k_pe_part(KDesc D)
     }
     if (threadIdx.x == 0)
     {
-        D.epart[2 * blockIdx.x] = sh[0][0];
-        D.epart[2 * blockIdx.x + 1] = sh[1][0];
+        D.epart[2 * blk] = sh[0][0];
+        D.epart[2 * blk + 1] = sh[1][0];
     }
 }
+__global__ __launch_bounds__(TPB) void k_pe_part(KDesc D) { b_pe_part(D, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(TPB) void kb_pe_part(BatchMap M)
+{
+    const int2 it = M.items[blockIdx.x];
+    const KDesc &D = M.descs[it.x];
+    b_pe_part(D, it.y, pe_blocks(D.ncell));
+}
 
-__global__ __launch_bounds__(1024) void k_pe_final(KDesc D, int nb)
+static __device__ void b_pe_final(const KDesc &D, int nb)
 {
     __shared__ double sh[2][1024];
     double pe = 0, emb = 0;
@@ -372,6 +339,20 @@ __global__ __launch_bounds__(1024) void k_pe_final(KDesc D, int nb)
         D.thermo[TH_PE] = sh[0][0] + sh[1][0];
         D.thermo[TH_EMB] = sh[1][0];
     }
+}
+__global__ __launch_bounds__(1024) void k_pe_final(KDesc D, int nb) { b_pe_final(D, nb); }
+__global__ __launch_bounds__(1024) void kb_pe_final(const KDesc *descs)
+{
+    const KDesc &D = descs[blockIdx.x];
+    b_pe_final(D, pe_blocks(D.ncell));
+}
+
+/* status words and thermo records of every system of a batch, for one copy to the host */
+__global__ __launch_bounds__(64) void kb_collect(const KDesc *descs, unsigned *status, double *thermo)
+{
+    const KDesc &D = descs[blockIdx.x];
+    if (threadIdx.x < 8) status[blockIdx.x * 8 + threadIdx.x] = D.counters[threadIdx.x];
+    if (threadIdx.x < 16) thermo[blockIdx.x * 16 + threadIdx.x] = D.thermo[threadIdx.x];
 }
 
 /* ----------------------------------------------------------- binning */
@@ -667,6 +648,7 @@ hipError_t launch_fill(double *p, long n, double v, hipStream_t st)
 
 hipError_t launchb_vv_start(const BatchPlan &P, hipStream_t st)
 {
+    if (P.nsys == 0) return hipSuccess;
     BatchMap ms = {P.descs, P.slot_items}, mc = {P.descs, P.cellblk_items};
     hipLaunchKernelGGL(kb_vv_start, dim3(P.n_slot), dim3(TPB), 0, st, ms);
     hipLaunchKernelGGL(kb_bin_rank, dim3(P.n_cellblk), dim3(TPB), 0, st, mc);
@@ -676,6 +658,7 @@ hipError_t launchb_vv_start(const BatchPlan &P, hipStream_t st)
 
 hipError_t launchb_vv_finish(const BatchPlan &P, hipStream_t st)
 {
+    if (P.nsys == 0) return hipSuccess;
     BatchMap ms = {P.descs, P.slot_items};
     hipLaunchKernelGGL(kb_vv_finish, dim3(P.n_slot), dim3(TPB), 0, st, ms);
     hipLaunchKernelGGL(kb_reduce_thermo, dim3(P.nsys), dim3(1024), 0, st, P.descs);
@@ -684,6 +667,7 @@ hipError_t launchb_vv_finish(const BatchPlan &P, hipStream_t st)
 
 hipError_t launchb_thermo(const BatchPlan &P, hipStream_t st)
 {
+    if (P.nsys == 0) return hipSuccess;
     BatchMap ms = {P.descs, P.slot_items};
     hipLaunchKernelGGL(kb_thermo, dim3(P.n_slot), dim3(TPB), 0, st, ms);
     hipLaunchKernelGGL(kb_reduce_thermo, dim3(P.nsys), dim3(1024), 0, st, P.descs);
@@ -692,6 +676,16 @@ hipError_t launchb_thermo(const BatchPlan &P, hipStream_t st)
 
 hipError_t launchb_reduce_pe(const BatchPlan &P, hipStream_t st)
 {
-    hipLaunchKernelGGL(kb_reduce_pe, dim3(P.nsys), dim3(1024), 0, st, P.descs);
+    if (P.nsys == 0) return hipSuccess;
+    BatchMap mp = {P.descs, P.pe_items};
+    hipLaunchKernelGGL(kb_pe_part, dim3(P.n_pe), dim3(TPB), 0, st, mp);
+    hipLaunchKernelGGL(kb_pe_final, dim3(P.nsys), dim3(1024), 0, st, P.descs);
+    return hipGetLastError();
+}
+
+hipError_t launchb_collect(const BatchPlan &P, hipStream_t st, unsigned *status, double *thermo)
+{
+    if (P.nsys == 0) return hipSuccess;
+    hipLaunchKernelGGL(kb_collect, dim3(P.nsys), dim3(64), 0, st, P.descs, status, thermo);
     return hipGetLastError();
 }

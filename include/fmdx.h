@@ -178,6 +178,31 @@ int fmdx_event_elapsed_ms(fmdx_system_t *s, int slot0, int slot1, float *ms);
 int fmdx_profile_enable(fmdx_system_t *s, int on);
 int fmdx_profile_read(fmdx_system_t *s, double *ms_by_phase /* 8 entries */, unsigned *count);
 
+/* ------------------------------------------------------------------ batches
+ * No counterpart in the reference (one system per fmd_t): independent systems
+ * advanced together, every kernel of a step covering all members through
+ * (system, block) work lists, the step replayed from a captured HIP graph.
+ * For ensembles of small and medium systems and for served jobs.  A member is
+ * an uploaded system without TTM; while it is a member only the batch calls
+ * act on it.  Its trajectory is the same alone and in any mix. */
+typedef struct fmdx_batch fmdx_batch_t;
+
+int fmdx_batch_create(fmdx_batch_t **out, int precision);
+void fmdx_batch_destroy(fmdx_batch_t *b);
+/* join: brings the member's forces and thermo up to date first (integrators.c:232);
+   dt, Berendsen flag, tau and T0 as fmd_dync_integrate / fmd_dync_equilibrate take them */
+int fmdx_batch_add(fmdx_batch_t *b, fmdx_system_t *s, double dt, int thermostat, double tau, double T0);
+int fmdx_batch_remove(fmdx_batch_t *b, fmdx_system_t *s);
+int fmdx_batch_size(fmdx_batch_t *b, int *n);
+int fmdx_batch_use_graph(fmdx_batch_t *b, int on);
+/* status words are read (and slot capacities grown) every `steps` steps; default 8 */
+int fmdx_batch_set_check_interval(fmdx_batch_t *b, int steps);
+/* velocity-Verlet steps of every member; synchronises at its end */
+int fmdx_batch_step(fmdx_batch_t *b, int nsteps);
+/* thermo record and status of every member, in join order; synchronises */
+int fmdx_batch_status(fmdx_batch_t *b, fmdx_thermo_t *out);
+int fmdx_batch_sync(fmdx_batch_t *b);
+
 #ifdef __cplusplus
 }
 #endif
