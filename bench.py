@@ -158,6 +158,74 @@ def cpu_baseline(setfl, cells=32, steps=20, seed=1):
     return dict(value=len(x) * steps / dt, natoms=len(x), steps=steps, seconds=dt)
 
 
+def batched_ensemble(nsys, steps, plain_sample, seed=3):
+    """The batched-ensemble workload: nsys periodic systems (LJ Ar or Morse Cu, log-uniform
+    500-50000 atoms, 50-800 K), all advanced together in mixed precision by the batch
+    engine; against the plain FP64 one-system-at-a-time time loop on a stratified sample."""
+    from fmd_amd import ensemble as E
+    specs = E.ensemble_specs(nsys, seed=seed)
+    jobs = [E.ensemble_job(sp, F.FMDX_PREC_MIXED) for sp in specs]
+    natoms = sum(j.natoms for j in jobs)
+    b = F.Batch(F.FMDX_PREC_MIXED)
+    for j in jobs:
+        b.add(j.dev, j.dt)
+    b.step(4)
+    t0 = time.perf_counter()
+    b.step(steps)
+    t = time.perf_counter() - t0
+    st = b.status()
+    errs = sum(1 for x in st if x.err)
+    b.close()
+    for j in jobs:
+        j.free()
+    # plain FP64, one system at a time, and the mixed-vs-FP64 force error of the same systems
+    sample = specs[:: max(1, nsys // plain_sample)]
+    per_step, ferr = [], []
+    for sp in sample:
+        j = E.ensemble_job(sp, F.FMDX_PREC_FP64)
+        f = j.f
+        f.fmd_dync_integrate(-1, (3 - 0.5) * j.dt, j.dt)
+        t1 = time.perf_counter()
+        f.fmd_dync_integrate(-1, (steps - 0.5) * j.dt, j.dt)
+        per_step.append((time.perf_counter() - t1) / steps)
+        a64 = f.atoms()
+        f.fmd_ext_setPrecision(F.FMDX_PREC_MIXED)
+        F._ck(F.load_fmdx().fmdx_update_forces(C.c_void_p(j.dev), -1, 0), "forces")
+        a32 = f.atoms()
+        scale = max(np.max(np.abs(a64["f"])), 1e-300)
+        ferr.append(float(np.max(np.abs(a32["f"] - a64["f"])) / scale))
+        j.free()
+    t_step = t / steps
+    jobs_s = nsys / (2000 * t_step)
+    plain_job_s = 1.0 / (2000 * float(np.mean(per_step)))
+    return dict(workload="%d periodic systems, LJ Ar or Morse Cu (p=1/2), FCC, log-uniform 500-50000 atoms "
+                         "(%d atoms in all), 50-800 K, dt=1 fs; job = 2000 steps of one system" % (nsys, natoms),
+                precision="mixed", timed_steps=steps, ms_per_step=t_step * 1e3,
+                atom_steps_per_s=natoms * steps / t, jobs_per_s=jobs_s,
+                plain_fp64_jobs_per_s=plain_job_s, speedup_vs_plain=jobs_s / plain_job_s,
+                plain_sample="%d of %d systems, %d timed steps each through fmd_dync_integrate"
+                             % (len(sample), nsys, steps),
+                max_rel_force_err_mixed_vs_fp64=max(ferr), members_with_errors=errs)
+
+
+def served_collisions(njobs, rate, seed=4):
+    """Served two-cluster Morse Cu collision jobs (Poisson arrivals) through the
+    continuous-batching loop in mixed precision, against the plain FP64
+    one-job-at-a-time loop on the same arrival stream."""
+    from fmd_amd import ensemble as E
+    from fmd_amd import serve as S
+    specs = E.collision_specs(njobs, seed=seed, rate=rate)
+    served = S.Server(E.collision_job, precision=F.FMDX_PREC_MIXED).run(specs)
+    plain = S.run_plain(specs, E.collision_job)
+    a, p = S.summarize(served), S.summarize(plain)
+    return dict(workload="%d jobs, Poisson arrivals at %g jobs/s; each two Morse-Cu FCC spheres R=15-50 A, "
+                         "20 A gap, 10-50 A/ps, impact parameter 0-(R1+R2), vacuum box, 1000-4000 steps of 1 fs"
+                         % (njobs, rate),
+                served_mixed=a, plain_fp64=p,
+                jobs_per_s_ratio=a["jobs_per_s"] / p["jobs_per_s"],
+                p99_ratio_plain_over_served=p["p99_s"] / a["p99_s"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,6 +235,13 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-collision measurement")
+    ap.add_argument("--batch-systems", type=int, default=512)
+    ap.add_argument("--batch-steps", type=int, default=40)
+    ap.add_argument("--plain-sample", type=int, default=16)
+    ap.add_argument("--serve-jobs", type=int, default=24)
+    ap.add_argument("--serve-rate", type=float, default=50.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
@@ -187,6 +262,13 @@ def main():
             p2 = F.FMDX_PREC_MIXED if prec == F.FMDX_PREC_FP64 else F.FMDX_PREC_FP64
             other = gpu_bench(args.cells, args.steps, args.warmup, p2, setfl)
             log("bench: other precision %s" % other)
+        ens = srv = None
+        if not args.no_batch:
+            ens = batched_ensemble(args.batch_systems, args.batch_steps, args.plain_sample)
+            log("bench: batched ensemble %s" % ens)
+        if not args.no_serve:
+            srv = served_collisions(args.serve_jobs, args.serve_rate)
+            log("bench: served collisions %s" % srv)
         cpu = None
         if not args.no_cpu_baseline:
             cpu = cpu_baseline(setfl)
@@ -224,6 +306,10 @@ def main():
                                   "value": other["natoms"] * other["steps"] / (other["gpu_ms"] * 1e-3),
                                   "ms_per_step": other["gpu_ms"] / other["steps"],
                                   "pass0_ms": other["pass0_ms"], "pass1_ms": other["pass1_ms"]}
+    if ens is not None:
+        out["batched_ensemble"] = ens
+    if srv is not None:
+        out["served_collisions"] = srv
     if cpu is not None:
         out["cpu_baseline"] = {"value": cpu["value"], "unit": "atom-steps/s", "cores": 1, "kind": "port",
                                "sample": "oracle (FP64 reference restatement, one thread) on EAM Cu %d atoms "

@@ -117,6 +117,7 @@ def load_fmdx():
         lib.fmdx_batch_step.argtypes = [P, C.c_int]
         lib.fmdx_batch_status.argtypes = [P, C.POINTER(Thermo)]
         lib.fmdx_batch_sync.argtypes = [P]
+        lib.fmdx_memory.argtypes = [P, C.POINTER(C.c_size_t)]
         _fmdx = lib
     return _fmdx
 

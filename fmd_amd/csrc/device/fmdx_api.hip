@@ -105,7 +105,7 @@ extern "C" int fmdx_system_create(fmdx_system_t **out, const double l[3], const 
     r |= dalloc(&D.nearly, ncell); r |= dalloc(&D.fill, ncell);
     r |= dalloc(&D.counters, 8);
     r |= dalloc(&D.cell_pe, ncell); r |= dalloc(&D.cell_emb, ncell);
-    r |= dalloc(&D.hbits, (size_t)ncell * HBITS_WORDS * 64); r |= dalloc(&D.htrips, ncell);
+    r |= dalloc(&D.htrips, ncell);
     r |= dalloc(&D.epart, 2 * NPART_MAX);
     r |= dalloc(&D.thermo, 16);
     r |= dalloc(&s->dpairs, MAXK * MAXK);
@@ -273,6 +273,8 @@ extern "C" int fmdx_set_eam(fmdx_system_t *s, int nelem, int nrho, double drho, 
     E.Fc = s->Fc; E.rhoc = s->rhoc; E.phic = s->phic;
     E.Fc32 = s->Fc32; E.rhoc32 = s->rhoc32; E.phic32 = s->phic32;
     E.rp = s->rp; E.rp32 = s->rp32;
+    /* hit words kept between the two EAM passes (one bit per lane and trip) */
+    if (!s->D.hbits && dalloc(&s->D.hbits, (size_t)s->D.ncell * HBITS_WORDS * 64)) return FMDX_ENOMEM;
     for (int k = 0; k < s->D.nkinds; k++) s->D.elem[k] = elem_of_kind ? elem_of_kind[k] : -1;
     return FMDX_OK;
 }
@@ -536,6 +538,23 @@ extern "C" int fmdx_clear_errors(fmdx_system_t *s)
     unsigned z = 0;
     CK(hipMemcpyAsync(&s->D.counters[3], &z, sizeof z, hipMemcpyHostToDevice, s->st));
     CK(hipStreamSynchronize(s->st));
+    return FMDX_OK;
+}
+
+/* device bytes held by a system (slot layout, per-cell arrays, tables) */
+extern "C" int fmdx_memory(fmdx_system_t *s, size_t *bytes)
+{
+    if (!s || !bytes) return FMDX_EARG;
+    const KDesc &D = s->D;
+    const size_t ns = (size_t)D.ncell * D.cap;
+    const size_t per_slot = 10 * sizeof(double) + 3 * sizeof(float) + 2 * sizeof(int) + sizeof(unsigned) + 2 +
+                            sizeof(unsigned long long) + sizeof(int);
+    const size_t nmov = s->natoms_uploaded + 64;
+    size_t b = s->uploaded ? ns * per_slot + nmov * (6 * sizeof(double) + 9) : 0;
+    b += (size_t)D.ncell * (6 * sizeof(int) + 2 * sizeof(double));
+    if (D.hbits) b += (size_t)D.ncell * HBITS_WORDS * 64 * sizeof(unsigned long long);
+    if (s->rhoc) b += (size_t)(D.eam.nrho + 6 * D.eam.nr2) * (sizeof(double4) + sizeof(float4));
+    *bytes = b;
     return FMDX_OK;
 }
 

@@ -318,9 +318,13 @@ static __device__ void b_cell(const KDesc &D, const int c)
     typedef Prec<R> P;
     enum { CT = P::CT };
     constexpr bool WITH_FP = PASS == 0 && (CAT == CAT_GEN || CAT == CAT_EAM1);
-    if (PASS == 1 && !D.has_eam)
+    if ((PASS == 1 && !D.has_eam) || D.cnt[c] == 0)
     {
-        if (threadIdx.x == 0) { D.cell_emb[c] = 0.0; D.htrips[c] = -1; }
+        if (threadIdx.x == 0)
+        {
+            if (PASS == 1) { D.cell_emb[c] = 0.0; D.htrips[c] = -1; }
+            else D.cell_pe[c] = 0.0;
+        }
         return;
     }
     __shared__ long nb_base[27];

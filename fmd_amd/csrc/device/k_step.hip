@@ -53,9 +53,16 @@ static __device__ void bin_flag(const KDesc &D, int c, long slot, double x, doub
 #pragma clang fp contract(off)
     int cx, cy, cz;
     cell_coords(D, c, cx, cy, cz);
-    int j0 = (int)floor(x / D.h0) + 1;
-    int j1 = (int)floor(y / D.h1) + 1;
-    int j2 = (int)floor(z / D.h2) + 1;
+    const double f0 = floor(x / D.h0), f1 = floor(y / D.h1), f2 = floor(z / D.h2);
+    /* a non-finite or far-away position cannot be binned (misc.c:281-291); the
+       comparisons run in FP64 so that no out-of-range conversion happens */
+    if (!(f0 >= -1.0 && f0 <= D.nc0 && f1 >= -1.0 && f1 <= D.nc1 && f2 >= -1.0 && f2 <= D.nc2))
+    {
+        set_err(D, FMDX_DEV_ERR_POSITION);
+        D.mv[slot] = MV_STAY;
+        return;
+    }
+    const int j0 = (int)f0 + 1, j1 = (int)f1 + 1, j2 = (int)f2 + 1;
     if (j0 < 0 || j0 >= D.nc0 + 2 || j1 < 0 || j1 >= D.nc1 + 2 || j2 < 0 || j2 >= D.nc2 + 2)
     {
         set_err(D, FMDX_DEV_ERR_POSITION);

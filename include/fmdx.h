@@ -171,6 +171,9 @@ int fmdx_ttm_upload_te(fmdx_system_t *s, const double *Te);
 /* bounding box of the atoms of a group (fmd_matt_findLimits, matter.c:152-213); synchronises */
 int fmdx_find_limits(fmdx_system_t *s, int group, double lo[3], double hi[3]);
 
+/* device bytes held by the system (admission control of served jobs) */
+int fmdx_memory(fmdx_system_t *s, size_t *bytes);
+
 /* timing helpers for benchmarks: HIP events on the system stream */
 int fmdx_event_record(fmdx_system_t *s, int slot);
 int fmdx_event_elapsed_ms(fmdx_system_t *s, int slot0, int slot1, float *ms);
