@@ -46,7 +46,9 @@ class TtmCfg(C.Structure):
                 ("T_G", C.c_void_p), ("Gt", C.c_void_p), ("G_DD", C.c_void_p),
                 ("laser_factor", C.c_double), ("laser_absdepth_inv", C.c_double),
                 ("laser_2sig2_inv", C.c_double), ("laser_t0", C.c_double), ("laser_tdiff", C.c_double),
-                ("frontsurf", C.c_double), ("min_atoms", C.c_uint)]
+                ("frontsurf", C.c_double), ("min_atoms", C.c_uint),
+                ("ncx", C.c_int), ("ncy", C.c_int), ("tcellh_x", C.c_double), ("tcellh_y", C.c_double),
+                ("kmode", C.c_int), ("solver3d", C.c_int)]
 
 
 _fmdx = None

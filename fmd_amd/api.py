@@ -79,6 +79,8 @@ _sigs = {
     "fmd_ext_loadAtoms": ([P, C.c_uint, P, P, P, P], None), "fmd_ext_distribute": ([P], C.c_bool),
     "fmd_ext_getDevice": ([P], P), "fmd_ext_setPrecision": ([P, C.c_int], None),
     "fmd_ext_getAtoms": ([P, P, P, P, P, P, P], C.c_uint), "fmd_ext_getPotentialEnergy": ([P], D),
+    "fmd_ext_ttm_setHeatConductivity_proportional": ([P, C.c_int, D], None),
+    "fmd_ext_ttm_useGridSolver": ([P, C.c_int, C.c_bool], None),
     "fmd_version_getString": ([], C.c_char_p),
 }
 

@@ -436,6 +436,10 @@ static TtmDev ttm_dev(fmdx_system_t *s, int on)
         T.tcellh_z = s->T.tcellh_z;
         T.xi = s->T.xi;
         T.vcm = s->T.vcm;
+        T.ncx = s->T.ncx;
+        T.ncy = s->T.ncy;
+        T.tcellh_x = s->T.tcellh_x;
+        T.tcellh_y = s->T.tcellh_y;
     }
     return T;
 }
@@ -607,15 +611,22 @@ static int ttm_alloc(fmdx_system_t *s, T **p, size_t n)
 
 extern "C" int fmdx_ttm_setup(fmdx_system_t *s, const fmdx_ttm_t *c, const double *Te_init)
 {
-    if (!s || !c || c->ncz < 1 || c->ncz > FMDX_TTM_MAX_CELLS || c->ratio < 1) return FMDX_EARG;
+    if (!s || !c || c->ncz < 1 || c->ratio < 1) return FMDX_EARG;
     if (c->type != 1 && c->type != 2) return FMDX_EARG;
+    const int ncx = c->ncx > 0 ? c->ncx : 1, ncy = c->ncy > 0 ? c->ncy : 1;
+    const int dim3 = ncx * ncy > 1 || c->kmode != 0 || c->solver3d;
+    if (!dim3 && c->ncz > FMDX_TTM_MAX_CELLS) return FMDX_EARG;
+    if (dim3 && (c->type != 1 || (long)ncx * ncy * (c->ncz + 2) > FMDX_TTM_MAX_TCELLS)) return FMDX_EARG;
     if (c->type == 2 && (c->nC < 2 || c->nG < 2 || !c->T_C || !c->Ct || !c->C_DD || !c->T_G || !c->Gt || !c->G_DD))
         return FMDX_EARG;
     for (int i = 0; i < s->nttm_mem; i++) hipFree(s->ttm_mem[i]);
     s->nttm_mem = 0;
     TtmState &S = s->T;
     memset(&S, 0, sizeof S);
-    S.type = c->type; S.ncz = c->ncz; S.n = c->ncz + 2; S.ratio = c->ratio;
+    S.type = c->type; S.ncz = c->ncz; S.ratio = c->ratio;
+    S.ncx = ncx; S.ncy = ncy; S.nzg = c->ncz + 2; S.n = ncx * ncy * S.nzg;
+    S.dim3 = dim3; S.kmode = c->kmode;
+    S.tcellh_x = c->tcellh_x; S.tcellh_y = c->tcellh_y;
     S.tcellh_z = c->tcellh_z; S.tvol = c->tcell_volume;
     S.C_gamma = c->C_gamma; S.K = c->K; S.G = c->G;
     S.Kzh_A = c->Kzh_A; S.Kzh_B = c->Kzh_B; S.Kzh_v2 = c->Kzh_v2;

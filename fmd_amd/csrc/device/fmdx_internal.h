@@ -57,8 +57,10 @@ struct TtmDev
     int on;
     int ncz;
     double tcellh_z;
-    const double *xi;      /* [ncz+2] */
-    const double *vcm;     /* [3*(ncz+2)] */
+    const double *xi;      /* [ncx*ncy*(ncz+2)] */
+    const double *vcm;     /* [3*ncx*ncy*(ncz+2)] */
+    int ncx, ncy;          /* a turi that is not 1 x 1 x n: friction on every component */
+    double tcellh_x, tcellh_y;
 };
 
 struct KDesc
@@ -120,10 +122,13 @@ struct BatchMap
     const int2 *items;
 };
 
-/* 1-D TTM state resident on the device (arrays of ncz+2 turi cells, ghosts included) */
+/* TTM state resident on the device: n = ncx*ncy*(ncz+2) turi cells, index
+   (ix*ncy + iy)*(ncz+2) + iz, ghosts in z only; dim3 selects the grid solver */
 struct TtmState
 {
     int type, ncz, n, ratio;
+    int ncx, ncy, nzg, dim3, kmode;
+    double tcellh_x, tcellh_y;
     double tcellh_z, tvol;
     double C_gamma, K, G, Kzh_A, Kzh_B, Kzh_v2;
     int nC, nG;

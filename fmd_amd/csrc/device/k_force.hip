@@ -547,10 +547,25 @@ static __device__ void b_cell(const KDesc &D, const int c)
                 const TtmDev &TT = D.ttm;
                 if (act && TT.on)
                 {
+                    /* add_ttm_term_to_forces, forces.c:292-330 (z only for a 1 x 1 x n turi) */
                     const double zabs = sizeof(R) == sizeof(double) ? (double)zi : (double)zi + cz * D.h2;
                     const int iz = (int)(zabs / TT.tcellh_z) + 1;
                     if (iz >= 0 && iz < TT.ncz + 2)
-                        sfz += TT.xi[iz] * D.mass[ki] * (D.vz[is] - TT.vcm[3 * iz + 2]);
+                    {
+                        if (TT.ncx * TT.ncy > 1)
+                        {
+                            int ix = (int)(D.x[is] / TT.tcellh_x), iy = (int)(D.y[is] / TT.tcellh_y);
+                            ix = ix < 0 ? 0 : (ix >= TT.ncx ? TT.ncx - 1 : ix);
+                            iy = iy < 0 ? 0 : (iy >= TT.ncy ? TT.ncy - 1 : iy);
+                            const int a = (ix * TT.ncy + iy) * (TT.ncz + 2) + iz;
+                            const double ximass = TT.xi[a] * D.mass[ki];
+                            sfx += ximass * (D.vx[is] - TT.vcm[3 * a]);
+                            sfy += ximass * (D.vy[is] - TT.vcm[3 * a + 1]);
+                            sfz += ximass * (D.vz[is] - TT.vcm[3 * a + 2]);
+                        }
+                        else
+                            sfz += TT.xi[iz] * D.mass[ki] * (D.vz[is] - TT.vcm[3 * iz + 2]);
+                    }
                 }
                 D.fx[is] = act ? sfx : 0.0;
                 D.fy[is] = act ? sfy : 0.0;

@@ -13,6 +13,14 @@
  * Te, Te2, Ti, Ke, Ce, G and xi in LDS for all sub-steps of an MD step.
  * The activation scan of presolve is sequential in the reference (a cell
  * activated earlier in the scan feeds the next), so one lane runs it.
+ *
+ * ncx x ncy x ncz turies (no counterpart in the reference, whose 3-D
+ * updater is missing, ttm.c:451-452): type 1 with Ce = gamma Te, constant G,
+ * and constant conductivity or kappa = K Te / Ti, periodic in x and y.  The
+ * atom sums go straight to HBM atomics (the grid is too large for per-block
+ * LDS histograms); one workgroup of 1024 lanes keeps Te and Te2 of up to
+ * FMDX_TTM_MAX_TCELLS cells in LDS for the sub-steps, each lane owning the
+ * same cells throughout so that its xi sums stay in registers.
  */
 #include "fmdx_internal.h"
 
@@ -264,11 +272,203 @@ __global__ __launch_bounds__(TTPB) void k_ttm_solve(TtmState S, double time, dou
     }
 }
 
+/* ------------------------------------------------ ncx x ncy x ncz turies */
+
+/* turi cell of a position, x and y clamped to the grid; a non-finite or
+   far-away z goes to the lower ghost cell (the comparisons run in FP64 so
+   that no out-of-range conversion happens) */
+static __device__ __forceinline__ int tcell3(const TtmState &S, double x, double y, double z)
+{
+    const double fx = x / S.tcellh_x, fy = y / S.tcellh_y, fz = z / S.tcellh_z;
+    const int ix = fx >= 0.0 ? (fx < S.ncx ? (int)fx : S.ncx - 1) : 0;
+    const int iy = fy >= 0.0 ? (fy < S.ncy ? (int)fy : S.ncy - 1) : 0;
+    const int iz = (fz >= -1.0 && fz < S.nzg - 1) ? (int)fz + 1 : 0;
+    return (ix * S.ncy + iy) * S.nzg + iz;
+}
+
+__global__ __launch_bounds__(TPB) void k_ttm_sum_mv3(KDesc D, TtmState S)
+{
+    each_slot(D, blockIdx.x, [&](long slot, int c) {
+        (void)c;
+        const int a = tcell3(S, D.x[slot], D.y[slot], D.z[slot]);
+        if (a < 0) return;
+        const double m = D.mass[D.kind[slot]];
+        atomicAdd(&S.mass[a], m);
+        atomicAdd(&S.vcm[3 * a], m * D.vx[slot]);
+        atomicAdd(&S.vcm[3 * a + 1], m * D.vy[slot]);
+        atomicAdd(&S.vcm[3 * a + 2], m * D.vz[slot]);
+    });
+}
+
+__global__ __launch_bounds__(TTPB) void k_ttm_vcm3(TtmState S)
+{
+    const int a = blockIdx.x * TTPB + threadIdx.x;
+    if (a >= S.n || S.mass[a] == 0.0) return;
+    for (int d = 0; d < 3; d++) S.vcm[3 * a + d] /= S.mass[a];
+}
+
+__global__ __launch_bounds__(TPB) void k_ttm_sum_temp3(KDesc D, TtmState S)
+{
+#pragma clang fp contract(off)
+    each_slot(D, blockIdx.x, [&](long slot, int c) {
+        (void)c;
+        const int a = tcell3(S, D.x[slot], D.y[slot], D.z[slot]);
+        if (a < 0) return;
+        const double m = D.mass[D.kind[slot]];
+        const double vx = D.vx[slot] - S.vcm[3 * a];
+        const double vy = D.vy[slot] - S.vcm[3 * a + 1];
+        const double vz = D.vz[slot] - S.vcm[3 * a + 2];
+        atomicAdd(&S.Ti[a], m * (vx * vx + vy * vy + vz * vz));
+        atomicAdd(&S.num[a], 1u);
+    });
+}
+
+__global__ __launch_bounds__(TTPB) void k_ttm_temp3(TtmState S)
+{
+#pragma clang fp contract(off)
+    const int a = blockIdx.x * TTPB + threadIdx.x;
+    if (a >= S.n || S.num[a] == 0) return;
+    S.Ti[a] /= (3.0 * KB_EV * S.num[a]);
+}
+
+/* the activation scan of presolve_common_types_1_2 over the grid, cells in
+   index order; a newly active cell takes the atom-weighted mean Te of its
+   active z, x and y neighbours (z first, as the 1-D scan) */
+__global__ __launch_bounds__(64) void k_ttm_presolve3(TtmState S)
+{
+#pragma clang fp contract(off)
+    if (threadIdx.x != 0) return;
+    for (int ix = 0; ix < S.ncx; ix++)
+        for (int iy = 0; iy < S.ncy; iy++)
+            for (int iz = 1; iz <= S.ncz; iz++)
+            {
+                const int a = (ix * S.ncy + iy) * S.nzg + iz;
+                S.xi[a] = 0.0;
+                if (S.Te[a] < 0.0 && S.num[a] >= S.min_atoms)
+                {
+                    const int nb[6] = {a - 1, a + 1,
+                                       (((ix + S.ncx - 1) % S.ncx) * S.ncy + iy) * S.nzg + iz,
+                                       (((ix + 1) % S.ncx) * S.ncy + iy) * S.nzg + iz,
+                                       (ix * S.ncy + (iy + S.ncy - 1) % S.ncy) * S.nzg + iz,
+                                       (ix * S.ncy + (iy + 1) % S.ncy) * S.nzg + iz};
+                    unsigned count = 0;
+                    double sum = 0.0;
+                    for (int k = 0; k < 6; k++)
+                        if (S.Te[nb[k]] >= 0.0)
+                        {
+                            sum += S.Te[nb[k]] * S.num[nb[k]];
+                            count += S.num[nb[k]];
+                        }
+                    S.Te[a] = (count > 0) ? sum / (int)count : S.Ti[a];
+                }
+                if (S.num[a] < S.min_atoms) S.Te[a] = S.Te2[a] = -1.0;
+            }
+}
+
+#define T3PB 1024
+#define T3OWN 4   /* cells owned per lane: T3PB * T3OWN >= FMDX_TTM_MAX_TCELLS */
+
+/* explicit FD sub-steps on the grid: Ce = C_gamma Te, constant G; kmode 0:
+   K (Te_hi - 2 Te + Te_lo) / h^2 per dimension (the 1-D type-1 expression,
+   ttm.c:281-287), kmode 1: kappa = K Te / Ti with face values
+   (kappa_i + kappa_nb) / 2 (the type-2 flux form, ttm.c:347-352); an inactive
+   neighbour is replaced by the cell itself */
+__global__ __launch_bounds__(T3PB) void k_ttm_solve3(TtmState S, double time, double mddt)
+{
+#pragma clang fp contract(off)
+    extern __shared__ double t3sh[];
+    double *Te = t3sh, *Te2 = t3sh + S.n;
+    for (int a = threadIdx.x; a < S.n; a += T3PB)
+    {
+        Te[a] = S.Te[a];
+        Te2[a] = S.Te2[a];
+    }
+    __syncthreads();
+    double xacc[T3OWN];
+    for (int k = 0; k < T3OWN; k++) xacc[k] = 0.0;
+    const bool calc_source = fabs(time - S.laser_t0) < S.laser_tdiff;
+    const double dt = mddt / S.ratio;
+    const double h2[3] = {S.tcellh_x * S.tcellh_x, S.tcellh_y * S.tcellh_y, S.tcellh_z * S.tcellh_z};
+    for (int j = 0; j < S.ratio; j++)
+    {
+        double spc = 0.0;
+        if (calc_source)
+        {
+            const double q = time + j * dt - S.laser_t0;
+            spc = S.laser_factor * exp(S.laser_2sig2_inv * (q * q));
+        }
+        for (int k = 0; k < T3OWN; k++)
+        {
+            const int a = threadIdx.x + k * T3PB;
+            if (a >= S.n) break;
+            const int iz = a % S.nzg, col = a / S.nzg;
+            if (iz < 1 || iz > S.ncz || Te[a] < 0) continue;
+            const int ix = col / S.ncy, iy = col % S.ncy;
+            double source = 0.0;
+            if (calc_source)
+            {
+                const double z = (iz - 1 + 0.5) * S.tcellh_z;
+                source = spc * exp(S.laser_absdepth_inv * (z - S.frontsurf));
+            }
+            int lo[3] = {(((ix + S.ncx - 1) % S.ncx) * S.ncy + iy) * S.nzg + iz,
+                         (ix * S.ncy + (iy + S.ncy - 1) % S.ncy) * S.nzg + iz, a - 1};
+            int hi[3] = {(((ix + 1) % S.ncx) * S.ncy + iy) * S.nzg + iz,
+                         (ix * S.ncy + (iy + 1) % S.ncy) * S.nzg + iz, a + 1};
+            for (int d = 0; d < 3; d++)
+            {
+                if (Te[lo[d]] < 0) lo[d] = a;
+                if (Te[hi[d]] < 0) hi[d] = a;
+            }
+            const double Ti = S.Ti[a];
+            double flux = 0.0;
+            if (S.kmode == 0)
+            {
+                for (int d = 0; d < 3; d++) flux += S.K * (Te[hi[d]] - 2 * Te[a] + Te[lo[d]]) / h2[d];
+            }
+            else
+            {
+                const double ka = S.K * Te[a] / Ti;
+                for (int d = 0; d < 3; d++)
+                {
+                    const double KeL = 0.5 * (ka + S.K * Te[lo[d]] / S.Ti[lo[d]]);
+                    const double KeR = 0.5 * (ka + S.K * Te[hi[d]] / S.Ti[hi[d]]);
+                    flux += (KeR * (Te[hi[d]] - Te[a]) - KeL * (Te[a] - Te[lo[d]])) / h2[d];
+                }
+            }
+            Te2[a] = Te[a] + dt / (S.C_gamma * Te[a]) * (flux - S.G * (Te[a] - Ti) + source);
+            xacc[k] += Te2[a];
+        }
+        __syncthreads();
+        double *t = Te2; Te2 = Te; Te = t;
+    }
+    for (int k = 0; k < T3OWN; k++)
+    {
+        const int a = threadIdx.x + k * T3PB;
+        if (a >= S.n) break;
+        const int iz = a % S.nzg;
+        if (iz >= 1 && iz <= S.ncz)
+            S.xi[a] = S.num[a] >= S.min_atoms
+                          ? S.G * S.tvol * (xacc[k] / S.ratio - S.Ti[a]) / (3 * KB_EV * S.num[a] * S.Ti[a])
+                          : 0.0;
+        S.Te[a] = Te[a];
+        S.Te2[a] = Te2[a];
+    }
+}
+
 hipError_t launch_ttm_fields(const KDesc &D, const TtmState &S, hipStream_t st)
 {
-    unsigned gs = (unsigned)(((long)D.ncell * D.cap + TTPB - 1) / TTPB);
-    unsigned gn = (S.n + TTPB - 1) / TTPB;
+    const unsigned gn = (S.n + TTPB - 1) / TTPB;
     hipLaunchKernelGGL(k_ttm_clear, dim3(gn), dim3(TTPB), 0, st, S);
+    if (S.dim3)
+    {
+        const unsigned gc = (unsigned)((D.ncell + CPB - 1) / CPB);
+        hipLaunchKernelGGL(k_ttm_sum_mv3, dim3(gc), dim3(TPB), 0, st, D, S);
+        hipLaunchKernelGGL(k_ttm_vcm3, dim3(gn), dim3(TTPB), 0, st, S);
+        hipLaunchKernelGGL(k_ttm_sum_temp3, dim3(gc), dim3(TPB), 0, st, D, S);
+        hipLaunchKernelGGL(k_ttm_temp3, dim3(gn), dim3(TTPB), 0, st, S);
+        return hipGetLastError();
+    }
+    const unsigned gs = (unsigned)(((long)D.ncell * D.cap + TTPB - 1) / TTPB);
     hipLaunchKernelGGL(k_ttm_sum_mv, dim3(gs), dim3(TTPB), 0, st, D, S);
     hipLaunchKernelGGL(k_ttm_vcm, dim3(gn), dim3(TTPB), 0, st, S);
     hipLaunchKernelGGL(k_ttm_sum_temp, dim3(gs), dim3(TTPB), 0, st, D, S);
@@ -278,13 +478,16 @@ hipError_t launch_ttm_fields(const KDesc &D, const TtmState &S, hipStream_t st)
 
 hipError_t launch_ttm_presolve(const TtmState &S, unsigned *err, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_ttm_presolve, dim3(1), dim3(TTPB), 0, st, S, err);
+    if (S.dim3) hipLaunchKernelGGL(k_ttm_presolve3, dim3(1), dim3(64), 0, st, S);
+    else hipLaunchKernelGGL(k_ttm_presolve, dim3(1), dim3(TTPB), 0, st, S, err);
     return hipGetLastError();
 }
 
 hipError_t launch_ttm_solve(const TtmState &S, double time, double dt, unsigned *err, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_ttm_solve, dim3(1), dim3(TTPB), 0, st, S, time, dt, err);
+    if (S.dim3)
+        hipLaunchKernelGGL(k_ttm_solve3, dim3(1), dim3(T3PB), sizeof(double) * 2 * S.n, st, S, time, dt);
+    else
+        hipLaunchKernelGGL(k_ttm_solve, dim3(1), dim3(TTPB), 0, st, S, time, dt, err);
     return hipGetLastError();
 }
-

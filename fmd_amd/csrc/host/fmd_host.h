@@ -126,7 +126,10 @@ typedef struct
     double laser_factor, laser_absdepth_inv, laser_2sig2_inv, laser_t0, laser_tdiff;
     double frontsurf;
     unsigned initial_atoms, min_atoms;
-    double *Te_host;             /* Te set before the device copy exists [ncz+2] */
+    int kmode;                   /* type 1: 0 constant K, 1 K * Te / Ti */
+    bool grid_solver;            /* grid solver forced for a 1 x 1 x n turi */
+    unsigned ncells;             /* tdims[0] * tdims[1] * (tdims[2] + 2) */
+    double *Te_host;             /* Te set before the device copy exists [ncells] */
     bool dev_ready;
     int iTe, ixi, inum, ivcm, iTi, imass;
 } turi_t_;

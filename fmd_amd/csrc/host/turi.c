@@ -94,9 +94,17 @@ fmd_handle_t fmd_turi_add(fmd_t *md, fmd_turi_t cat, int dimx, int dimy, int dim
         fmdh_err_not_supported(md, false, HERE, "Only TTM turies are computed by this build.");
         return -1;
     }
-    if (dimx != 1 || dimy != 1 || dimz < 1 || dimz > FMDX_TTM_MAX_CELLS)
+    if (dimx < 1 || dimy < 1 || dimz < 1)
     {
-        fmdh_err_not_supported(md, false, HERE, "Only 1 x 1 x n TTM turies are supported (as in the reference).");
+        fmdh_err_unacceptable_int(md, false, HERE, "turi dimension", dimx < 1 ? dimx : (dimy < 1 ? dimy : dimz));
+        return -1;
+    }
+    if (dimx == 1 && dimy == 1 ? dimz > FMDX_TTM_MAX_CELLS
+                               : (cat != FMD_TURI_TTM_TYPE1 ||
+                                  (long)dimx * dimy * (dimz + 2) > FMDX_TTM_MAX_TCELLS))
+    {
+        fmdh_err_not_supported(md, false, HERE,
+                               "TTM turies: 1 x 1 x n (n <= 1024), or type 1 with dimx*dimy*(dimz+2) <= 4000.");
         return -1;
     }
     turi_t_ *nt = realloc(md->turies, (md->nturies + 1) * sizeof(turi_t_));
@@ -121,7 +129,8 @@ fmd_handle_t fmd_turi_add(fmd_t *md, fmd_turi_t cat, int dimx, int dimy, int dim
     t->imass = t->fields[t->ivcm].dep[0];
     t->CellActivFrac = 0.1;
     t->ratio = 1;
-    t->Te_host = calloc(dimz + 2, sizeof(double));
+    t->ncells = (unsigned)(dimx * dimy * (dimz + 2));
+    t->Te_host = calloc(t->ncells, sizeof(double));
     md->active_ttm_turi = (int)md->nturies;
     return (fmd_handle_t)md->nturies++;
 }
@@ -220,7 +229,25 @@ void _fmd_ttm_setHeatConductivity_zhigilei(fmd_t *md, fmd_handle_t turi, fmd_ttm
 void _fmd_ttm_setHeatConductivity_constant1(fmd_t *md, fmd_handle_t turi, fmd_real_t k)
 {
     turi_t_ *t = ttm_of(md, turi, true, false);
-    if (t) t->K = k * WATT_PER_METER_KELVIN;
+    if (t)
+    {
+        t->K = k * WATT_PER_METER_KELVIN;
+        t->kmode = 0;
+    }
+}
+
+void fmd_ext_ttm_setHeatConductivity_proportional(fmd_t *md, fmd_handle_t turi, fmd_real_t K0)
+{
+    turi_t_ *t = ttm_of(md, turi, true, false);
+    if (!t) return;
+    t->K = K0 * WATT_PER_METER_KELVIN;
+    t->kmode = 1;
+}
+
+void fmd_ext_ttm_useGridSolver(fmd_t *md, fmd_handle_t turi, bool on)
+{
+    turi_t_ *t = ttm_of(md, turi, true, false);
+    if (t) t->grid_solver = on;
 }
 
 void _fmd_ttm_setHeatConductivity_constant2(fmd_t *md, fmd_handle_t turi, fmd_ttm_heat_conductivity_constant_t k)
@@ -249,12 +276,15 @@ void fmd_ttm_setElectronTemperature(fmd_t *md, fmd_handle_t turi, fmd_real_t Te)
 {
     turi_t_ *t = ttm_of(md, turi, true, true);
     if (!t) return;
-    for (unsigned i = 1; i <= t->tdims[2]; i++) t->Te_host[i] = Te;
+    const unsigned nzg = t->tdims[2] + 2;
+    for (unsigned a = 0; a < t->ncells; a++)
+        if (a % nzg >= 1 && a % nzg <= t->tdims[2]) t->Te_host[a] = Te;
     if (t->dev_ready && md->dev)
     {
-        double *buf = malloc((t->tdims[2] + 2) * sizeof(double));
+        double *buf = malloc(t->ncells * sizeof(double));
         fmdx_ttm_download(md->dev, buf, NULL, NULL, NULL, NULL, NULL);
-        for (unsigned i = 1; i <= t->tdims[2]; i++) buf[i] = Te;
+        for (unsigned a = 0; a < t->ncells; a++)
+            if (a % nzg >= 1 && a % nzg <= t->tdims[2]) buf[a] = Te;
         fmdx_ttm_upload_te(md->dev, buf);
         free(buf);
     }
@@ -314,6 +344,12 @@ static bool ttm_device_setup(fmd_t *md, turi_t_ *t)
     c.laser_2sig2_inv = t->laser_2sig2_inv; c.laser_t0 = t->laser_t0; c.laser_tdiff = t->laser_tdiff;
     c.frontsurf = t->frontsurf;
     c.min_atoms = t->min_atoms;
+    c.ncx = (int)t->tdims[0];
+    c.ncy = (int)t->tdims[1];
+    c.tcellh_x = t->tcellh[0];
+    c.tcellh_y = t->tcellh[1];
+    c.kmode = t->kmode;
+    c.solver3d = t->grid_solver;
     if (fmdx_ttm_setup(md->dev, &c, t->Te_host) != FMDX_OK)
     {
         fmdh_err_function_failed(md, true, HERE, "fmdx_ttm_setup");
@@ -481,47 +517,47 @@ fmd_array3s_t *fmd_field_getArray(fmd_t *md, fmd_handle_t turi, fmd_handle_t fie
     turi_t_ *t = &md->turies[turi];
     if (field < 0 || field >= t->nfields) return NULL;
     int cat = t->fields[field].cat;
-    unsigned n = t->tdims[2];
+    const unsigned nx = t->tdims[0], ny = t->tdims[1], nz = t->tdims[2], nzg = nz + 2, n = t->ncells;
     if (!t->dev_ready || !md->dev)
     {
         fmdh_err_unprepared(md, false, HERE, "turi field");
         return NULL;
     }
-    double *Te = malloc((n + 2) * sizeof(double)), *Ti = malloc((n + 2) * sizeof(double));
-    double *vcm = malloc(3 * (n + 2) * sizeof(double)), *mass = malloc((n + 2) * sizeof(double));
-    unsigned *num = malloc((n + 2) * sizeof(unsigned));
+    double *Te = malloc(n * sizeof(double)), *Ti = malloc(n * sizeof(double));
+    double *vcm = malloc(3 * n * sizeof(double)), *mass = malloc(n * sizeof(double));
+    unsigned *num = malloc(n * sizeof(unsigned));
     fmdx_ttm_download(md->dev, Te, Ti, NULL, vcm, num, mass);
     fmd_array3s_t *a = NULL;
+    size_t esz = cat == FMD_FIELD_VCM ? 3 * sizeof(double) : (cat == FMD_FIELD_NUMBER ? sizeof(unsigned)
+                                                                                      : sizeof(double));
     switch (cat)
     {
-        case FMD_FIELD_VCM:
-            a = fmdh_array3s_create(1, 1, n, 3 * sizeof(double));
-            for (unsigned i = 0; i < n; i++)
-                for (int d = 0; d < 3; d++) ((double ***)a->data)[0][0][3 * i + d] = vcm[3 * (i + 1) + d];
-            break;
-        case FMD_FIELD_NUMBER_DENSITY:
-            a = fmdh_array3s_create(1, 1, n, sizeof(double));
-            for (unsigned i = 0; i < n; i++) ((double ***)a->data)[0][0][i] = num[i + 1] / t->tcell_volume;
-            break;
-        case FMD_FIELD_TEMPERATURE:
-            a = fmdh_array3s_create(1, 1, n, sizeof(double));
-            for (unsigned i = 0; i < n; i++) ((double ***)a->data)[0][0][i] = Ti[i + 1];
-            break;
-        case FMD_FIELD_TTM_TE:
-            a = fmdh_array3s_create(1, 1, n, sizeof(double));
-            for (unsigned i = 0; i < n; i++) ((double ***)a->data)[0][0][i] = Te[i + 1];
-            break;
-        case FMD_FIELD_MASS:
-            a = fmdh_array3s_create(1, 1, n, sizeof(double));
-            for (unsigned i = 0; i < n; i++) ((double ***)a->data)[0][0][i] = mass[i + 1] * MD_MASS_UNIT;
-            break;
-        case FMD_FIELD_NUMBER:
-            a = fmdh_array3s_create(1, 1, n, sizeof(unsigned));
-            for (unsigned i = 0; i < n; i++) ((unsigned ***)a->data)[0][0][i] = num[i + 1];
+        case FMD_FIELD_VCM: case FMD_FIELD_NUMBER_DENSITY: case FMD_FIELD_TEMPERATURE:
+        case FMD_FIELD_TTM_TE: case FMD_FIELD_MASS: case FMD_FIELD_NUMBER:
+            a = fmdh_array3s_create(nx, ny, nz, esz);
             break;
         default:
             fmdh_err_unacceptable_int(md, false, HERE, "field category", cat);
     }
+    if (a)
+        for (unsigned ix = 0; ix < nx; ix++)
+            for (unsigned iy = 0; iy < ny; iy++)
+                for (unsigned iz = 0; iz < nz; iz++)
+                {
+                    const unsigned s_ = (ix * ny + iy) * nzg + iz + 1;
+                    char *row = (char *)((void ***)a->data)[ix][iy];
+                    switch (cat)
+                    {
+                        case FMD_FIELD_VCM:
+                            for (int d = 0; d < 3; d++) ((double *)row)[3 * iz + d] = vcm[3 * s_ + d];
+                            break;
+                        case FMD_FIELD_NUMBER_DENSITY: ((double *)row)[iz] = num[s_] / t->tcell_volume; break;
+                        case FMD_FIELD_TEMPERATURE: ((double *)row)[iz] = Ti[s_]; break;
+                        case FMD_FIELD_TTM_TE: ((double *)row)[iz] = Te[s_]; break;
+                        case FMD_FIELD_MASS: ((double *)row)[iz] = mass[s_] * MD_MASS_UNIT; break;
+                        case FMD_FIELD_NUMBER: ((unsigned *)row)[iz] = num[s_]; break;
+                    }
+                }
     free(Te); free(Ti); free(vcm); free(mass); free(num);
     if (a)
     {

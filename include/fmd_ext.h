@@ -39,6 +39,15 @@ unsigned fmd_ext_getAtoms(fmd_t *md, double *x, double *v, double *f, int *group
 /* Potential energy of the last force evaluation (eV). */
 double fmd_ext_getPotentialEnergy(fmd_t *md);
 
+/* TTM turies beyond 1 x 1 x n (fmd_turi_add with dimx or dimy > 1, type 1):
+   the reference accepts them but has no updater for them (ttm.c:451-452).
+   This build solves them on the device, periodic in x and y.
+   Conductivity kappa = K0 * Te / Tl (K0 in W/m/K) instead of a constant: */
+void fmd_ext_ttm_setHeatConductivity_proportional(fmd_t *md, fmd_handle_t turi, fmd_real_t K0);
+/* Solve a 1 x 1 x n type-1 turi with the grid solver (used to show that the
+   grid solver reduces to the reference's 1-D scheme). */
+void fmd_ext_ttm_useGridSolver(fmd_t *md, fmd_handle_t turi, bool on);
+
 #ifdef __cplusplus
 }
 #endif

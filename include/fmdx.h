@@ -34,7 +34,8 @@ extern "C" {
 
 #define FMDX_MAX_KINDS 8
 #define FMDX_MAX_ELEMENTS 4
-#define FMDX_TTM_MAX_CELLS 1024
+#define FMDX_TTM_MAX_CELLS 1024      /* cells along z of a 1 x 1 x n turi */
+#define FMDX_TTM_MAX_TCELLS 4000     /* ncx * ncy * (ncz + 2) of a turi solved on the ncx x ncy x ncz grid */
 
 typedef struct fmdx_system fmdx_system_t;
 
@@ -90,7 +91,11 @@ typedef struct
     unsigned cap;            /* slots per cell */
 } fmdx_thermo_t;
 
-/* 1-D two-temperature model description (one turi of 1 x 1 x ncz cells) */
+/* two-temperature model description: the reference's 1 x 1 x ncz turi, and
+   (no counterpart in the reference, whose 3-D updater is missing,
+   ttm.c:451-452) ncx x ncy x ncz turies of type 1, periodic in x and y, with
+   constant conductivity or kappa = K * Te / Ti.  Arrays are indexed
+   (ix * ncy + iy) * (ncz + 2) + iz with a ghost layer in z only. */
 typedef struct
 {
     int type;                /* 1: constant K, G, Ce = gamma Te; 2: tabulated Ce(Te), G(Te), Zhigilei K */
@@ -106,6 +111,10 @@ typedef struct
     double laser_factor, laser_absdepth_inv, laser_2sig2_inv, laser_t0, laser_tdiff;
     double frontsurf;
     unsigned min_atoms;
+    int ncx, ncy;                        /* turi cells along x and y (0 means 1) */
+    double tcellh_x, tcellh_y;
+    int kmode;                           /* type 1: 0 constant K, 1 kappa = K * Te / Ti */
+    int solver3d;                        /* 1: grid solver even for 1 x 1 x ncz (reduction tests) */
 } fmdx_ttm_t;
 
 /* devices */
@@ -163,7 +172,7 @@ int fmdx_ttm_update(fmdx_system_t *s, double time, double dt);
 int fmdx_ttm_fields(fmdx_system_t *s);
 int fmdx_ttm_presolve(fmdx_system_t *s);
 int fmdx_ttm_solve(fmdx_system_t *s, double time, double dt);
-/* synchronises; arrays of ncz+2 entries (ghost cells included), any may be NULL */
+/* synchronises; arrays of ncx*ncy*(ncz+2) entries (z ghosts included), any may be NULL */
 int fmdx_ttm_download(fmdx_system_t *s, double *Te, double *Ti, double *xi, double *vcm /* 3*(ncz+2) */,
                       unsigned *num, double *mass);
 int fmdx_ttm_upload_te(fmdx_system_t *s, const double *Te);

@@ -122,7 +122,9 @@ typedef struct
 {
     int on;
     orc_ttm_cfg_t c;
-    int n;                /* ncz + 2 */
+    int n;                /* ncx * ncy * (ncz + 2) */
+    int ncx, ncy, nzg;    /* nzg = ncz + 2 */
+    int dim3;             /* ncx x ncy x ncz solver */
     double tcellh[3], tvol;
     unsigned *num;
     double *mass, *vcm, *Ti, *Te, *Te2, *xi, *Ke, *Ce, *Gc, *CDD, *GDD;
@@ -282,12 +284,18 @@ double orc_spline_deriv(double h, const double *ya, const double *yDDa, double x
     return spl_der(a, b, ya, lo, hi, yDDa, h);
 }
 
-/* EAM_convert_r_to_r2, eam.c:324-341 */
+/* EAM_convert_r_to_r2, eam.c:324-341.  At the last point sqrt(i*dr2)/dr can
+   reach Nr-1 exactly; _fmd_spline_val (cspline.c:76-89) then reads element
+   Nr, one past the table, with weight b = 0.  A zero pad element keeps that
+   read inside the buffer and leaves the value unchanged. */
 static void eam_r_to_r2(oeam *e, const double *src, double *dst)
 {
-    double *dd = malloc(e->Nr * sizeof(double));
-    orc_spline_prepare(e->dr, src, e->Nr, dd);
-    for (int i = 0; i < e->Nr2; i++) dst[i] = orc_spline_val(e->dr, src, dd, sqrt(i * e->dr2));
+    double *y = calloc(e->Nr + 1, sizeof(double));
+    double *dd = calloc(e->Nr + 1, sizeof(double));
+    memcpy(y, src, e->Nr * sizeof(double));
+    orc_spline_prepare(e->dr, y, e->Nr, dd);
+    for (int i = 0; i < e->Nr2; i++) dst[i] = orc_spline_val(e->dr, y, dd, sqrt(i * e->dr2));
+    free(y);
     free(dd);
 }
 
@@ -1377,14 +1385,19 @@ int orc_integrate_vv(orc_t *o, int ag, double *time, double duration, double dt,
 
 /* ------------------------------------------------------------------- TTM */
 
-/* turi geometry for a 1-D TTM turi (1 x 1 x ncz), turi.c:648-738 with ns=1 */
+/* turi geometry, turi.c:648-738 with ns=1: tcellh = l / tdims.  The
+   reference's turi is 1 x 1 x ncz; ncx x ncy x ncz is this build's extension. */
 int orc_ttm_init(orc_t *o, const orc_ttm_cfg_t *cfg, double Te_init)
 {
     ottm *t = &o->ttm;
     t->c = *cfg;
-    t->n = cfg->ncz + 2;
-    t->tcellh[0] = o->l[0] / 1;
-    t->tcellh[1] = o->l[1] / 1;
+    t->ncx = cfg->ncx > 0 ? cfg->ncx : 1;
+    t->ncy = cfg->ncy > 0 ? cfg->ncy : 1;
+    t->nzg = cfg->ncz + 2;
+    t->n = t->ncx * t->ncy * t->nzg;
+    t->dim3 = t->ncx * t->ncy > 1 || cfg->kmode != 0 || cfg->solver3d;
+    t->tcellh[0] = o->l[0] / t->ncx;
+    t->tcellh[1] = o->l[1] / t->ncy;
     t->tcellh[2] = o->l[2] / cfg->ncz;
     t->tvol = t->tcellh[0] * t->tcellh[1] * t->tcellh[2];
     t->num = calloc(t->n, sizeof(unsigned));
@@ -1397,7 +1410,9 @@ int orc_ttm_init(orc_t *o, const orc_ttm_cfg_t *cfg, double Te_init)
     t->Ke = calloc(t->n, sizeof(double));
     t->Ce = calloc(t->n, sizeof(double));
     t->Gc = calloc(t->n, sizeof(double));
-    for (int i = 1; i <= cfg->ncz; i++) t->Te[i] = Te_init;   /* fmd_ttm_setElectronTemperature, ttm.c:739-755 */
+    /* fmd_ttm_setElectronTemperature, ttm.c:739-755: interior cells */
+    for (int a = 0; a < t->ncx * t->ncy; a++)
+        for (int i = 1; i <= cfg->ncz; i++) t->Te[a * t->nzg + i] = Te_init;
     if (cfg->type == 2)
     {
         t->T_C = malloc(cfg->nC * sizeof(double)); t->Ct = malloc(cfg->nC * sizeof(double));
@@ -1587,12 +1602,203 @@ static void ttm_solve(orc_t *o, double time, double mddt)
         }
 }
 
+/* ---- ncx x ncy x ncz turies (this build's extension; periodic in x and y,
+   a ghost layer in z as in the 1-D turi) ---- */
+
+static inline int tidx(const ottm *t, int ix, int iy, int iz) { return (ix * t->ncy + iy) * t->nzg + iz; }
+
+/* turi cell of a position: (int)(x / tcellh) per dimension, z shifted by the
+   ghost layer (turi.c:1136 with itc_glob_to_loc); x and y are clamped to the
+   grid, where a coordinate within rounding of the box length lands */
+static int ttm_cell3(const ottm *t, const double *x)
+{
+    double fx = x[0] / t->tcellh[0], fy = x[1] / t->tcellh[1], fz = x[2] / t->tcellh[2];
+    int ix = fx >= 0.0 ? (fx < t->ncx ? (int)fx : t->ncx - 1) : 0;
+    int iy = fy >= 0.0 ? (fy < t->ncy ? (int)fy : t->ncy - 1) : 0;
+    /* a non-finite or far-away z goes to the lower ghost cell */
+    int iz = (fz >= -1.0 && fz < t->nzg - 1) ? (int)fz + 1 : 0;
+    return tidx(t, ix, iy, iz);
+}
+
+static void ttm_fields3(orc_t *o)
+{
+    ottm *t = &o->ttm;
+    for (int a = 0; a < t->n; a++)
+    {
+        t->mass[a] = 0.0;
+        t->vcm[3 * a] = t->vcm[3 * a + 1] = t->vcm[3 * a + 2] = 0.0;
+        t->Ti[a] = 0.0;
+        t->num[a] = 0;
+    }
+    for (int ic = 0; ic < o->nci; ic++)
+    {
+        ocell *c = &o->grid[ic];
+        for (int i = 0; i < c->n; i++)
+        {
+            int a = ttm_cell3(t, &c->x[3 * i]);
+            double m = o->mass[c->kind[i]];
+            t->mass[a] += m;
+            for (int d = 0; d < 3; d++) t->vcm[3 * a + d] += m * c->v[3 * i + d];
+        }
+    }
+    for (int a = 0; a < t->n; a++)
+        if (t->mass[a] != 0.0)
+            for (int d = 0; d < 3; d++) t->vcm[3 * a + d] /= t->mass[a];
+    for (int ic = 0; ic < o->nci; ic++)
+    {
+        ocell *c = &o->grid[ic];
+        for (int i = 0; i < c->n; i++)
+        {
+            int a = ttm_cell3(t, &c->x[3 * i]);
+            double vt[3];
+            for (int d = 0; d < 3; d++) vt[d] = c->v[3 * i + d] - t->vcm[3 * a + d];
+            double m = o->mass[c->kind[i]];
+            t->Ti[a] += m * (vt[0] * vt[0] + vt[1] * vt[1] + vt[2] * vt[2]);
+            ++t->num[a];
+        }
+    }
+    for (int a = 0; a < t->n; a++)
+        if (t->num[a]) t->Ti[a] /= (3.0 * ORC_KB * t->num[a]);
+}
+
+/* presolve_common_types_1_2 generalised: the same sequential scan (cells in
+   index order), a newly active cell taking the atom-weighted mean Te of its
+   active z, x and y neighbours (z first, as in the 1-D scan) */
+static void ttm_presolve3(orc_t *o)
+{
+    ottm *t = &o->ttm;
+    for (int ix = 0; ix < t->ncx; ix++)
+        for (int iy = 0; iy < t->ncy; iy++)
+            for (int iz = 1; iz <= t->c.ncz; iz++)
+            {
+                int a = tidx(t, ix, iy, iz);
+                t->xi[a] = 0.0;
+                if (t->Te[a] < 0.0 && t->num[a] >= t->min_atoms)
+                {
+                    int nb[6] = {tidx(t, ix, iy, iz - 1), tidx(t, ix, iy, iz + 1),
+                                 tidx(t, (ix + t->ncx - 1) % t->ncx, iy, iz), tidx(t, (ix + 1) % t->ncx, iy, iz),
+                                 tidx(t, ix, (iy + t->ncy - 1) % t->ncy, iz), tidx(t, ix, (iy + 1) % t->ncy, iz)};
+                    int count = 0;
+                    double sum = 0.0;
+                    for (int k = 0; k < 6; k++)
+                        if (t->Te[nb[k]] >= 0.0) { sum += t->Te[nb[k]] * t->num[nb[k]]; count += t->num[nb[k]]; }
+                    t->Te[a] = (count > 0) ? sum / count : t->Ti[a];
+                }
+                if (t->num[a] < t->min_atoms) t->Te[a] = t->Te2[a] = -1.0;
+            }
+}
+
+/* explicit FD sub-steps on the ncx x ncy x ncz turi.  Ce = C_gamma * Te, G
+   constant; kmode 0: K (Te_hi - 2 Te + Te_lo) / h^2 per dimension, the 1-D
+   type-1 expression (ttm.c:281-287); kmode 1: kappa = K * Te / Ti with face
+   values (kappa_i + kappa_nb) / 2, the type-2 flux form (ttm.c:347-352).
+   An inactive neighbour is replaced by the cell itself (zero flux). */
+static void ttm_solve3(orc_t *o, double time, double mddt)
+{
+    ottm *t = &o->ttm;
+    int CalcSource = fabs(time - t->c.laser_t0) < t->c.laser_tdiff;
+    double dt = mddt / t->c.ratio;
+    const double h2[3] = {t->tcellh[0] * t->tcellh[0], t->tcellh[1] * t->tcellh[1], t->tcellh[2] * t->tcellh[2]};
+    for (int j = 0; j < t->c.ratio; j++)
+    {
+        double spc = 0.0;
+        if (CalcSource)
+        {
+            double q = time + j * dt - t->c.laser_t0;
+            spc = t->c.laser_factor * exp(t->c.laser_2sig2_inv * (q * q));
+        }
+        for (int ix = 0; ix < t->ncx; ix++)
+            for (int iy = 0; iy < t->ncy; iy++)
+                for (int iz = 1; iz <= t->c.ncz; iz++)
+                {
+                    int a = tidx(t, ix, iy, iz);
+                    if (t->Te[a] < 0) continue;
+                    double source = 0.0;
+                    if (CalcSource)
+                    {
+                        double z = (iz - 1 + 0.5) * t->tcellh[2];
+                        source = spc * exp(t->c.laser_absdepth_inv * (z - t->frontsurf));
+                    }
+                    int lo[3] = {tidx(t, (ix + t->ncx - 1) % t->ncx, iy, iz), tidx(t, ix, (iy + t->ncy - 1) % t->ncy, iz),
+                                 tidx(t, ix, iy, iz - 1)};
+                    int hi[3] = {tidx(t, (ix + 1) % t->ncx, iy, iz), tidx(t, ix, (iy + 1) % t->ncy, iz),
+                                 tidx(t, ix, iy, iz + 1)};
+                    for (int d = 0; d < 3; d++)
+                    {
+                        if (t->Te[lo[d]] < 0) lo[d] = a;
+                        if (t->Te[hi[d]] < 0) hi[d] = a;
+                    }
+                    double flux = 0.0;
+                    if (t->c.kmode == 0)
+                    {
+                        for (int d = 0; d < 3; d++)
+                            flux += t->c.K * (t->Te[hi[d]] - 2 * t->Te[a] + t->Te[lo[d]]) / h2[d];
+                    }
+                    else
+                    {
+                        double ka = t->c.K * t->Te[a] / t->Ti[a];
+                        for (int d = 0; d < 3; d++)
+                        {
+                            double KeL = 0.5 * (ka + t->c.K * t->Te[lo[d]] / t->Ti[lo[d]]);
+                            double KeR = 0.5 * (ka + t->c.K * t->Te[hi[d]] / t->Ti[hi[d]]);
+                            flux += (KeR * (t->Te[hi[d]] - t->Te[a]) - KeL * (t->Te[a] - t->Te[lo[d]])) / h2[d];
+                        }
+                    }
+                    t->Te2[a] = t->Te[a] + dt / (t->c.C_gamma * t->Te[a]) * (flux - t->c.G * (t->Te[a] - t->Ti[a]) +
+                                                                               source);
+                    t->xi[a] += t->Te2[a];
+                }
+        double *tmp = t->Te2; t->Te2 = t->Te; t->Te = tmp;
+    }
+    for (int a = 0; a < t->n; a++)
+    {
+        int iz = a % t->nzg;
+        if (iz >= 1 && iz <= t->c.ncz && t->num[a] >= t->min_atoms)
+            t->xi[a] = t->c.G * t->tvol * (t->xi[a] / t->c.ratio - t->Ti[a]) / (3 * ORC_KB * t->num[a] * t->Ti[a]);
+    }
+}
+
+void orc_ttm_set_fields(orc_t *o, const double *Te, const double *Ti, const unsigned *num, double frontsurf,
+                        unsigned min_atoms)
+{
+    ottm *t = &o->ttm;
+    for (int a = 0; a < t->n; a++)
+    {
+        t->Te[a] = Te[a];
+        t->Te2[a] = Te[a];
+        t->Ti[a] = Ti[a];
+        t->num[a] = num[a];
+        t->xi[a] = 0.0;
+    }
+    t->frontsurf = frontsurf;
+    t->min_atoms = min_atoms;
+}
+
+void orc_ttm_solve_only(orc_t *o, double time, double dt)
+{
+    ottm *t = &o->ttm;
+    if (t->dim3) ttm_solve3(o, time, dt);
+    else
+    {
+        for (int i = 1; i <= t->c.ncz; i++) t->xi[i] = 0.0;
+        if (t->c.type == 2) ttm_type2_update(o);
+        ttm_solve(o, time, dt);
+    }
+}
+
 void orc_ttm_update(orc_t *o, double time, double dt)
 {
     ottm *t = &o->ttm;
     if (!t->on) return;
     if (!((time >= t->c.start - dt / 2.0) && ((time < t->c.stop + dt / 2.0) || t->c.stop < t->c.start)))
         return;
+    if (t->dim3)
+    {
+        ttm_fields3(o);
+        ttm_presolve3(o);
+        ttm_solve3(o, time, dt);
+        return;
+    }
     ttm_fields(o);
     ttm_presolve(o);
     if (t->c.type == 2) ttm_type2_update(o);
@@ -1601,7 +1807,7 @@ void orc_ttm_update(orc_t *o, double time, double dt)
 
 static double g_ttm_time, g_ttm_dt;
 
-/* add_ttm_term_to_forces, forces.c:292-330, 1-D branch */
+/* add_ttm_term_to_forces, forces.c:292-330 */
 static void ttm_forces(orc_t *o, int ag)
 {
     ottm *t = &o->ttm;
@@ -1615,6 +1821,14 @@ static void ttm_forces(orc_t *o, int ag)
         {
             if (!ACTIVE(ag, c, i)) continue;
             double m = o->mass[c->kind[i]];
+            if (t->ncx * t->ncy > 1)
+            {
+                /* forces.c:317-328, a turi that is not 1 x 1 x n: every component */
+                int a = ttm_cell3(t, &c->x[3 * i]);
+                double ximass = t->xi[a] * m;
+                for (int d = 0; d < 3; d++) c->F[3 * i + d] += ximass * (c->v[3 * i + d] - t->vcm[3 * a + d]);
+                continue;
+            }
             int iz = ttm_iz(o, &c->x[3 * i]);
             c->F[3 * i + 2] += t->xi[iz] * m * (c->v[3 * i + 2] - t->vcm[3 * iz + 2]);
         }

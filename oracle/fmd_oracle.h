@@ -85,7 +85,12 @@ int orc_integrate_vv(orc_t *o, int active_group, double *time, double duration, 
 /* timer predicate, timer.c:27-36 */
 int orc_timer_is_its_time(double t, double dt_half, double start, double interval);
 
-/* 1-D TTM (type 1 and 2) */
+/* TTM.  The reference's 1-D turi (1 x 1 x ncz, type 1 and 2), and this
+   build's extension to ncx x ncy x ncz turies periodic in x and y, type 1
+   with constant conductivity (kmode 0) or kappa = K * Te / Ti (kmode 1).
+   parity unpinned by the reference for the extension itself: it is pinned by
+   reduction to the 1-D scheme, by the type-2 special case (Zhigilei A = 0,
+   linear Ce table, constant G table) and by the flux-conservation identity. */
 typedef struct
 {
     int type;              /* 1 or 2 */
@@ -98,11 +103,18 @@ typedef struct
     double act_frac;
     double laser_factor, laser_absdepth_inv, laser_2sig2_inv, laser_t0, laser_tdiff;
     double start, stop;
+    int ncx, ncy;          /* turi cells along x and y (0 means 1) */
+    int kmode;             /* 0: constant K; 1: K * Te / Ti */
+    int solver3d;          /* 1: use the ncx x ncy x ncz solver even for a 1 x 1 x ncz turi */
 } orc_ttm_cfg_t;
 int orc_ttm_init(orc_t *o, const orc_ttm_cfg_t *cfg, double Te_init);
 void orc_ttm_get_ready(orc_t *o, int active_group);
 void orc_ttm_update(orc_t *o, double time, double dt);   /* fields + presolve + solve */
-/* per z-cell arrays, length ncz+2 (ghosts included) */
+/* field injection and the FD sub-steps alone (pin tests of the solver) */
+void orc_ttm_set_fields(orc_t *o, const double *Te, const double *Ti, const unsigned *num, double frontsurf,
+                        unsigned min_atoms);
+void orc_ttm_solve_only(orc_t *o, double time, double dt);
+/* per turi-cell arrays, length ncx*ncy*(ncz+2), index (ix*ncy + iy)*(ncz+2) + iz (z ghosts included) */
 void orc_ttm_arrays(orc_t *o, double *Te, double *Ti, double *xi, double *vcmz, unsigned *num,
                     double *mass);
 void orc_ttm_params(orc_t *o, double *frontsurf, unsigned *min_atoms, double *tcellh_z);

@@ -27,7 +27,8 @@ class TtmCfg(C.Structure):
                 ("act_frac", C.c_double),
                 ("laser_factor", C.c_double), ("laser_absdepth_inv", C.c_double),
                 ("laser_2sig2_inv", C.c_double), ("laser_t0", C.c_double), ("laser_tdiff", C.c_double),
-                ("start", C.c_double), ("stop", C.c_double)]
+                ("start", C.c_double), ("stop", C.c_double),
+                ("ncx", C.c_int), ("ncy", C.c_int), ("kmode", C.c_int), ("solver3d", C.c_int)]
 
 
 def lib():
@@ -71,6 +72,8 @@ def lib():
         L.orc_ttm_update.argtypes = [P, C.c_double, C.c_double]
         L.orc_ttm_arrays.argtypes = [P] + [P] * 6
         L.orc_ttm_params.argtypes = [P, P, P, P]
+        L.orc_ttm_set_fields.argtypes = [P, P, P, P, C.c_double, C.c_uint]
+        L.orc_ttm_solve_only.argtypes = [P, C.c_double, C.c_double]
         L.orc_integrate_se.argtypes = [P, C.c_int, P, C.c_double, C.c_double]
         L.orc_spline_prepare.argtypes = [C.c_double, P, C.c_int, P]
         L.orc_spline_prepare_adv.argtypes = [P, P, C.c_int, P]
@@ -233,6 +236,15 @@ class Oracle:
         num = np.zeros(n, np.uint32)
         self.L.orc_ttm_arrays(self.h, _p(Te), _p(Ti), _p(xi), _p(vz), _p(num), _p(mass))
         return dict(Te=Te, Ti=Ti, xi=xi, vcmz=vz, num=num, mass=mass)
+
+    def ttm_set_fields(self, Te, Ti, num, frontsurf=0.0, min_atoms=1):
+        Te = np.ascontiguousarray(Te, dtype=np.float64)
+        Ti = np.ascontiguousarray(Ti, dtype=np.float64)
+        num = np.ascontiguousarray(num, dtype=np.uint32)
+        self.L.orc_ttm_set_fields(self.h, _p(Te), _p(Ti), _p(num), frontsurf, min_atoms)
+
+    def ttm_solve_only(self, time, dt):
+        self.L.orc_ttm_solve_only(self.h, time, dt)
 
     def ttm_params(self):
         fs, h = C.c_double(), C.c_double()
