@@ -226,6 +226,95 @@ def served_collisions(njobs, rate, seed=4):
                 p99_ratio_plain_over_served=p["p99_s"] / a["p99_s"])
 
 
+def ttm_slab(steps, precision, setfl, seed=5, pad=320.0):
+    """The TTM-MD workload: EAM Cu slab of 40 x 40 x 400 FCC cells (2,560,000 atoms), periodic in
+    x and y, free surfaces in z with `pad` A of vacuum on each side; a 4 x 4 x 200 TTM turi with
+    Ce = gamma Te (96.6 J/m^3/K^2), kappa = kappa0 Te / Tl (401 W/m/K), G = 1e17 W/m^3/K and the
+    Gaussian source (absorbed fluence 100 J/m^2, 15 nm depth, t0 = 2 ps, FWHM 0.8326 ps, i.e.
+    exp(-((t - 2 ps) / 0.5 ps)^2)); 50 TTM sub-steps per 1 fs MD step; Te = Tl = 300 K at start:
+    the lattice gets 600 K velocities and is brought to 300 K by 0.4 ps of Berendsen dynamics
+    before the turi exists.  The vacuum keeps the explicit TTM step inside its stability limit
+    (kappa0 / (gamma Tl) grows as the lattice cools).  Runs through the public API
+    (fmd_dync_integrate: symplectic Euler with the TTM turi)."""
+    from fmd_amd import api
+    a = W.CU["a"]
+    n = (40, 40, 400)
+    rng = np.random.default_rng(seed)
+    x = W.fcc_positions(*n, a, origin=(0.0, 0.0, pad)) + rng.normal(0.0, 0.02, size=(4 * n[0] * n[1] * n[2], 3))
+    L = [n[0] * a, n[1] * a, n[2] * a + 2 * pad]
+    x[:, 0] = np.mod(x[:, 0], L[0])
+    x[:, 1] = np.mod(x[:, 1], L[1])
+    v = W.maxwell_velocities(len(x), 63.546, 600.0, rng)
+    f = api.Fmd()
+    f.fmd_box_setSize(*L)
+    f.fmd_box_setPBC(True, True, False)
+    f.set_kinds(["Cu"], [63.546])
+    pot = f.fmd_pot_eam_alloy_load(setfl.encode())
+    f.fmd_pot_apply(0, 0, pot)
+    f.load_atoms(x, v)
+    f.fmd_ext_setPrecision(precision)
+    if not f.fmd_ext_distribute():
+        raise RuntimeError("distribute failed")
+    f.fmd_dync_equilibrate(-1, 0.4, 2e-3, 0.05, 300.0)
+    T_start = f.fmd_matt_getTemperature()
+    dims = (4, 4, 200)
+    turi = f.fmd_turi_add(api.TURI_TTM_TYPE1, *dims, 0.0, -1.0)
+    gamma, k0, G, ratio, dt = 96.6, 401.0, 1e17, 50, 1e-3
+    f._fmd_ttm_setHeatCapacity_linear(turi, api.HeatCapLinear(gamma))
+    f.fmd_ext_ttm_setHeatConductivity_proportional(turi, k0)
+    f._fmd_ttm_setCouplingFactor_constant1(turi, G)
+    f.fmd_ttm_setElectronTemperature(turi, 300.0)
+    f.fmd_ttm_setTimestepRatio(turi, ratio)
+    f._fmd_ttm_setLaserSource_gaussian(turi, api.LaserGaussian(100.0, 0.0, 2e-12, 0.8326e-12, 15e-9))
+    # explicit-FD stability at Tl = 300 K: 2 D dt_sub sum(1/h^2) with D = kappa0 / (gamma Tl)
+    h = [L[0] / dims[0] * 1e-10, L[1] / dims[1] * 1e-10, L[2] / dims[2] * 1e-10]
+    cfl = 2 * (k0 / (gamma * 300.0)) * (dt * 1e-12 / ratio) * sum(1.0 / (q * q) for q in h)
+    f.fmd_dync_integrate(-1, (2 - 0.5) * dt, dt)
+    t0 = time.perf_counter()
+    f.fmd_dync_integrate(-1, (steps - 0.5) * dt, dt)
+    wall = time.perf_counter() - t0
+    te = f.field_array(turi, f.fmd_field_find(turi, api.FIELD_TTM_TE))
+    ti = f.field_array(turi, f.fmd_field_find(turi, api.FIELD_TEMPERATURE))
+    T = f.fmd_matt_getTemperature()
+    act = te >= 0
+    tl_min = float(np.min(ti[act])) if np.any(act) else float("nan")
+    natoms = len(x)
+    f.free()
+    return dict(workload="EAM Cu slab 40x40x400 FCC (%d atoms), PBC x,y, free z with %g A vacuum per side; "
+                         "TTM 4x4x200, Ce=96.6 Te J/m^3/K, kappa=401 Te/Tl W/m/K, G=1e17 W/m^3/K, source "
+                         "100 J/m^2 absorbed, 15 nm, exp(-((t-2 ps)/0.5 ps)^2), 50 sub-steps per 1 fs step"
+                         % (natoms, pad),
+                precision="mixed" if precision == F.FMDX_PREC_MIXED else "fp64", timed_steps=steps,
+                ms_per_step=wall / steps * 1e3, atom_steps_per_s=natoms * steps / wall,
+                cfl_number_at_300K=cfl, cfl_number_coldest_cell=cfl * 300.0 / tl_min,
+                lattice_T_start=T_start, lattice_T_after=T, max_Te_after=float(np.max(te)),
+                active_ttm_cells=int(np.sum(act)))
+
+
+def nve_drift(cells, steps, precision, setfl, seed=1, dt=2e-3):
+    """Total-energy drift of the EAM Cu crystal in NVE (eV/atom/ns), velocity Verlet at dt."""
+    fx = F.load_fmdx()
+    lib, md, dev, n = make_system(cells, seed, setfl, precision)
+    F._ck(fx.fmdx_update_forces(dev, -1, 0), "forces")
+    F._ck(fx.fmdx_thermo(dev, -1), "thermo")
+    th = F.Thermo()
+    F._ck(fx.fmdx_read_thermo(dev, C.byref(th)), "read_thermo")
+    e0 = th.pe + th.ke
+    es = [e0]
+    for k in range(steps):
+        F._ck(fx.fmdx_vv_start(dev, -1, dt, 0, 1.0, 0.0), "vv_start")
+        F._ck(fx.fmdx_update_forces(dev, -1, 0), "forces")
+        F._ck(fx.fmdx_vv_finish(dev, -1, dt), "vv_finish")
+        if (k + 1) % 10 == 0:
+            F._ck(fx.fmdx_read_thermo(dev, C.byref(th)), "read_thermo")
+            es.append(th.pe + th.ke)
+    lib.fmd_free(md)
+    t = np.arange(len(es)) * 10 * dt * 1e-3   # ns
+    slope = float(np.polyfit(t, np.array(es) / n, 1)[0])
+    return dict(steps=steps, dt_ps=dt, natoms=n, drift_eV_per_atom_per_ns=slope,
+                max_dev_eV_per_atom=float(np.max(np.abs(np.array(es) - e0)) / n))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -242,6 +331,10 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--serve-jobs", type=int, default=24)
     ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--no-ttm", action="store_true", help="skip the TTM slab measurement")
+    ap.add_argument("--ttm-steps", type=int, default=200)
+    ap.add_argument("--no-drift", action="store_true", help="skip the NVE drift measurement")
+    ap.add_argument("--drift-steps", type=int, default=500)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
@@ -262,7 +355,21 @@ def main():
             p2 = F.FMDX_PREC_MIXED if prec == F.FMDX_PREC_FP64 else F.FMDX_PREC_FP64
             other = gpu_bench(args.cells, args.steps, args.warmup, p2, setfl)
             log("bench: other precision %s" % other)
-        ens = srv = None
+        ens = srv = ttm = None
+        drift = {}
+        if not args.no_drift:
+            cu01 = os.path.join(ROOT, "tests", "golden", "data", "Cu01.eam.alloy")
+            for pname, pv in (("fp64", F.FMDX_PREC_FP64), ("mixed", F.FMDX_PREC_MIXED)):
+                drift["cu01_" + pname] = nve_drift(min(args.cells, 40), args.drift_steps, pv, cu01)
+                drift["synthetic_" + pname] = nve_drift(min(args.cells, 40), args.drift_steps, pv, setfl)
+            drift["cu01_mixed_minus_fp64_eV_per_atom_per_ns"] = (drift["cu01_mixed"]["drift_eV_per_atom_per_ns"] -
+                                                                drift["cu01_fp64"]["drift_eV_per_atom_per_ns"])
+            drift["note"] = ("Cu01 is the smooth Mishin setfl; the synthetic table is truncated at 5.5 A "
+                             "without a taper, so its drift measures the cutoff jumps")
+            log("bench: NVE drift %s" % drift)
+        if not args.no_ttm:
+            ttm = ttm_slab(args.ttm_steps, F.FMDX_PREC_MIXED, setfl)
+            log("bench: TTM slab %s" % ttm)
         if not args.no_batch:
             ens = batched_ensemble(args.batch_systems, args.batch_steps, args.plain_sample)
             log("bench: batched ensemble %s" % ens)
@@ -306,6 +413,10 @@ def main():
                                   "value": other["natoms"] * other["steps"] / (other["gpu_ms"] * 1e-3),
                                   "ms_per_step": other["gpu_ms"] / other["steps"],
                                   "pass0_ms": other["pass0_ms"], "pass1_ms": other["pass1_ms"]}
+    if drift:
+        out["nve_drift"] = drift
+    if ttm is not None:
+        out["ttm_slab"] = ttm
     if ens is not None:
         out["batched_ensemble"] = ens
     if srv is not None:
