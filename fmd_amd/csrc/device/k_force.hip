@@ -331,6 +331,7 @@ static __device__ void b_cell(const KDesc &D, const int c)
     __shared__ int nb_pre[28];
     __shared__ R nb_sx[27], nb_sy[27], nb_sz[27];
     __shared__ Tile<R, WITH_FP> T;
+    __shared__ short selfpos[256];   /* tile position of home atom j, -1 when not in the tile */
     extern __shared__ PairParam pp[];   /* nkinds^2 entries, sized at launch */
     /* the stripe sums reuse the tile once the scan is over */
     double(*red)[WG] = reinterpret_cast<double(*)[WG]>(&T);
@@ -393,6 +394,21 @@ static __device__ void b_cell(const KDesc &D, const int c)
     const bool replay = PASS == 0 && one_block && hw >= 0;
     PairParam p1;
     if (CAT != CAT_GEN) p1 = pp[0];
+    /* the home cell's box in the frame of the staged positions, and the culling
+       radius with a margin over the rounding of the box distance */
+    R bx0, bx1, by0, by1, bz0, bz1;
+    if (sizeof(R) == sizeof(double))
+    {
+        bx0 = (R)(cx * D.h0); bx1 = (R)((cx + 1) * D.h0);
+        by0 = (R)(cy * D.h1); by1 = (R)((cy + 1) * D.h1);
+        bz0 = (R)(cz * D.h2); bz1 = (R)((cz + 1) * D.h2);
+    }
+    else
+    {
+        bx0 = by0 = bz0 = (R)0;
+        bx1 = (R)D.h0; by1 = (R)D.h1; bz1 = (R)D.h2;
+    }
+    const R cull2 = rcut2 * (sizeof(R) == sizeof(double) ? (R)(1.0 + 1e-12) : (R)(1.0 + 1e-5));
     double cell_acc = 0.0;
 
     for (int h0 = 0; h0 < nh; h0 += WG)
@@ -413,22 +429,28 @@ static __device__ void b_cell(const KDesc &D, const int c)
             if (PASS == 0 && D.has_eam) Fpi = (R)D.aux[is];
         }
         const PairParam *ppi = pp + ki * D.nkinds;
-        const int selfm = nb_pre[13] + h0 + a;   /* candidate index of the home atom itself */
         Acc acc[4];
         for (int q = 0; q < 4; q++) acc[q].zero();
         int wi = 0;   /* hit words so far (wave-uniform) */
-        for (int cs = 0; cs < ncand; cs += CT)
+        int gpos = 0; /* next candidate of the 27-cell list to stage (wave-uniform) */
+        while (gpos < ncand)
         {
-            const int nt = min((int)CT, ncand - cs);
-            const int trips = (nt + S - 1) / S;
-            const int npad = trips * S;
             __syncthreads();   /* the previous tile is no longer read */
-            for (int m = lane; m < npad; m += WG)
+            for (int j = lane; j < nh; j += WG) selfpos[j] = -1;
+            __syncthreads();
+            /* fill the tile with the candidates that can reach the home cell: a
+               candidate farther than the cutoff from the cell's box cannot pair
+               with any home atom; survivors are compacted in list order */
+            int nt = 0;
+            while (gpos < ncand && nt <= (int)CT - WG)
             {
-                if (m < nt)
+                const int g = gpos + lane;
+                bool keep = false;
+                R x = 0, y = 0, z = 0, fpv = 0;
+                int kj = 0, lo = 0;
+                if (g < ncand)
                 {
-                    const int g = cs + m;
-                    int lo = 0, hi = 27;
+                    int hi = 27;
                     while (hi - lo > 1)
                     {
                         const int mid = (lo + hi) >> 1;
@@ -436,18 +458,35 @@ static __device__ void b_cell(const KDesc &D, const int c)
                         else hi = mid;
                     }
                     const long js = nb_base[lo] + (g - nb_pre[lo]);
-                    int kj = D.kind[js];
-                    R x = P::px(D)[js] + nb_sx[lo];
-                    R y = P::py(D)[js] + nb_sy[lo];
-                    R z = P::pz(D)[js] + nb_sz[lo];
-                    if (ag >= 0 && D.grp[js] != ag) { kj = NOKIND; x = y = z = qnan; }
-                    T.put(m, x, y, z, WITH_FP ? (R)D.aux[js] : (R)0, kj);
+                    x = P::px(D)[js] + nb_sx[lo];
+                    y = P::py(D)[js] + nb_sy[lo];
+                    z = P::pz(D)[js] + nb_sz[lo];
+                    const R ex = fmax(fmax(bx0 - x, x - bx1), (R)0);
+                    const R ey = fmax(fmax(by0 - y, y - by1), (R)0);
+                    const R ez = fmax(fmax(bz0 - z, z - bz1), (R)0);
+                    keep = ex * ex + ey * ey + ez * ez <= cull2 && (ag < 0 || D.grp[js] == ag);
+                    if (keep)
+                    {
+                        kj = D.kind[js];
+                        if (WITH_FP) fpv = (R)D.aux[js];
+                    }
                 }
-                else
-                    T.put(m, qnan, qnan, qnan, (R)0, NOKIND);
+                const unsigned long long bal = __ballot(keep);
+                if (keep)
+                {
+                    const int off = nt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                                       __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+                    T.put(off, x, y, z, fpv, kj);
+                    if (lo == 13) selfpos[g - nb_pre[13]] = (short)off;
+                }
+                nt += __popcll(bal);
+                gpos += WG;
             }
+            const int trips = (nt + S - 1) / S;
+            const int npad = trips * S;
+            for (int m = nt + lane; m < npad; m += WG) T.put(m, qnan, qnan, qnan, (R)0, NOKIND);
             __syncthreads();
-            const int selft = selfm - cs;
+            const int selft = (s < S) ? selfpos[h0 + a] : -1;
             for (int g0 = 0; g0 < trips; g0 += 64)
             {
                 const int ng = min(64, trips - g0);
