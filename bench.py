@@ -36,6 +36,9 @@ METRIC = "atom-steps/s EAM Cu 4M atoms (%HBM roofline); batched jobs/s + p99 vs 
 HBM_PEAK_GBS = 8000.0
 # algorithmic HBM bytes per atom of the EAM force kernel (pass 0): read x (24) + F' (8), write F (24)
 PASS0_BYTES = {F.FMDX_PREC_FP64: 56.0, F.FMDX_PREC_MIXED: 12.0 + 8.0 + 24.0}
+PASS0_KERNEL = {F.FMDX_PREC_FP64: "k_cell<double,0,EAM1>", F.FMDX_PREC_MIXED: "k_cell<float,0,EAM1>"}
+# HBM bytes per launch of the dominant kernels, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "eam_cu_4m_traffic.json")
 # algorithmic HBM bytes per atom-step of the whole EAM step (integrate 120 + bin 8 + rho 32 + force 56)
 STEP_BYTES = {F.FMDX_PREC_FP64: 216.0, F.FMDX_PREC_MIXED: 120.0 + 8.0 + 28.0 + 12.0 + 44.0}
 
@@ -315,6 +318,34 @@ def nve_drift(cells, steps, precision, setfl, seed=1, dt=2e-3):
                 max_dev_eV_per_atom=float(np.max(np.abs(np.array(es) - e0)) / n))
 
 
+def measured_traffic(kernel, natoms):
+    """HBM bytes per launch of `kernel` on this workload, from the committed counter summary"""
+    try:
+        with open(TRAFFIC_FILE) as fp:
+            t = json.load(fp)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    if not k or k.get("natoms") != natoms:
+        return None
+    return k.get("hbm_bytes")
+
+
+def copy_calibration(mib):
+    """one plain device-to-device copy of `mib` MiB, three times: the reference reading of the
+    FETCH_SIZE / WRITE_SIZE counters (each copy reads and writes exactly mib MiB)"""
+    import torch
+    n = mib << 20
+    a = torch.empty(n, dtype=torch.uint8, device="cuda")
+    a.fill_(1)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log("bench: copied %d bytes x3" % n)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -335,6 +366,8 @@ def main():
     ap.add_argument("--ttm-steps", type=int, default=200)
     ap.add_argument("--no-drift", action="store_true", help="skip the NVE drift measurement")
     ap.add_argument("--drift-steps", type=int, default=500)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain MIB-MiB device copy (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
@@ -342,6 +375,9 @@ def main():
     if F.device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     prec = F.FMDX_PREC_FP64 if args.precision == "fp64" else F.FMDX_PREC_MIXED
     with tempfile.TemporaryDirectory() as td:
         setfl = os.path.join(td, "Cu_synth.eam.alloy")
@@ -402,8 +438,8 @@ def main():
                                % (args.cells, args.cells, args.cells, n),
                    "natoms": n, "precision": args.precision},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_pairs64<0> (EAM force pass)" if prec == F.FMDX_PREC_FP64 else "k_pairs32<0>",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(PASS0_KERNEL[prec], n),
+                     "kernel": PASS0_KERNEL[prec] + " (EAM force pass)",
                      "kernel_ms": r["pass0_ms"], "algorithmic_bytes_per_atom": PASS0_BYTES[prec]},
         "step_roofline_frac": STEP_BYTES[prec] * value / (HBM_PEAK_GBS * 1e9),
         "pass1_ms": r["pass1_ms"],

@@ -625,17 +625,44 @@ static __device__ void b_cell(const KDesc &D, const int c)
     }
 }
 
-/* Workgroups are dispatched to the 8 XCDs round-robin; give each XCD a
-   contiguous range of cells so that neighbour cells share its L2. */
-static __device__ __forceinline__ int xcd_cell(int b, int n)
+/* Cell visiting order of the single-system pair kernels.  Workgroups are
+   dispatched to the 8 XCDs round-robin, so block b runs on XCD b % 8; each
+   XCD gets a contiguous run of a sequence that walks the cells in y-strips
+   of STRIP_W cells: strip by strip, plane x by plane x, then y and z.  A
+   cell's atoms are staged by the 27 cells around it; in this order they are
+   reused within about four strip-planes (STRIP_W * nc2 cells each, about
+   2 MB for 4M-atom copper), which stay in the XCD's 4 MB L2, where the
+   storage order (z fastest, then y, then x) would need three whole planes. */
+#define STRIP_W 16
+static __device__ __forceinline__ int xcd_cell(const KDesc &D, int b, int n)
 {
     const int q = n >> 3, r = n & 7;
     const int x = b & 7, i = b >> 3;
-    return x * q + min(x, r) + i;
+    const int k = x * q + min(x, r) + i;   /* position in the strip sequence */
+    const int full = D.nc1 / STRIP_W;
+    const int per_strip = D.nc0 * STRIP_W * D.nc2;
+    int s, j, w;
+    if (k < full * per_strip)
+    {
+        s = k / per_strip;
+        j = k - s * per_strip;
+        w = STRIP_W;
+    }
+    else
+    {
+        s = full;
+        j = k - full * per_strip;
+        w = D.nc1 - full * STRIP_W;
+    }
+    const int px = j / (w * D.nc2);
+    const int rem = j - px * w * D.nc2;
+    const int py = s * STRIP_W + rem / D.nc2;
+    const int pz = rem % D.nc2;
+    return (px * D.nc1 + py) * D.nc2 + pz;
 }
 
 template <typename R, int PASS, int CAT>
-__global__ __launch_bounds__(WG) void k_cell(KDesc D) { b_cell<R, PASS, CAT>(D, xcd_cell(blockIdx.x, gridDim.x)); }
+__global__ __launch_bounds__(WG) void k_cell(KDesc D) { b_cell<R, PASS, CAT>(D, xcd_cell(D, blockIdx.x, gridDim.x)); }
 
 template <typename R, int PASS, int CAT>
 __global__ __launch_bounds__(WG) void kb_cell(BatchMap M)
