@@ -467,7 +467,8 @@ static __device__ void b_cell(const KDesc &D, const int c)
                     keep = ex * ex + ey * ey + ez * ez <= cull2 && (ag < 0 || D.grp[js] == ag);
                     if (keep)
                     {
-                        kj = D.kind[js];
+                        /* only the generic kernel dispatches on the candidate's kind */
+                        if (CAT == CAT_GEN) kj = D.kind[js];
                         if (WITH_FP) fpv = (R)D.aux[js];
                     }
                 }
