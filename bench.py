@@ -318,8 +318,9 @@ def nve_drift(cells, steps, precision, setfl, seed=1, dt=2e-3):
                 max_dev_eV_per_atom=float(np.max(np.abs(np.array(es) - e0)) / n))
 
 
-def measured_traffic(kernel, natoms):
-    """HBM bytes per launch of `kernel` on this workload, from the committed counter summary"""
+def measured_counters(kernel, natoms):
+    """rocprofv3 figures of `kernel` on this workload from the committed counter summary:
+    HBM bytes per launch and the busy fractions of VALU, TA and TD (None when not profiled)"""
     try:
         with open(TRAFFIC_FILE) as fp:
             t = json.load(fp)
@@ -328,7 +329,12 @@ def measured_traffic(kernel, natoms):
     k = t.get("kernels", {}).get(kernel)
     if not k or k.get("natoms") != natoms:
         return None
-    return k.get("hbm_bytes")
+    return k
+
+
+def measured_traffic(kernel, natoms):
+    k = measured_counters(kernel, natoms)
+    return k.get("hbm_bytes") if k else None
 
 
 def copy_calibration(mib):
@@ -440,7 +446,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(PASS0_KERNEL[prec], n),
                      "kernel": PASS0_KERNEL[prec] + " (EAM force pass)",
-                     "kernel_ms": r["pass0_ms"], "algorithmic_bytes_per_atom": PASS0_BYTES[prec]},
+                     "kernel_ms": r["pass0_ms"], "algorithmic_bytes_per_atom": PASS0_BYTES[prec],
+                     "unit_busy": {k: v for k, v in (measured_counters(PASS0_KERNEL[prec], n) or {}).items()
+                                   if k.endswith("_busy")} or None},
         "step_roofline_frac": STEP_BYTES[prec] * value / (HBM_PEAK_GBS * 1e9),
         "pass1_ms": r["pass1_ms"],
     }
