@@ -10,6 +10,14 @@ the reference's 1-D turi; the reference has no 3-D solver, ttm.c:451-452):
   (kappa = Ce v^2 / (3 B Ti)); the two agree to rounding;
 * with G = 0 and no source the flux form conserves sum(gamma Te dTe) to
   rounding (telescoping face fluxes, periodic x,y, inactive z boundaries).
+
+and of the reference's 1-D type-2 solver with Zhigilei conductivity (A != 0),
+using example 04's Ce and G tables:
+
+* one sub-step balances sum Ce dTe against the coupling and the laser source
+  (the face fluxes cancel), with Ce and G from scipy's natural splines;
+* two cells exchange heat at the mean of the kinetic-theory conductivities
+  Ce v^2 / (3 (A Te^2 + B Ti)).
 """
 import math
 
@@ -169,3 +177,96 @@ def test_flux_form_conserves_electron_energy(rng, kmode):
     de = Te[act] * (Te2[act] - Te[act])            # gamma V / dt times the flux sum
     assert abs(de.sum()) < 1e-11 * np.abs(de).sum()
     assert np.abs(de).sum() > 0
+
+
+def _table(path, tscale, vscale):
+    d = np.loadtxt(path)
+    return np.ascontiguousarray(d[:, 0] * tscale), np.ascontiguousarray(d[:, 1] * vscale)
+
+
+@pytest.mark.parametrize("with_source", [False, True])
+def test_type2_zhigilei_sub_step_conserves_energy(rng, data_dir, with_source):
+    """Reference type 2 with example 04's tables and Zhigilei constants (A != 0):
+    the face conductivities are shared by both cells of a face (ttm.c:341-348)
+    and inactive neighbours give zero flux, so one explicit sub-step satisfies
+        sum_i Ce_i (Te'_i - Te_i) = dt sum_i (-G_i (Te_i - Ti_i) + S_i)
+    whatever Ke is; Ce_i and G_i are evaluated here with scipy's natural cubic
+    splines of the tables, independently of the oracle.  A face conductivity
+    taken from one side only, a flux with a wrong index or a term divided by
+    the wrong Ce breaks the identity."""
+    from scipy.interpolate import CubicSpline
+    import os
+    T_C, Ct = _table(os.path.join(data_dir, "Ce_Cu.dat"), 1e4, 1e5 * J_M3K2)
+    T_G, Gt = _table(os.path.join(data_dir, "G_Cu.dat"), 1e4, 1e17 * W_M3K)
+    nz = 24
+    Te = np.full(nz + 2, -1.0)
+    Te[2:nz] = rng.uniform(1000.0, 30000.0, nz - 2)
+    Te[nz // 2] = -1.0                          # a deactivated cell inside the slab
+    Ti = np.zeros(nz + 2)
+    Ti[2:nz] = rng.uniform(300.0, 900.0, nz - 2)
+    num = np.zeros(nz + 2, np.uint32)
+    num[2:nz] = 30
+    num[nz // 2] = 0
+    c = cfg(nz, ratio=1, laser=LASER if with_source else None)
+    c.type = 2
+    c.Kzh_A, c.Kzh_B, c.Kzh_v2 = 2.66e6 / 1e12, 2.41e11 / 1e12, (1.57e6 * 1e-2) ** 2
+    c.nC, c.nG = len(T_C), len(T_G)
+    c.T_C, c.Ct, c.T_G, c.Gt = T_C.ctypes.data, Ct.ctypes.data, T_G.ctypes.data, Gt.ctypes.data
+    keep = (T_C, Ct, T_G, Gt)
+    o = Oracle([20.0, 20.0, 10.0 * (nz + 2)], [1, 1, 0], [63.546], ["Cu"])
+    o.ttm_init(c, 300.0)
+    dt, t = 2e-5, LASER["t0"] * 1e12
+    o.ttm_set_fields(Te, Ti, num, frontsurf=10.0, min_atoms=5)
+    o.ttm_solve_only(t if with_source else 0.0, dt)
+    Te2 = o.ttm_arrays(nz + 2)["Te"]
+    act = Te >= 0
+    assert np.array_equal(act, Te2 >= 0)
+    Ce = CubicSpline(T_C, Ct, bc_type="natural")(Te[act])
+    G = CubicSpline(T_G, Gt, bc_type="natural")(Te[act])
+    lhs = Ce * (Te2[act] - Te[act])
+    rhs = -dt * G * (Te[act] - Ti[act])
+    if with_source:
+        # the reference's Gaussian pulse at the centre of each cell, ttm.c:327-337
+        fs, _, h = o.ttm_params()
+        z = (np.nonzero(act)[0] - 1 + 0.5) * h
+        rhs = rhs + dt * c.laser_factor * np.exp(c.laser_2sig2_inv * (t - c.laser_t0) ** 2) * \
+            np.exp(c.laser_absdepth_inv * (z - fs))
+    scale = np.abs(lhs).sum()
+    assert np.abs(lhs - rhs).max() > 1e-6 * scale      # cells exchange heat: the per-cell identity fails
+    assert abs(lhs.sum() - rhs.sum()) < 1e-10 * scale  # the sum holds
+    del keep
+
+
+def test_type2_two_cell_exchange_follows_zhigilei_conductivity(rng, data_dir):
+    """Two active cells: the heat passed across their face in one sub-step,
+    (Ce_1 dTe_1 / dt + G_1 (Te_1 - Ti_1)) dz^2 / (Te_2 - Te_1), equals the mean
+    of the kinetic-theory conductivities K = Ce v^2 tau / 3 of the two cells
+    with the Zhigilei scattering rate 1/tau = A Te^2 + B Ti.  Checked at two
+    values of A so that both scattering terms are exercised."""
+    from scipy.interpolate import CubicSpline
+    import os
+    T_C, Ct = _table(os.path.join(data_dir, "Ce_Cu.dat"), 1e4, 1e5 * J_M3K2)
+    T_G, Gt = _table(os.path.join(data_dir, "G_Cu.dat"), 1e4, 1e17 * W_M3K)
+    ce = CubicSpline(T_C, Ct, bc_type="natural")
+    gg = CubicSpline(T_G, Gt, bc_type="natural")
+    B, v2 = 2.41e11 / 1e12, (1.57e6 * 1e-2) ** 2
+    for A in (2.66e6 / 1e12, 4 * 2.66e6 / 1e12):
+        Te = np.array([-1.0, 4000.0, 15000.0, -1.0])
+        Ti = np.array([0.0, 400.0, 700.0, 0.0])
+        num = np.array([0, 30, 30, 0], np.uint32)
+        c = cfg(2, ratio=1)
+        c.type = 2
+        c.Kzh_A, c.Kzh_B, c.Kzh_v2 = A, B, v2
+        c.nC, c.nG = len(T_C), len(T_G)
+        c.T_C, c.Ct, c.T_G, c.Gt = T_C.ctypes.data, Ct.ctypes.data, T_G.ctypes.data, Gt.ctypes.data
+        o = Oracle([20.0, 20.0, 40.0], [1, 1, 0], [63.546], ["Cu"])
+        o.ttm_init(c, 300.0)
+        dt = 1e-5
+        o.ttm_set_fields(Te, Ti, num, frontsurf=0.0, min_atoms=5)
+        o.ttm_solve_only(0.0, dt)
+        Te2 = o.ttm_arrays(4)["Te"]
+        h = o.ttm_params()[2]
+        k_obs = (ce(Te[1]) * (Te2[1] - Te[1]) / dt + gg(Te[1]) * (Te[1] - Ti[1])) * h * h / (Te[2] - Te[1])
+        kin = [ce(Te[i]) * v2 / 3.0 / (A * Te[i] ** 2 + B * Ti[i]) for i in (1, 2)]
+        assert k_obs == pytest.approx(0.5 * (kin[0] + kin[1]), rel=1e-9)
+        del c
