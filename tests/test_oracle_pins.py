@@ -6,7 +6,8 @@
 * spline preparation: scipy.interpolate.CubicSpline(bc_type='natural');
 * EAM r^2 tables: the analytic functions they tabulate;
 * symmetry: a perfect periodic FCC lattice has zero force and equal density;
-* conservation: total force is zero with full periodicity; translation invariance.
+* conservation: total force is zero with full periodicity; translation invariance;
+* the EAM + LJ mixture: brute force composed from its three pair families.
 """
 import math
 
@@ -220,3 +221,70 @@ def test_spline_prepare_matches_scipy(rng):
     xk = np.sort(rng.uniform(0, 10, 60))
     yk = np.sin(xk)
     assert rel_err(O.spline_prepare_adv(xk, yk), CubicSpline(xk, yk, bc_type="natural")(xk, 2)) < 1e-9
+
+
+def test_hybrid_eam_lj_brute_force(rng, cu01):
+    """Example 02's mixture (Cu EAM, Ar-Ar LJ, Cu-Ar LJ; examples/02_collision.c:78-85)
+    against an O(N^2) brute force that composes the three parts: densities of Cu
+    atoms from Cu neighbours only, EAM forces between Cu atoms, LJ forces with the
+    pair's own parameters for Ar-Ar and Cu-Ar, energy F(rho) + phi/2 + LJ/2."""
+    xcu = lattice_fcc((4, 4, 4), 3.6316, rng, jitter=0.03, origin=(2.0, 2.0, 2.0))
+    xar = lattice_fcc((3, 3, 3), 5.26, rng, jitter=0.05, origin=(17.0, 2.0, 2.0))
+    x = np.vstack([xcu, xar])
+    kind = np.r_[np.zeros(len(xcu)), np.ones(len(xar))].astype(np.int32)
+    L = [36.0, 26.0, 26.0]
+    lj_ar, lj_cuar = (3.40, 0.0104, 2.5 * 3.40), (2.87, 0.0652, 2.5 * 2.87)
+    pots = [("eam", cu01, [(0, 0)]), ("lj", 1, 1) + lj_ar, ("lj", 0, 1) + lj_cuar]
+    o, _ = build(L, [1, 1, 1], [63.546, 39.948], ["Cu", "Ar"], pots, x, kind=kind, device=False)
+    o.forces()
+    a = o.atoms()
+    t = o.eam_tables()
+    rr = np.arange(t["nr2"]) * t["dr2"]
+    rho_s = CubicSpline(rr, t["rho"][0], bc_type=((2, 0.0), (2, 0.0)))
+    phi_s = CubicSpline(rr, t["phi"][0, 0], bc_type=((2, 0.0), (2, 0.0)))
+    Fs = CubicSpline(np.arange(t["nrho"]) * t["drho"], t["F"][0], bc_type=((2, 0.0), (2, 0.0)))
+    xs, ks = a["x"], a["kind"]
+    n = len(xs)
+    cu = ks == 0
+
+    def lj(r2, p):
+        sig, eps, rc = p
+        m = r2 < rc * rc
+        s6 = (sig * sig / r2) ** 3
+        return np.where(m, 4 * eps * (s6 * s6 - s6), 0.0), np.where(m, 4 * eps * (-6 * s6 * s6 + 3 * s6) / r2, 0.0)
+
+    geo = []
+    rho = np.zeros(n)
+    for i in range(n):
+        d = minimg(xs[i] - xs, L, (1, 1, 1))
+        r2 = np.einsum("ij,ij->i", d, d)
+        r2[i] = np.inf
+        geo.append((d, r2))
+        if cu[i]:
+            m = cu & (r2 < t["rc2"])
+            rho[i] = np.sum(rho_s(r2[m]))
+    Fp = np.where(cu, Fs(rho, 1), 0.0)
+    e = np.sum(Fs(rho[cu]))
+    f = np.zeros_like(xs)
+    for i in range(n):
+        d, r2 = geo[i]
+        if cu[i]:
+            m = cu & (r2 < t["rc2"])
+            mag = 2 * (Fp[i] * rho_s(r2[m], 1) + Fp[m] * rho_s(r2[m], 1) + phi_s(r2[m], 1))
+            f[i] -= np.sum(mag[:, None] * d[m], axis=0)
+            e += 0.5 * np.sum(phi_s(r2[m]))
+            other = ~cu
+            u, du = lj(r2[other], lj_cuar)
+        else:
+            u1, du1 = lj(r2[~cu], lj_ar)
+            u2, du2 = lj(r2[cu], lj_cuar)
+            f[i] -= 2.0 * np.sum(du1[:, None] * d[~cu], axis=0)
+            e += 0.5 * np.sum(u1)
+            other = cu
+            u, du = u2, du2
+        f[i] -= 2.0 * np.sum(du[:, None] * d[other], axis=0)
+        e += 0.5 * np.sum(u)
+    assert rel_err(a["f"], f) < 1e-9
+    assert rel_err(a["aux"][cu], Fp[cu]) < 1e-9
+    assert o.energies()["pe"] == pytest.approx(e, rel=1e-11)
+    assert np.max(np.abs(a["f"].sum(axis=0))) < 1e-9 * np.max(np.abs(f))
