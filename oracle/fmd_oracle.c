@@ -1786,6 +1786,21 @@ void orc_ttm_solve_only(orc_t *o, double time, double dt)
     }
 }
 
+/* presolve (activation / deactivation, ttm.c:194-248) and solve on injected fields */
+void orc_ttm_presolve_solve(orc_t *o, double time, double dt)
+{
+    ottm *t = &o->ttm;
+    if (t->dim3)
+    {
+        ttm_presolve3(o);
+        ttm_solve3(o, time, dt);
+        return;
+    }
+    ttm_presolve(o);
+    if (t->c.type == 2) ttm_type2_update(o);
+    ttm_solve(o, time, dt);
+}
+
 void orc_ttm_update(orc_t *o, double time, double dt)
 {
     ottm *t = &o->ttm;

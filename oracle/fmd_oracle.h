@@ -114,6 +114,7 @@ void orc_ttm_update(orc_t *o, double time, double dt);   /* fields + presolve + 
 void orc_ttm_set_fields(orc_t *o, const double *Te, const double *Ti, const unsigned *num, double frontsurf,
                         unsigned min_atoms);
 void orc_ttm_solve_only(orc_t *o, double time, double dt);
+void orc_ttm_presolve_solve(orc_t *o, double time, double dt);
 /* per turi-cell arrays, length ncx*ncy*(ncz+2), index (ix*ncy + iy)*(ncz+2) + iz (z ghosts included) */
 void orc_ttm_arrays(orc_t *o, double *Te, double *Ti, double *xi, double *vcmz, unsigned *num,
                     double *mass);

@@ -74,6 +74,7 @@ def lib():
         L.orc_ttm_params.argtypes = [P, P, P, P]
         L.orc_ttm_set_fields.argtypes = [P, P, P, P, C.c_double, C.c_uint]
         L.orc_ttm_solve_only.argtypes = [P, C.c_double, C.c_double]
+        L.orc_ttm_presolve_solve.argtypes = [P, C.c_double, C.c_double]
         L.orc_integrate_se.argtypes = [P, C.c_int, P, C.c_double, C.c_double]
         L.orc_spline_prepare.argtypes = [C.c_double, P, C.c_int, P]
         L.orc_spline_prepare_adv.argtypes = [P, P, C.c_int, P]
@@ -245,6 +246,9 @@ class Oracle:
 
     def ttm_solve_only(self, time, dt):
         self.L.orc_ttm_solve_only(self.h, time, dt)
+
+    def ttm_presolve_solve(self, time, dt):
+        self.L.orc_ttm_presolve_solve(self.h, time, dt)
 
     def ttm_params(self):
         fs, h = C.c_double(), C.c_double()

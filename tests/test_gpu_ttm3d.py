@@ -93,3 +93,41 @@ def test_ttm_grid_solver_reduces_to_1d(rng, cu01):
     f, turi, o, L = _run(rng, cu01, dims, proportional=False, grid_solver=True)
     _compare(f, turi, o, L, dims)
     f.free()
+
+
+def test_device_activation_order_matches_oracle(rng):
+    """The device presolve (one lane, cells in order) activates and deactivates
+    turi cells exactly as ttm.c:194-230: atoms placed so that the six cells hold
+    20, 30, 40, 2, 3 and 60 atoms, Te injected as 1000, -, -, 2000, -, - (K = G
+    = 0, no source, so the sub-step keeps what the presolve set); expected
+    1000, 1000, 1062.5, -1, -1, Ti of cell 6 (hand-derived in
+    test_oracle_order.py), and the oracle on the device's own fields agrees."""
+    import fmd_amd as F
+    from tests.pairsys import build
+    counts = [20, 30, 40, 2, 3, 60]
+    xs = []
+    for k, n in enumerate(counts):
+        for _ in range(n):
+            xs.append([rng.uniform(0.5, 9.5), rng.uniform(0.5, 9.5), k * 10.0 + rng.uniform(0.5, 9.5)])
+    x = np.array(xs)
+    v = rng.normal(0.0, 5.0, size=x.shape)
+    _, d = build([10.0, 10.0, 60.0], [1, 1, 0], [63.546], ["Cu"], [("lj", 0, 0, 2.0, 0.0001, 2.5)], x, v=v)
+    c = F.TtmCfg()
+    c.type, c.ncz, c.ratio = 1, 6, 1
+    c.tcellh_z, c.tcell_volume = 10.0, 1000.0
+    c.C_gamma, c.K, c.G = 96.6 * 6.2415091259e-12, 0.0, 0.0
+    c.laser_t0, c.laser_tdiff = 1e9, -1.0
+    c.frontsurf, c.min_atoms = 0.0, 5
+    c.ncx = c.ncy = 1
+    te0 = np.array([-1.0, 1000.0, -1.0, -1.0, 2000.0, -1.0, -1.0, -1.0])
+    d.ttm_setup(c, te0)
+    d.ttm_update(0.0, 1e-3)
+    r = d.ttm_download(8)
+    assert list(r["num"][1:7]) == counts
+    assert list(r["Te"][1:7]) == [1000.0, 1000.0, 1062.5, -1.0, -1.0, r["Ti"][6]]
+    oc = cfg(6, ratio=1, K=0.0, G=0.0)
+    o = Oracle([10.0, 10.0, 60.0], [1, 1, 0], [63.546], ["Cu"])
+    o.ttm_init(oc, 300.0)
+    o.ttm_set_fields(te0, r["Ti"], r["num"], frontsurf=0.0, min_atoms=5)
+    o.ttm_presolve_solve(0.0, 1e-3)
+    assert np.array_equal(o.ttm_arrays(8)["Te"][1:7], r["Te"][1:7])

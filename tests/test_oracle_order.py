@@ -165,3 +165,33 @@ def test_ttm_type1_single_substep(rng):
         xi = cfg.G * (L * L * tcz) * (Te2[i] - Ti) / (3 * O.KB * sel.sum() * Ti)
         assert r["xi"][i] == pytest.approx(xi, rel=1e-11)
     assert r["Te"][1] == -1.0 and r["Te"][4] == -1.0
+
+
+def test_ttm_activation_order_hand_derived():
+    """Cell activation and deactivation of the 1-D presolve (ttm.c:194-230), on
+    injected fields, with K = G = 0 and no source so that the sub-step leaves Te
+    as the presolve set it.  The loop runs upwards: a cell activated in this pass
+    lends its new Te to the next cell, while the next cell's old Te is used
+    even if that cell is deactivated later in the same pass.  Hand-derived:
+      cell 1: active, kept                         1000
+      cell 2: inactive, 30 atoms: from cell 1      (1000*20)/20 = 1000
+      cell 3: inactive, 40 atoms: cells 2 (new) and 4 (old, 2 atoms)
+                                                   (1000*30 + 2000*2)/32 = 1062.5
+      cell 4: 2 atoms < 5: deactivated             -1
+      cell 5: 3 atoms: stays inactive              -1
+      cell 6: inactive, 60 atoms, no active neighbour: Ti = 350"""
+    cfg = O.TtmCfg()
+    cfg.type, cfg.ncz, cfg.ratio = 1, 6, 1
+    cfg.C_gamma, cfg.K, cfg.G = 96.6 * 6.2415091259e-12, 0.0, 0.0
+    cfg.act_frac, cfg.start, cfg.stop = 0.1, 0.0, -1.0
+    cfg.laser_tdiff = -1.0
+    cfg.laser_t0 = 1e9
+    o = O.Oracle([10.0, 10.0, 60.0], [1, 1, 0], [63.546], ["Cu"])
+    o.ttm_init(cfg, 300.0)
+    Te = np.array([-1.0, 1000.0, -1.0, -1.0, 2000.0, -1.0, -1.0, -1.0])
+    Ti = np.array([0.0, 300.0, 310.0, 320.0, 330.0, 340.0, 350.0, 0.0])
+    num = np.array([0, 20, 30, 40, 2, 3, 60, 0], np.uint32)
+    o.ttm_set_fields(Te, Ti, num, frontsurf=0.0, min_atoms=5)
+    o.ttm_presolve_solve(0.0, 1e-3)
+    got = o.ttm_arrays(8)["Te"]
+    assert list(got[1:7]) == [1000.0, 1000.0, 1062.5, -1.0, -1.0, 350.0]
