@@ -18,6 +18,21 @@ def lattice_fcc(n, a, rng=None, jitter=0.0, origin=(0.0, 0.0, 0.0)):
     return x
 
 
+def random_separated(n, L, dmin, rng):
+    """n uniform points in the periodic box L, at least dmin apart under the minimum image"""
+    L = np.asarray(L, dtype=np.float64)
+    pts = []
+    while len(pts) < n:
+        p = rng.uniform(0.0, 1.0, 3) * L
+        if pts:
+            d = np.asarray(pts) - p
+            d -= L * np.round(d / L)
+            if np.min(np.einsum("ij,ij->i", d, d)) < dmin * dmin:
+                continue
+        pts.append(p)
+    return np.array(pts)
+
+
 def build(l, pbc, masses_amu, names, pots, x, v=None, group=None, kind=None,
           precision=F.FMDX_PREC_FP64, device=True):
     """pots: ('lj', k1, k2, sigma, eps, rc) | ('morse', k1, k2, D0, alpha, r0, rc) | ('eam', path, [(k1, k2), ...])"""

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import fmd_amd as F
-from tests.pairsys import build, lattice_fcc, rel_err
+from tests.pairsys import build, lattice_fcc, random_separated, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -89,3 +89,17 @@ def test_mixed_precision_lj(rng):
     o, d = build([L] * 3, [1, 1, 1], [AR["m"]], ["Ar"], [("lj", 0, 0, AR["sig"], AR["eps"], AR["rc"])], x,
                  precision=F.FMDX_PREC_MIXED)
     _compare(o, d, 1e-5, 1e-6)
+
+
+@pytest.mark.parametrize("precision", [F.FMDX_PREC_FP64, F.FMDX_PREC_MIXED])
+def test_degenerate_boxes_1_and_2_cells(rng, precision):
+    # 1, 2 and 3 cells along x, y, z: the stencil meets the same cell through
+    # several images, and an atom pairs with its own images (pinned on the
+    # oracle by an all-images brute force in test_oracle_pins.py)
+    rc = AR["rc"]
+    L = [1.5 * rc, 2.3 * rc, 3.2 * rc]
+    x = random_separated(60, L, 3.0, rng)
+    o, d = build(L, [1, 1, 1], [AR["m"]], ["Ar"], [("lj", 0, 0, AR["sig"], AR["eps"], rc)], x, precision=precision)
+    assert list(o.nc) == [1, 2, 3]
+    tol = (1e-10, 1e-10) if precision == F.FMDX_PREC_FP64 else (1e-5, 1e-6)
+    _compare(o, d, *tol)

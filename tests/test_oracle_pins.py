@@ -16,7 +16,7 @@ import pytest
 from scipy.interpolate import CubicSpline
 
 from oracle import oracle as O
-from tests.pairsys import build, lattice_fcc, rel_err
+from tests.pairsys import build, lattice_fcc, random_separated, rel_err
 
 MU = O.MASS_UNIT
 
@@ -288,3 +288,44 @@ def test_hybrid_eam_lj_brute_force(rng, cu01):
     assert rel_err(a["aux"][cu], Fp[cu]) < 1e-9
     assert o.energies()["pe"] == pytest.approx(e, rel=1e-11)
     assert np.max(np.abs(a["f"].sum(axis=0))) < 1e-9 * np.max(np.abs(f))
+
+
+def brute_images(x, L, pair):
+    """all pairs with j's periodic images shifted by n L, n in {-1,0,1}^3 (self only for n != 0);
+    exact whenever every L >= rc, since farther images lie beyond the cutoff"""
+    n = len(x)
+    f = np.zeros_like(x)
+    e = 0.0
+    shifts = np.array([[a, b, c] for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1)], dtype=float) * L
+    for i in range(n):
+        for s in shifts:
+            d = x[i] - (x + s)
+            r2 = np.einsum("ij,ij->i", d, d)
+            m = r2 > 0
+            u, du = pair(r2[m])
+            e += 0.5 * np.sum(u)
+            f[i] -= 2.0 * np.sum(du[:, None] * d[m], axis=0)
+    return f, e
+
+
+def test_lj_degenerate_boxes_all_images(rng):
+    """Boxes with 1 and 2 cells along a dimension (md-ghost.c:85-107 copies the
+    single cell to both ghost faces): an atom pairs with several images of the
+    same neighbour and with its own images."""
+    sig, eps, rc = 3.405, 0.0104, 8.5125
+    L = [1.5 * rc, 2.3 * rc, 3.2 * rc]
+    n = 60
+    x = random_separated(n, L, 3.0, rng)
+    o, _ = build(L, [1, 1, 1], [39.948], ["Ar"], [("lj", 0, 0, sig, eps, rc)], x, device=False)
+    assert list(o.nc) == [1, 2, 3]
+    o.forces()
+    a = o.atoms()
+
+    def lj(r2):
+        m = r2 < rc * rc
+        s6 = (sig * sig / np.where(m, r2, 1.0)) ** 3
+        return np.where(m, 4 * eps * (s6 * s6 - s6), 0.0), np.where(m, 4 * eps * (-6 * s6 * s6 + 3 * s6) / r2, 0.0)
+
+    f, e = brute_images(a["x"], np.array(L), lj)
+    assert rel_err(a["f"], f) < 1e-11
+    assert o.energies()["pe"] == pytest.approx(e, rel=1e-12)
