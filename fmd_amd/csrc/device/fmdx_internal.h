@@ -10,7 +10,8 @@
  *             of those slots is the reference's within-cell atom order.
  *   per slot: x y z vx vy vz fx fy fz aux (f64), xr yr zr (f32, cell-relative,
  *             mixed build), grp (i32), aid (u32), kind (u8), mv (i32 binning
- *             destination), code (u8 migration code)
+ *             destination during a re-binning, the atom's turi index during
+ *             the TTM field sums), code (u8 migration code)
  */
 #ifndef FMDX_INTERNAL_H
 #define FMDX_INTERNAL_H

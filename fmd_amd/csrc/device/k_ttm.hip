@@ -1,144 +1,166 @@
 /*
- * k_ttm.hip -- 1-D two-temperature model on the device.
+ * k_ttm.hip -- two-temperature model on the device.
  *
- *   k_ttm_sum_mv     update_field_vcm_and_mass, turi.c:1112-1178
- *   k_ttm_sum_temp   update_field_temperature_and_number, turi.c:1190-1261
+ *   k_ttm_sums       update_field_vcm_and_mass + update_field_temperature_and_number,
+ *                    turi.c:1112-1261
  *   k_ttm_presolve   presolve_common_types_1_2 (+ type2_update_Ce_Ke_G), ttm.c:154-248
  *   k_ttm_solve      ttm_type1_solve_1d / ttm_type2_solve_1d, ttm.c:250-383
  *
- * The atom sums are segmented reductions into turi cells: every block first
- * accumulates into an LDS histogram and then adds its non-empty bins to
- * HBM with one atomic per bin, so the atomic count is about one per block.
- * The FD grid is small (<= FMDX_TTM_MAX_CELLS cells); one workgroup keeps
- * Te, Te2, Ti, Ke, Ce, G and xi in LDS for all sub-steps of an MD step.
- * The activation scan of presolve is sequential in the reference (a cell
- * activated earlier in the scan feeds the next), so one lane runs it.
+ * The atom sums are gathers, not scatters: one workgroup per turi cell walks
+ * the MD cells that can hold its atoms (the cells its box overlaps, with a
+ * margin of one for rounding), keeps the atoms whose turi index is its own,
+ * sums them lane by lane and combines the lanes in a fixed tree, first m and
+ * m v (then vcm), then m |v - vcm|^2 and the count (then Ti).  No atomics:
+ * every run gives the same bits.  The FD grid is small (<= FMDX_TTM_MAX_CELLS
+ * cells); one workgroup keeps Te, Te2, Ti, Ke, Ce, G and xi in LDS for all
+ * sub-steps of an MD step.  The activation scan of presolve is sequential in
+ * the reference (a cell activated earlier in the scan feeds the next), so one
+ * lane runs it.
  *
  * ncx x ncy x ncz turies (no counterpart in the reference, whose 3-D
  * updater is missing, ttm.c:451-452): type 1 with Ce = gamma Te, constant G,
  * and constant conductivity or kappa = K Te / Ti, periodic in x and y.  The
- * atom sums go straight to HBM atomics (the grid is too large for per-block
- * LDS histograms); one workgroup of 1024 lanes keeps Te and Te2 of up to
+ * presolve runs one lane per (x, y) column, each scanning z as the 1-D scan;
+ * one workgroup of 1024 lanes keeps Te (and the cells' kappa) of up to
  * FMDX_TTM_MAX_TCELLS cells in LDS for the sub-steps, each lane owning the
- * same cells throughout so that its xi sums stay in registers.
+ * same cells throughout so that its new values and xi sums stay in registers.
  */
 #include "fmdx_internal.h"
 
 #define TTPB 256
+#define T3PB_P 256   /* lanes of the grid presolve (one per column) */
 
 
 
-static __device__ __forceinline__ int ttm_bin(const TtmState &S, double z)
+
+/* turi cell of a position on an ncx x ncy x ncz turi: x and y clamped to the
+   grid; a non-finite or far-away z goes to the lower ghost cell (the
+   comparisons run in FP64 so that no out-of-range conversion happens) */
+static __device__ __forceinline__ int tcell3(const TtmState &S, double x, double y, double z)
 {
-    return (int)(z / S.tcellh_z) + 1;
+    const double fx = x / S.tcellh_x, fy = y / S.tcellh_y, fz = z / S.tcellh_z;
+    const int ix = fx >= 0.0 ? (fx < S.ncx ? (int)fx : S.ncx - 1) : 0;
+    const int iy = fy >= 0.0 ? (fy < S.ncy ? (int)fy : S.ncy - 1) : 0;
+    const int iz = (fz >= -1.0 && fz < S.nzg - 1) ? (int)fz + 1 : 0;
+    return (ix * S.ncy + iy) * S.nzg + iz;
 }
 
-__global__ __launch_bounds__(TTPB) void k_ttm_clear(TtmState S)
+/* MD cells [lo, hi] along one dimension that can hold atoms of turi slab i of
+   n (width tw; MD cells of width h, nc of them): the overlap plus one cell of
+   margin; the first and last slabs also take everything beyond them */
+static __device__ __forceinline__ void md_range(int i, int n, double tw, double h, int nc, int &lo, int &hi)
 {
-    int i = blockIdx.x * TTPB + threadIdx.x;
-    if (i >= S.n) return;
-    S.mass[i] = 0.0;
-    S.vcm[3 * i] = S.vcm[3 * i + 1] = S.vcm[3 * i + 2] = 0.0;
-    S.Ti[i] = 0.0;
-    S.num[i] = 0;
+    lo = i <= 0 ? 0 : (int)floor(i * tw / h) - 1;
+    hi = i >= n - 1 ? nc - 1 : (int)floor((i + 1) * tw / h) + 1;
+    lo = lo < 0 ? 0 : (lo > nc - 1 ? nc - 1 : lo);
+    hi = hi < lo ? lo : (hi > nc - 1 ? nc - 1 : hi);
 }
 
-/* sum of m and m v per turi cell (all atoms, no group filter, as in the reference) */
-__global__ __launch_bounds__(TTPB) void k_ttm_sum_mv(KDesc D, TtmState S)
+/* sum of lane values in a fixed order (tree over the workgroup); every lane gets the result */
+static __device__ __forceinline__ double block_sum(double v, double *sh)
 {
-    __shared__ double hm[FMDX_TTM_MAX_CELLS + 2], hx[FMDX_TTM_MAX_CELLS + 2], hy[FMDX_TTM_MAX_CELLS + 2],
-        hz[FMDX_TTM_MAX_CELLS + 2];
-    __shared__ int lo, hi;
-    for (int i = threadIdx.x; i < S.n; i += TTPB) hm[i] = hx[i] = hy[i] = hz[i] = 0.0;
-    if (threadIdx.x == 0) { lo = S.n; hi = -1; }
     __syncthreads();
-    long slot = (long)blockIdx.x * TTPB + threadIdx.x;
-    if (slot < (long)D.ncell * D.cap)
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = TTPB / 2; o > 0; o >>= 1)
     {
-        int c = (int)(slot / D.cap);
-        int k = (int)(slot - (long)c * D.cap);
-        if (k < D.cnt[c])
-        {
-            int iz = ttm_bin(S, D.z[slot]);
-            if (iz >= 0 && iz < S.n)
-            {
-                double m = D.mass[D.kind[slot]];
-                atomicAdd(&hm[iz], m);
-                atomicAdd(&hx[iz], m * D.vx[slot]);
-                atomicAdd(&hy[iz], m * D.vy[slot]);
-                atomicAdd(&hz[iz], m * D.vz[slot]);
-                atomicMin(&lo, iz);
-                atomicMax(&hi, iz);
-            }
-        }
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
     }
-    __syncthreads();
-    for (int i = lo + threadIdx.x; i <= hi; i += TTPB)
-        if (hm[i] != 0.0)
-        {
-            atomicAdd(&S.mass[i], hm[i]);
-            atomicAdd(&S.vcm[3 * i], hx[i]);
-            atomicAdd(&S.vcm[3 * i + 1], hy[i]);
-            atomicAdd(&S.vcm[3 * i + 2], hz[i]);
-        }
+    return sh[0];
 }
 
-__global__ __launch_bounds__(TTPB) void k_ttm_vcm(TtmState S)
+/* turi index of every atom, into the binning scratch word mv (free between
+   steps: the next re-binning rewrites it for every atom before reading it).
+   1-D turi (G3 false): (int)(z / tcellh_z) + 1 as turi.c:1136, -1 outside
+   [0, n); grid turi: tcell3. */
+template <bool G3>
+__global__ __launch_bounds__(TPB) void k_ttm_index(KDesc D, TtmState S)
 {
-    int i = blockIdx.x * TTPB + threadIdx.x;
-    if (i < 1 || i > S.ncz) return;
-    for (int d = 0; d < 3; d++) S.vcm[3 * i + d] /= S.mass[i];
+    each_slot(D, blockIdx.x, [&](long slot, int c) {
+        (void)c;
+        int a;
+        if (G3) a = tcell3(S, D.x[slot], D.y[slot], D.z[slot]);
+        else
+        {
+            a = (int)(D.z[slot] / S.tcellh_z) + 1;
+            a = (a >= 0 && a < S.n) ? a : -1;
+        }
+        D.mv[slot] = a;
+    });
 }
 
-/* sum of m |v - vcm|^2 and atom count per turi cell */
-__global__ __launch_bounds__(TTPB) void k_ttm_sum_temp(KDesc D, TtmState S)
+/* Per turi cell a (one workgroup): mass, vcm, Ti and the atom count of the
+   atoms whose turi index is a (all atoms, no group filter, as in the
+   reference), from the MD cells that can hold them, 16 lanes per MD cell.
+   1-D turi (G3 false): as turi.c:1146-1150, 1244-1248 vcm and Ti are divided
+   in every owned cell 1..ncz, empty or not, and the ghost cells keep the raw
+   sums.  Grid turi (G3): only non-empty cells are divided. */
+template <bool G3>
+__global__ __launch_bounds__(TTPB) void k_ttm_sums(KDesc D, TtmState S)
 {
 #pragma clang fp contract(off)
-    __shared__ double ht[FMDX_TTM_MAX_CELLS + 2];
-    __shared__ unsigned hn[FMDX_TTM_MAX_CELLS + 2];
-    __shared__ int lo, hi;
-    for (int i = threadIdx.x; i < S.n; i += TTPB) { ht[i] = 0.0; hn[i] = 0; }
-    if (threadIdx.x == 0) { lo = S.n; hi = -1; }
-    __syncthreads();
-    long slot = (long)blockIdx.x * TTPB + threadIdx.x;
-    if (slot < (long)D.ncell * D.cap)
+    __shared__ double sh[TTPB];
+    const int a = blockIdx.x;
+    const int iz = a % S.nzg, col = a / S.nzg;
+    int x0 = 0, x1 = D.nc0 - 1, y0 = 0, y1 = D.nc1 - 1, z0, z1;
+    if (G3)
     {
-        int c = (int)(slot / D.cap);
-        int k = (int)(slot - (long)c * D.cap);
-        if (k < D.cnt[c])
-        {
-            int iz = ttm_bin(S, D.z[slot]);
-            if (iz >= 0 && iz < S.n)
-            {
-                double m = D.mass[D.kind[slot]];
-                double vx = D.vx[slot] - S.vcm[3 * iz];
-                double vy = D.vy[slot] - S.vcm[3 * iz + 1];
-                double vz = D.vz[slot] - S.vcm[3 * iz + 2];
-                atomicAdd(&ht[iz], m * (vx * vx + vy * vy + vz * vz));
-                atomicAdd(&hn[iz], 1u);
-                atomicMin(&lo, iz);
-                atomicMax(&hi, iz);
-            }
-        }
+        md_range(col / S.ncy, S.ncx, S.tcellh_x, D.h0, D.nc0, x0, x1);
+        md_range(col % S.ncy, S.ncy, S.tcellh_y, D.h1, D.nc1, y0, y1);
     }
-    __syncthreads();
-    for (int i = lo + threadIdx.x; i <= hi; i += TTPB)
-        if (hn[i] != 0)
+    md_range(iz - 1, S.ncz, S.tcellh_z, D.h2, D.nc2, z0, z1);
+    if (iz == 0) { z0 = 0; z1 = D.nc2 > 1 ? 1 : 0; }
+    if (iz == S.nzg - 1) { z1 = D.nc2 - 1; }
+    const int nx = x1 - x0 + 1, ny = y1 - y0 + 1, nz = z1 - z0 + 1;
+    const int items = nx * ny * nz * LPC;
+    /* item k: MD cell k / LPC of the box, atoms k % LPC, + LPC, ... of it */
+    auto visit = [&](auto f) __attribute__((always_inline)) {
+        for (int k = threadIdx.x; k < items; k += TTPB)
         {
-            atomicAdd(&S.Ti[i], ht[i]);
-            atomicAdd(&S.num[i], hn[i]);
+            const int b = k / LPC, cz = z0 + b % nz, t = b / nz;
+            const int c = ((x0 + t / ny) * D.nc1 + (y0 + t % ny)) * D.nc2 + cz;
+            const long base = (long)c * D.cap;
+            for (int j = k % LPC, n = D.cnt[c]; j < n; j += LPC)
+                if (D.mv[base + j] == a) f(base + j);
         }
+    };
+    double m = 0.0, px = 0.0, py = 0.0, pz = 0.0;
+    visit([&](long s) {
+        const double mm = D.mass[D.kind[s]];
+        m += mm;
+        px += mm * D.vx[s];
+        py += mm * D.vy[s];
+        pz += mm * D.vz[s];
+    });
+    m = block_sum(m, sh);
+    px = block_sum(px, sh);
+    py = block_sum(py, sh);
+    pz = block_sum(pz, sh);
+    const bool owned = iz >= 1 && iz <= S.ncz;
+    const bool divide = G3 ? m != 0.0 : owned;
+    const double vx0 = divide ? px / m : px, vy0 = divide ? py / m : py, vz0 = divide ? pz / m : pz;
+    double e = 0.0, cnt = 0.0;
+    visit([&](long s) {
+        const double mm = D.mass[D.kind[s]];
+        const double vx = D.vx[s] - vx0, vy = D.vy[s] - vy0, vz = D.vz[s] - vz0;
+        e += mm * (vx * vx + vy * vy + vz * vz);
+        cnt += 1.0;
+    });
+    e = block_sum(e, sh);
+    cnt = block_sum(cnt, sh);
+    if (threadIdx.x == 0)
+    {
+        const unsigned nn = (unsigned)cnt;
+        S.mass[a] = m;
+        S.vcm[3 * a] = vx0;
+        S.vcm[3 * a + 1] = vy0;
+        S.vcm[3 * a + 2] = vz0;
+        S.num[a] = nn;
+        S.Ti[a] = (G3 ? nn != 0 : owned) ? e / (3.0 * KB_EV * nn) : e;
+    }
 }
 
-__global__ __launch_bounds__(TTPB) void k_ttm_temp(TtmState S)
-{
-#pragma clang fp contract(off)
-    int i = blockIdx.x * TTPB + threadIdx.x;
-    if (i < 1 || i > S.ncz) return;
-    S.Ti[i] /= (3.0 * KB_EV * S.num[i]);
-}
-
-/* FIND_klo_AND_ETC (cspline.h:29-44) + SPLINE_VAL (cspline.h:26) */
 static __device__ double table_spline(const double *X, const double *Y, const double *YDD, int n, double x)
 {
 #pragma clang fp contract(off)
@@ -274,95 +296,47 @@ __global__ __launch_bounds__(TTPB) void k_ttm_solve(TtmState S, double time, dou
 
 /* ------------------------------------------------ ncx x ncy x ncz turies */
 
-/* turi cell of a position, x and y clamped to the grid; a non-finite or
-   far-away z goes to the lower ghost cell (the comparisons run in FP64 so
-   that no out-of-range conversion happens) */
-static __device__ __forceinline__ int tcell3(const TtmState &S, double x, double y, double z)
-{
-    const double fx = x / S.tcellh_x, fy = y / S.tcellh_y, fz = z / S.tcellh_z;
-    const int ix = fx >= 0.0 ? (fx < S.ncx ? (int)fx : S.ncx - 1) : 0;
-    const int iy = fy >= 0.0 ? (fy < S.ncy ? (int)fy : S.ncy - 1) : 0;
-    const int iz = (fz >= -1.0 && fz < S.nzg - 1) ? (int)fz + 1 : 0;
-    return (ix * S.ncy + iy) * S.nzg + iz;
-}
-
-__global__ __launch_bounds__(TPB) void k_ttm_sum_mv3(KDesc D, TtmState S)
-{
-    each_slot(D, blockIdx.x, [&](long slot, int c) {
-        (void)c;
-        const int a = tcell3(S, D.x[slot], D.y[slot], D.z[slot]);
-        if (a < 0) return;
-        const double m = D.mass[D.kind[slot]];
-        atomicAdd(&S.mass[a], m);
-        atomicAdd(&S.vcm[3 * a], m * D.vx[slot]);
-        atomicAdd(&S.vcm[3 * a + 1], m * D.vy[slot]);
-        atomicAdd(&S.vcm[3 * a + 2], m * D.vz[slot]);
-    });
-}
-
-__global__ __launch_bounds__(TTPB) void k_ttm_vcm3(TtmState S)
-{
-    const int a = blockIdx.x * TTPB + threadIdx.x;
-    if (a >= S.n || S.mass[a] == 0.0) return;
-    for (int d = 0; d < 3; d++) S.vcm[3 * a + d] /= S.mass[a];
-}
-
-__global__ __launch_bounds__(TPB) void k_ttm_sum_temp3(KDesc D, TtmState S)
+/* Activation / deactivation on the grid (the extension of
+   presolve_common_types_1_2, ttm.c:194-230): one lane per (x, y) column scans
+   z upwards as the 1-D scan does (the cell below already processed in this
+   pass, the cell above not yet); the x and y neighbours contribute their Te
+   from before the pass, kept in LDS, so the columns are independent.  A newly
+   active cell takes the atom-weighted mean Te of its active neighbours in the
+   order z-, z+, x-, x+, y-, y+, or its Ti when it has none. */
+__global__ __launch_bounds__(T3PB_P) void k_ttm_presolve3(TtmState S)
 {
 #pragma clang fp contract(off)
-    each_slot(D, blockIdx.x, [&](long slot, int c) {
-        (void)c;
-        const int a = tcell3(S, D.x[slot], D.y[slot], D.z[slot]);
-        if (a < 0) return;
-        const double m = D.mass[D.kind[slot]];
-        const double vx = D.vx[slot] - S.vcm[3 * a];
-        const double vy = D.vy[slot] - S.vcm[3 * a + 1];
-        const double vz = D.vz[slot] - S.vcm[3 * a + 2];
-        atomicAdd(&S.Ti[a], m * (vx * vx + vy * vy + vz * vz));
-        atomicAdd(&S.num[a], 1u);
-    });
-}
-
-__global__ __launch_bounds__(TTPB) void k_ttm_temp3(TtmState S)
-{
-#pragma clang fp contract(off)
-    const int a = blockIdx.x * TTPB + threadIdx.x;
-    if (a >= S.n || S.num[a] == 0) return;
-    S.Ti[a] /= (3.0 * KB_EV * S.num[a]);
-}
-
-/* the activation scan of presolve_common_types_1_2 over the grid, cells in
-   index order; a newly active cell takes the atom-weighted mean Te of its
-   active z, x and y neighbours (z first, as the 1-D scan) */
-__global__ __launch_bounds__(64) void k_ttm_presolve3(TtmState S)
-{
-#pragma clang fp contract(off)
-    if (threadIdx.x != 0) return;
-    for (int ix = 0; ix < S.ncx; ix++)
-        for (int iy = 0; iy < S.ncy; iy++)
-            for (int iz = 1; iz <= S.ncz; iz++)
+    extern __shared__ double before[];
+    for (int a = threadIdx.x; a < S.n; a += T3PB_P) before[a] = S.Te[a];
+    __syncthreads();
+    for (int col = threadIdx.x; col < S.ncx * S.ncy; col += T3PB_P)
+    {
+        const int ix = col / S.ncy, iy = col % S.ncy;
+        const int xl = ((ix + S.ncx - 1) % S.ncx) * S.ncy + iy, xh = ((ix + 1) % S.ncx) * S.ncy + iy;
+        const int yl = ix * S.ncy + (iy + S.ncy - 1) % S.ncy, yh = ix * S.ncy + (iy + 1) % S.ncy;
+        for (int iz = 1; iz <= S.ncz; iz++)
+        {
+            const int a = col * S.nzg + iz;
+            S.xi[a] = 0.0;
+            if (S.Te[a] < 0.0 && S.num[a] >= S.min_atoms)
             {
-                const int a = (ix * S.ncy + iy) * S.nzg + iz;
-                S.xi[a] = 0.0;
-                if (S.Te[a] < 0.0 && S.num[a] >= S.min_atoms)
+                const int nb[6] = {a - 1, a + 1, xl * S.nzg + iz, xh * S.nzg + iz, yl * S.nzg + iz, yh * S.nzg + iz};
+                unsigned count = 0;
+                double sum = 0.0;
+                for (int k = 0; k < 6; k++)
                 {
-                    const int nb[6] = {a - 1, a + 1,
-                                       (((ix + S.ncx - 1) % S.ncx) * S.ncy + iy) * S.nzg + iz,
-                                       (((ix + 1) % S.ncx) * S.ncy + iy) * S.nzg + iz,
-                                       (ix * S.ncy + (iy + S.ncy - 1) % S.ncy) * S.nzg + iz,
-                                       (ix * S.ncy + (iy + 1) % S.ncy) * S.nzg + iz};
-                    unsigned count = 0;
-                    double sum = 0.0;
-                    for (int k = 0; k < 6; k++)
-                        if (S.Te[nb[k]] >= 0.0)
-                        {
-                            sum += S.Te[nb[k]] * S.num[nb[k]];
-                            count += S.num[nb[k]];
-                        }
-                    S.Te[a] = (count > 0) ? sum / (int)count : S.Ti[a];
+                    const double te = k < 2 ? S.Te[nb[k]] : before[nb[k]];
+                    if (te >= 0.0)
+                    {
+                        sum += te * S.num[nb[k]];
+                        count += S.num[nb[k]];
+                    }
                 }
-                if (S.num[a] < S.min_atoms) S.Te[a] = S.Te2[a] = -1.0;
+                S.Te[a] = (count > 0) ? sum / (int)count : S.Ti[a];
             }
+            if (S.num[a] < S.min_atoms) S.Te[a] = S.Te2[a] = -1.0;
+        }
+    }
 }
 
 #define T3PB 1024
@@ -372,23 +346,37 @@ __global__ __launch_bounds__(64) void k_ttm_presolve3(TtmState S)
    K (Te_hi - 2 Te + Te_lo) / h^2 per dimension (the 1-D type-1 expression,
    ttm.c:281-287), kmode 1: kappa = K Te / Ti with face values
    (kappa_i + kappa_nb) / 2 (the type-2 flux form, ttm.c:347-352); an inactive
-   neighbour is replaced by the cell itself */
+   neighbour is replaced by the cell itself.  Each lane owns up to T3OWN cells
+   for the whole call: their new Te, the sum of their sub-step values (for xi)
+   and the depth factor of the laser source, exp((z - front) / -Lp), which
+   does not change between sub-steps, stay in registers.  Per sub-step the
+   cells' kappa are computed once into LDS (each is used by seven cells), the
+   new values are computed from LDS, and written back after a barrier. */
 __global__ __launch_bounds__(T3PB) void k_ttm_solve3(TtmState S, double time, double mddt)
 {
 #pragma clang fp contract(off)
     extern __shared__ double t3sh[];
-    double *Te = t3sh, *Te2 = t3sh + S.n;
-    for (int a = threadIdx.x; a < S.n; a += T3PB)
-    {
-        Te[a] = S.Te[a];
-        Te2[a] = S.Te2[a];
-    }
-    __syncthreads();
-    double xacc[T3OWN];
-    for (int k = 0; k < T3OWN; k++) xacc[k] = 0.0;
+    double *Te = t3sh, *kap = t3sh + S.n;
+    for (int a = threadIdx.x; a < S.n; a += T3PB) Te[a] = S.Te[a];
     const bool calc_source = fabs(time - S.laser_t0) < S.laser_tdiff;
     const double dt = mddt / S.ratio;
     const double h2[3] = {S.tcellh_x * S.tcellh_x, S.tcellh_y * S.tcellh_y, S.tcellh_z * S.tcellh_z};
+    /* other: the cell's value in the second buffer of the reference's swap; a
+       cell that is not updated in a sub-step takes it, as a swap would give */
+    double xacc[T3OWN], nt[T3OWN], other[T3OWN], depth[T3OWN];
+    for (int k = 0; k < T3OWN; k++)
+    {
+        xacc[k] = 0.0;
+        depth[k] = 0.0;
+        const int a = threadIdx.x + k * T3PB;
+        other[k] = a < S.n ? S.Te2[a] : 0.0;
+        if (calc_source && a < S.n)
+        {
+            const double z = (a % S.nzg - 1 + 0.5) * S.tcellh_z;
+            depth[k] = exp(S.laser_absdepth_inv * (z - S.frontsurf));
+        }
+    }
+    __syncthreads();
     for (int j = 0; j < S.ratio; j++)
     {
         double spc = 0.0;
@@ -397,19 +385,24 @@ __global__ __launch_bounds__(T3PB) void k_ttm_solve3(TtmState S, double time, do
             const double q = time + j * dt - S.laser_t0;
             spc = S.laser_factor * exp(S.laser_2sig2_inv * (q * q));
         }
+        if (S.kmode == 1)
+        {
+            for (int k = 0; k < T3OWN; k++)
+            {
+                const int a = threadIdx.x + k * T3PB;
+                if (a < S.n) kap[a] = S.K * Te[a] / S.Ti[a];
+            }
+            __syncthreads();
+        }
         for (int k = 0; k < T3OWN; k++)
         {
             const int a = threadIdx.x + k * T3PB;
-            if (a >= S.n) break;
+            nt[k] = 0.0;
+            if (a >= S.n) continue;
             const int iz = a % S.nzg, col = a / S.nzg;
+            nt[k] = other[k];
             if (iz < 1 || iz > S.ncz || Te[a] < 0) continue;
             const int ix = col / S.ncy, iy = col % S.ncy;
-            double source = 0.0;
-            if (calc_source)
-            {
-                const double z = (iz - 1 + 0.5) * S.tcellh_z;
-                source = spc * exp(S.laser_absdepth_inv * (z - S.frontsurf));
-            }
             int lo[3] = {(((ix + S.ncx - 1) % S.ncx) * S.ncy + iy) * S.nzg + iz,
                          (ix * S.ncy + (iy + S.ncy - 1) % S.ncy) * S.nzg + iz, a - 1};
             int hi[3] = {(((ix + 1) % S.ncx) * S.ncy + iy) * S.nzg + iz,
@@ -427,19 +420,27 @@ __global__ __launch_bounds__(T3PB) void k_ttm_solve3(TtmState S, double time, do
             }
             else
             {
-                const double ka = S.K * Te[a] / Ti;
+                const double ka = kap[a];
                 for (int d = 0; d < 3; d++)
                 {
-                    const double KeL = 0.5 * (ka + S.K * Te[lo[d]] / S.Ti[lo[d]]);
-                    const double KeR = 0.5 * (ka + S.K * Te[hi[d]] / S.Ti[hi[d]]);
+                    const double KeL = 0.5 * (ka + kap[lo[d]]);
+                    const double KeR = 0.5 * (ka + kap[hi[d]]);
                     flux += (KeR * (Te[hi[d]] - Te[a]) - KeL * (Te[a] - Te[lo[d]])) / h2[d];
                 }
             }
-            Te2[a] = Te[a] + dt / (S.C_gamma * Te[a]) * (flux - S.G * (Te[a] - Ti) + source);
-            xacc[k] += Te2[a];
+            const double source = calc_source ? spc * depth[k] : 0.0;
+            nt[k] = Te[a] + dt / (S.C_gamma * Te[a]) * (flux - S.G * (Te[a] - Ti) + source);
+            xacc[k] += nt[k];
         }
         __syncthreads();
-        double *t = Te2; Te2 = Te; Te = t;
+        for (int k = 0; k < T3OWN; k++)
+        {
+            const int a = threadIdx.x + k * T3PB;
+            if (a >= S.n) continue;
+            other[k] = Te[a];
+            Te[a] = nt[k];
+        }
+        __syncthreads();
     }
     for (int k = 0; k < T3OWN; k++)
     {
@@ -451,34 +452,29 @@ __global__ __launch_bounds__(T3PB) void k_ttm_solve3(TtmState S, double time, do
                           ? S.G * S.tvol * (xacc[k] / S.ratio - S.Ti[a]) / (3 * KB_EV * S.num[a] * S.Ti[a])
                           : 0.0;
         S.Te[a] = Te[a];
-        S.Te2[a] = Te2[a];
+        S.Te2[a] = other[k];
     }
 }
 
 hipError_t launch_ttm_fields(const KDesc &D, const TtmState &S, hipStream_t st)
 {
-    const unsigned gn = (S.n + TTPB - 1) / TTPB;
-    hipLaunchKernelGGL(k_ttm_clear, dim3(gn), dim3(TTPB), 0, st, S);
+    const dim3 gs((D.ncell + CPB - 1) / CPB);
     if (S.dim3)
     {
-        const unsigned gc = (unsigned)((D.ncell + CPB - 1) / CPB);
-        hipLaunchKernelGGL(k_ttm_sum_mv3, dim3(gc), dim3(TPB), 0, st, D, S);
-        hipLaunchKernelGGL(k_ttm_vcm3, dim3(gn), dim3(TTPB), 0, st, S);
-        hipLaunchKernelGGL(k_ttm_sum_temp3, dim3(gc), dim3(TPB), 0, st, D, S);
-        hipLaunchKernelGGL(k_ttm_temp3, dim3(gn), dim3(TTPB), 0, st, S);
-        return hipGetLastError();
+        hipLaunchKernelGGL(k_ttm_index<true>, gs, dim3(TPB), 0, st, D, S);
+        hipLaunchKernelGGL(k_ttm_sums<true>, dim3(S.n), dim3(TTPB), 0, st, D, S);
     }
-    const unsigned gs = (unsigned)(((long)D.ncell * D.cap + TTPB - 1) / TTPB);
-    hipLaunchKernelGGL(k_ttm_sum_mv, dim3(gs), dim3(TTPB), 0, st, D, S);
-    hipLaunchKernelGGL(k_ttm_vcm, dim3(gn), dim3(TTPB), 0, st, S);
-    hipLaunchKernelGGL(k_ttm_sum_temp, dim3(gs), dim3(TTPB), 0, st, D, S);
-    hipLaunchKernelGGL(k_ttm_temp, dim3(gn), dim3(TTPB), 0, st, S);
+    else
+    {
+        hipLaunchKernelGGL(k_ttm_index<false>, gs, dim3(TPB), 0, st, D, S);
+        hipLaunchKernelGGL(k_ttm_sums<false>, dim3(S.n), dim3(TTPB), 0, st, D, S);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_ttm_presolve(const TtmState &S, unsigned *err, hipStream_t st)
 {
-    if (S.dim3) hipLaunchKernelGGL(k_ttm_presolve3, dim3(1), dim3(64), 0, st, S);
+    if (S.dim3) hipLaunchKernelGGL(k_ttm_presolve3, dim3(1), dim3(T3PB_P), sizeof(double) * S.n, st, S);
     else hipLaunchKernelGGL(k_ttm_presolve, dim3(1), dim3(TTPB), 0, st, S, err);
     return hipGetLastError();
 }

@@ -1661,12 +1661,20 @@ static void ttm_fields3(orc_t *o)
         if (t->num[a]) t->Ti[a] /= (3.0 * ORC_KB * t->num[a]);
 }
 
-/* presolve_common_types_1_2 generalised: the same sequential scan (cells in
-   index order), a newly active cell taking the atom-weighted mean Te of its
-   active z, x and y neighbours (z first, as in the 1-D scan) */
+/* Activation / deactivation on the ncx x ncy x ncz turi (this build's
+   extension of presolve_common_types_1_2, ttm.c:194-230).  Each (x, y) column
+   is scanned upwards in z exactly as the reference's 1-D scan: the cell below
+   has already been processed in this pass, the cell above has not.  The x and
+   y neighbours contribute their Te from before the pass, so the columns are
+   independent and a 1 x 1 x n turi reduces to the reference scan (its x and y
+   neighbours are the cell itself, inactive while it is being activated).  A
+   newly active cell takes the atom-weighted mean Te of its active neighbours
+   in the order z-, z+, x-, x+, y-, y+, or its Ti when it has none. */
 static void ttm_presolve3(orc_t *o)
 {
     ottm *t = &o->ttm;
+    double *before = malloc(t->n * sizeof(double));
+    memcpy(before, t->Te, t->n * sizeof(double));
     for (int ix = 0; ix < t->ncx; ix++)
         for (int iy = 0; iy < t->ncy; iy++)
             for (int iz = 1; iz <= t->c.ncz; iz++)
@@ -1675,17 +1683,21 @@ static void ttm_presolve3(orc_t *o)
                 t->xi[a] = 0.0;
                 if (t->Te[a] < 0.0 && t->num[a] >= t->min_atoms)
                 {
-                    int nb[6] = {tidx(t, ix, iy, iz - 1), tidx(t, ix, iy, iz + 1),
-                                 tidx(t, (ix + t->ncx - 1) % t->ncx, iy, iz), tidx(t, (ix + 1) % t->ncx, iy, iz),
-                                 tidx(t, ix, (iy + t->ncy - 1) % t->ncy, iz), tidx(t, ix, (iy + 1) % t->ncy, iz)};
+                    const int nb[6] = {tidx(t, ix, iy, iz - 1), tidx(t, ix, iy, iz + 1),
+                                       tidx(t, (ix + t->ncx - 1) % t->ncx, iy, iz), tidx(t, (ix + 1) % t->ncx, iy, iz),
+                                       tidx(t, ix, (iy + t->ncy - 1) % t->ncy, iz), tidx(t, ix, (iy + 1) % t->ncy, iz)};
                     int count = 0;
                     double sum = 0.0;
                     for (int k = 0; k < 6; k++)
-                        if (t->Te[nb[k]] >= 0.0) { sum += t->Te[nb[k]] * t->num[nb[k]]; count += t->num[nb[k]]; }
+                    {
+                        const double te = k < 2 ? t->Te[nb[k]] : before[nb[k]];
+                        if (te >= 0.0) { sum += te * t->num[nb[k]]; count += t->num[nb[k]]; }
+                    }
                     t->Te[a] = (count > 0) ? sum / count : t->Ti[a];
                 }
                 if (t->num[a] < t->min_atoms) t->Te[a] = t->Te2[a] = -1.0;
             }
+    free(before);
 }
 
 /* explicit FD sub-steps on the ncx x ncy x ncz turi.  Ce = C_gamma * Te, G
