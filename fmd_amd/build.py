@@ -84,10 +84,29 @@ def build_program(src, out):
     return out
 
 
+REF_EXAMPLES = "/root/reference/examples"
+EXAMPLES_OUT = os.path.join(ORACLE, "_ref", "examples")
+
+
+def build_reference_examples():
+    """The reference's example programs compiled unchanged against include/fmd.h and
+    linked to libfmd.so (gcc X.c -lfmd, examples/01_argon.c:9), when the reference
+    tree is present; the binaries go to oracle/_ref/examples/ (not committed) and
+    travel with the tree to the GPU host, where tests/test_gpu_examples.py runs them."""
+    if not os.path.isdir(REF_EXAMPLES):
+        return []
+    out = []
+    for name in sorted(os.listdir(REF_EXAMPLES)):
+        if name.endswith(".c"):
+            out.append(build_program(os.path.join(REF_EXAMPLES, name), os.path.join(EXAMPLES_OUT, name[:-2] + ".x")))
+    return out
+
+
 def build_all():
     build_device()
     build_host()
     build_oracle()
+    build_reference_examples()
 
 
 if __name__ == "__main__":
