@@ -362,17 +362,38 @@ __global__ __launch_bounds__(T3PB) void k_ttm_solve3(TtmState S, double time, do
     const double dt = mddt / S.ratio;
     const double h2[3] = {S.tcellh_x * S.tcellh_x, S.tcellh_y * S.tcellh_y, S.tcellh_z * S.tcellh_z};
     /* other: the cell's value in the second buffer of the reference's swap; a
-       cell that is not updated in a sub-step takes it, as a swap would give */
-    double xacc[T3OWN], nt[T3OWN], other[T3OWN], depth[T3OWN];
+       cell that is not updated in a sub-step takes it, as a swap would give.
+       The owned cells' neighbours (x-, y-, z-, x+, y+, z+), Ti and whether they
+       are interior are fixed for the call. */
+    double xacc[T3OWN], nt[T3OWN], other[T3OWN], depth[T3OWN], tiv[T3OWN];
+    int nbi[T3OWN][6];
+    bool inner[T3OWN];
     for (int k = 0; k < T3OWN; k++)
     {
         xacc[k] = 0.0;
         depth[k] = 0.0;
+        tiv[k] = 0.0;
+        inner[k] = false;
         const int a = threadIdx.x + k * T3PB;
+        for (int d = 0; d < 6; d++) nbi[k][d] = a;
         other[k] = a < S.n ? S.Te2[a] : 0.0;
-        if (calc_source && a < S.n)
+        if (a >= S.n) continue;
+        const int iz = a % S.nzg, col = a / S.nzg;
+        const int ix = col / S.ncy, iy = col % S.ncy;
+        inner[k] = iz >= 1 && iz <= S.ncz;
+        tiv[k] = S.Ti[a];
+        if (inner[k])
         {
-            const double z = (a % S.nzg - 1 + 0.5) * S.tcellh_z;
+            nbi[k][0] = (((ix + S.ncx - 1) % S.ncx) * S.ncy + iy) * S.nzg + iz;
+            nbi[k][1] = (ix * S.ncy + (iy + S.ncy - 1) % S.ncy) * S.nzg + iz;
+            nbi[k][2] = a - 1;
+            nbi[k][3] = (((ix + 1) % S.ncx) * S.ncy + iy) * S.nzg + iz;
+            nbi[k][4] = (ix * S.ncy + (iy + 1) % S.ncy) * S.nzg + iz;
+            nbi[k][5] = a + 1;
+        }
+        if (calc_source)
+        {
+            const double z = (iz - 1 + 0.5) * S.tcellh_z;
             depth[k] = exp(S.laser_absdepth_inv * (z - S.frontsurf));
         }
     }
@@ -390,29 +411,23 @@ __global__ __launch_bounds__(T3PB) void k_ttm_solve3(TtmState S, double time, do
             for (int k = 0; k < T3OWN; k++)
             {
                 const int a = threadIdx.x + k * T3PB;
-                if (a < S.n) kap[a] = S.K * Te[a] / S.Ti[a];
+                if (a < S.n) kap[a] = S.K * Te[a] / tiv[k];
             }
             __syncthreads();
         }
         for (int k = 0; k < T3OWN; k++)
         {
             const int a = threadIdx.x + k * T3PB;
-            nt[k] = 0.0;
-            if (a >= S.n) continue;
-            const int iz = a % S.nzg, col = a / S.nzg;
             nt[k] = other[k];
-            if (iz < 1 || iz > S.ncz || Te[a] < 0) continue;
-            const int ix = col / S.ncy, iy = col % S.ncy;
-            int lo[3] = {(((ix + S.ncx - 1) % S.ncx) * S.ncy + iy) * S.nzg + iz,
-                         (ix * S.ncy + (iy + S.ncy - 1) % S.ncy) * S.nzg + iz, a - 1};
-            int hi[3] = {(((ix + 1) % S.ncx) * S.ncy + iy) * S.nzg + iz,
-                         (ix * S.ncy + (iy + 1) % S.ncy) * S.nzg + iz, a + 1};
+            if (a >= S.n || !inner[k] || Te[a] < 0) continue;
+            int lo[3] = {nbi[k][0], nbi[k][1], nbi[k][2]};
+            int hi[3] = {nbi[k][3], nbi[k][4], nbi[k][5]};
             for (int d = 0; d < 3; d++)
             {
                 if (Te[lo[d]] < 0) lo[d] = a;
                 if (Te[hi[d]] < 0) hi[d] = a;
             }
-            const double Ti = S.Ti[a];
+            const double Ti = tiv[k];
             double flux = 0.0;
             if (S.kmode == 0)
             {
