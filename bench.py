@@ -229,7 +229,7 @@ def served_collisions(njobs, rate, seed=4):
                 p99_ratio_plain_over_served=p["p99_s"] / a["p99_s"])
 
 
-def ttm_slab(steps, precision, setfl, seed=5, pad=320.0):
+def ttm_slab(steps, precision, setfl, seed=5, pad=320.0, pulse_to_ps=3.0):
     """The TTM-MD workload: EAM Cu slab of 40 x 40 x 400 FCC cells (2,560,000 atoms), periodic in
     x and y, free surfaces in z with `pad` A of vacuum on each side; a 4 x 4 x 200 TTM turi with
     Ce = gamma Te (96.6 J/m^3/K^2), kappa = kappa0 Te / Tl (401 W/m/K), G = 1e17 W/m^3/K and the
@@ -281,7 +281,19 @@ def ttm_slab(steps, precision, setfl, seed=5, pad=320.0):
     T = f.fmd_matt_getTemperature()
     act = te >= 0
     tl_min = float(np.min(ti[act])) if np.any(act) else float("nan")
+    te_max, n_active = float(np.max(te)), int(np.sum(act))
     natoms = len(x)
+    # untimed: carry on through the pulse (peak at 2 ps) and sample the electrons and the lattice
+    pulse = dict(t_ps=[], max_Te=[], mean_Te_active=[], lattice_T=[])
+    while pulse_to_ps and f.fmd_dync_getTime() < pulse_to_ps - 0.5 * dt:
+        chunk = min(0.25, pulse_to_ps - f.fmd_dync_getTime())
+        f.fmd_dync_integrate(-1, chunk - 0.5 * dt, dt)
+        te = f.field_array(turi, f.fmd_field_find(turi, api.FIELD_TTM_TE))
+        a = te >= 0
+        pulse["t_ps"].append(round(f.fmd_dync_getTime(), 4))
+        pulse["max_Te"].append(float(np.max(te)))
+        pulse["mean_Te_active"].append(float(np.mean(te[a])) if np.any(a) else float("nan"))
+        pulse["lattice_T"].append(f.fmd_matt_getTemperature())
     f.free()
     return dict(workload="EAM Cu slab 40x40x400 FCC (%d atoms), PBC x,y, free z with %g A vacuum per side; "
                          "TTM 4x4x200, Ce=96.6 Te J/m^3/K, kappa=401 Te/Tl W/m/K, G=1e17 W/m^3/K, source "
@@ -290,8 +302,8 @@ def ttm_slab(steps, precision, setfl, seed=5, pad=320.0):
                 precision="mixed" if precision == F.FMDX_PREC_MIXED else "fp64", timed_steps=steps,
                 ms_per_step=wall / steps * 1e3, atom_steps_per_s=natoms * steps / wall,
                 cfl_number_at_300K=cfl, cfl_number_coldest_cell=cfl * 300.0 / tl_min,
-                lattice_T_start=T_start, lattice_T_after=T, max_Te_after=float(np.max(te)),
-                active_ttm_cells=int(np.sum(act)))
+                lattice_T_start=T_start, lattice_T_after=T, max_Te_after=te_max,
+                active_ttm_cells=n_active, through_the_pulse=pulse if pulse["t_ps"] else None)
 
 
 def nve_drift(cells, steps, precision, setfl, seed=1, dt=2e-3):
