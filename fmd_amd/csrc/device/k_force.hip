@@ -36,7 +36,7 @@
  *
  * Two precisions share this code: R = double (positions, arithmetic and sums
  * in FP64) and R = float (FP32 positions relative to the home cell origin,
- * FP32 pair arithmetic, FP64 sums of the FP32 pair terms, FP64 results).
+ * FP32 pair arithmetic, Kahan-compensated FP32 sums, FP64 results).
  */
 #include "fmdx_internal.h"
 
@@ -126,9 +126,7 @@ template <bool FP> struct Tile<float, FP>
     }
 };
 
-/* accumulators: FP64 in both builds; the mixed build's FP32 pair terms are
-   widened exactly and summed in FP64 (two instructions per term, fewer than a
-   compensated FP32 sum and more accurate) */
+/* accumulators: plain FP64, or Kahan-compensated FP32 */
 struct AccD
 {
     double v;
@@ -138,10 +136,16 @@ struct AccD
 };
 struct AccF
 {
-    double v;
-    __device__ __forceinline__ void zero() { v = 0.0; }
-    __device__ __forceinline__ void add(float x) { v += (double)x; }
-    __device__ __forceinline__ double get() const { return v; }
+    float s, c;
+    __device__ __forceinline__ void zero() { s = 0.0f; c = 0.0f; }
+    __device__ __forceinline__ void add(float v)
+    {
+        float y = v - c;
+        float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ double get() const { return (double)s - (double)c; }
 };
 template <typename R> struct AccOf;
 template <> struct AccOf<double> { typedef AccD T; };

@@ -64,7 +64,7 @@ enum
 enum
 {
     FMDX_PREC_FP64 = 0,    /* FP64 everywhere */
-    FMDX_PREC_MIXED = 1    /* FP32 pair arithmetic, FP64 accumulation and integration */
+    FMDX_PREC_MIXED = 1    /* FP32 pair arithmetic + Kahan accumulation, FP64 integration */
 };
 
 /* pair categories: same numbering as the reference's potcat_t, src/potential.h:26-32 */
