@@ -252,6 +252,8 @@ extern "C" int fmdx_set_eam(fmdx_system_t *s, int nelem, int nrho, double drho, 
     for (long k = 0; k < m2; k++) { hRP[2 * k] = hR[k]; hRP[2 * k + 1] = hP[k]; }
     std::vector<float4> fRP(hRP.size());
     cvt(hRP, fRP);
+    /* tables of an earlier call may still be read by work queued on the stream */
+    CK(hipStreamSynchronize(s->st));
     dfree(s->Fc); dfree(s->rhoc); dfree(s->phic); dfree(s->Fc32); dfree(s->rhoc32); dfree(s->phic32);
     dfree(s->rp); dfree(s->rp32);
     int r = 0;
@@ -259,20 +261,22 @@ extern "C" int fmdx_set_eam(fmdx_system_t *s, int nelem, int nrho, double drho, 
     r |= dalloc(&s->Fc, hF.size()); r |= dalloc(&s->rhoc, hR.size()); r |= dalloc(&s->phic, hP.size());
     r |= dalloc(&s->Fc32, fF.size()); r |= dalloc(&s->rhoc32, fR.size()); r |= dalloc(&s->phic32, fP.size());
     if (r) return FMDX_ENOMEM;
-    CK(hipMemcpy(s->Fc, hF.data(), hF.size() * sizeof(double4), hipMemcpyHostToDevice));
-    CK(hipMemcpy(s->rhoc, hR.data(), hR.size() * sizeof(double4), hipMemcpyHostToDevice));
-    CK(hipMemcpy(s->phic, hP.data(), hP.size() * sizeof(double4), hipMemcpyHostToDevice));
-    CK(hipMemcpy(s->Fc32, fF.data(), fF.size() * sizeof(float4), hipMemcpyHostToDevice));
-    CK(hipMemcpy(s->rhoc32, fR.data(), fR.size() * sizeof(float4), hipMemcpyHostToDevice));
-    CK(hipMemcpy(s->phic32, fP.data(), fP.size() * sizeof(float4), hipMemcpyHostToDevice));
-    CK(hipMemcpy(s->rp, hRP.data(), hRP.size() * sizeof(double4), hipMemcpyHostToDevice));
-    CK(hipMemcpy(s->rp32, fRP.data(), fRP.size() * sizeof(float4), hipMemcpyHostToDevice));
+    CK(hipMemcpyAsync(s->Fc, hF.data(), hF.size() * sizeof(double4), hipMemcpyHostToDevice, s->st));
+    CK(hipMemcpyAsync(s->rhoc, hR.data(), hR.size() * sizeof(double4), hipMemcpyHostToDevice, s->st));
+    CK(hipMemcpyAsync(s->phic, hP.data(), hP.size() * sizeof(double4), hipMemcpyHostToDevice, s->st));
+    CK(hipMemcpyAsync(s->Fc32, fF.data(), fF.size() * sizeof(float4), hipMemcpyHostToDevice, s->st));
+    CK(hipMemcpyAsync(s->rhoc32, fR.data(), fR.size() * sizeof(float4), hipMemcpyHostToDevice, s->st));
+    CK(hipMemcpyAsync(s->phic32, fP.data(), fP.size() * sizeof(float4), hipMemcpyHostToDevice, s->st));
+    CK(hipMemcpyAsync(s->rp, hRP.data(), hRP.size() * sizeof(double4), hipMemcpyHostToDevice, s->st));
+    CK(hipMemcpyAsync(s->rp32, fRP.data(), fRP.size() * sizeof(float4), hipMemcpyHostToDevice, s->st));
     EamDev &E = s->D.eam;
     E.ne = nelem; E.nrho = nrho; E.nr2 = nr2;
     E.drho = drho; E.inv_drho = 1.0 / drho; E.dr2 = dr2; E.inv_dr2 = 1.0 / dr2; E.rc2 = rc2;
     E.Fc = s->Fc; E.rhoc = s->rhoc; E.phic = s->phic;
     E.Fc32 = s->Fc32; E.rhoc32 = s->rhoc32; E.phic32 = s->phic32;
     E.rp = s->rp; E.rp32 = s->rp32;
+    /* the kernels run on the system stream: the tables have landed before any of them reads */
+    CK(hipStreamSynchronize(s->st));
     /* hit words kept between the two EAM passes (one bit per lane and trip) */
     if (!s->D.hbits && dalloc(&s->D.hbits, (size_t)s->D.ncell * HBITS_WORDS * 64)) return FMDX_ENOMEM;
     for (int k = 0; k < s->D.nkinds; k++) s->D.elem[k] = elem_of_kind ? elem_of_kind[k] : -1;
@@ -335,15 +339,15 @@ extern "C" int fmdx_upload(fmdx_system_t *s, unsigned natoms, const unsigned *ce
         if (group) g[i] = group[i];
         aid[i] = atomid ? atomid[i] : i;
     }
-    CK(hipMemcpy(dstart, start.data(), sizeof(int) * (D.ncell + 1), hipMemcpyHostToDevice));
-    CK(hipMemcpy(D.cnt, cell_count, sizeof(int) * D.ncell, hipMemcpyHostToDevice));
+    CK(hipMemcpyAsync(dstart, start.data(), sizeof(int) * (D.ncell + 1), hipMemcpyHostToDevice, s->st));
+    CK(hipMemcpyAsync(D.cnt, cell_count, sizeof(int) * D.ncell, hipMemcpyHostToDevice, s->st));
     if (natoms)
     {
-        CK(hipMemcpy(dx, x, sizeof(double) * 3 * natoms, hipMemcpyHostToDevice));
-        CK(hipMemcpy(dv, v ? v : vz.data(), sizeof(double) * 3 * natoms, hipMemcpyHostToDevice));
-        CK(hipMemcpy(dg, g.data(), sizeof(int) * natoms, hipMemcpyHostToDevice));
-        CK(hipMemcpy(da, aid.data(), sizeof(unsigned) * natoms, hipMemcpyHostToDevice));
-        CK(hipMemcpy(dk, k8.data(), natoms, hipMemcpyHostToDevice));
+        CK(hipMemcpyAsync(dx, x, sizeof(double) * 3 * natoms, hipMemcpyHostToDevice, s->st));
+        CK(hipMemcpyAsync(dv, v ? v : vz.data(), sizeof(double) * 3 * natoms, hipMemcpyHostToDevice, s->st));
+        CK(hipMemcpyAsync(dg, g.data(), sizeof(int) * natoms, hipMemcpyHostToDevice, s->st));
+        CK(hipMemcpyAsync(da, aid.data(), sizeof(unsigned) * natoms, hipMemcpyHostToDevice, s->st));
+        CK(hipMemcpyAsync(dk, k8.data(), natoms, hipMemcpyHostToDevice, s->st));
     }
     /* binning moves atoms into slots that held none; their force words must
        read zero until the next force evaluation, as for a fresh host cell */
