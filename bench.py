@@ -135,8 +135,35 @@ def gpu_bench(cells, steps, warmup, precision, setfl, seed=1):
     return res
 
 
-def cpu_baseline(setfl, cells=32, steps=20, seed=1):
-    """The oracle (FP64 restatement of the reference, one thread) on a bounded sample."""
+def host_threads():
+    """Host threads this process may use: its CPU affinity, capped by OMP_NUM_THREADS when set."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        n = min(n, int(os.environ.get("OMP_NUM_THREADS", n)))
+    except ValueError:
+        pass
+    return max(1, n)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fp:
+            for line in fp:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(setfl, cells=32, steps=20, seed=1, threads=1):
+    """The oracle (FP64 restatement of the reference) on a bounded sample, on `threads` host
+    threads at once: each thread builds and steps its own copy of the sample, and ctypes drops
+    the GIL inside every oracle call, so the copies run in parallel.  The value is the copies'
+    atom-steps over the wall time from a common start until the last copy ends: the host's
+    throughput for this algorithm with every thread busy, which bounds what a domain-decomposed
+    run of the reference could reach on the same threads."""
+    import threading
     from oracle.oracle import Oracle
     a = W.CU["a"]
     rng = np.random.default_rng(seed)
@@ -144,21 +171,42 @@ def cpu_baseline(setfl, cells=32, steps=20, seed=1):
     L = cells * a
     x = np.mod(x, L)
     v = W.maxwell_velocities(len(x), 63.546, 600.0, rng)
-    o = Oracle([L] * 3, [1, 1, 1], [63.546], ["Cu"])
-    o.load_eam(setfl)
-    o.apply_eam(0, 0)
-    o.make_grid()
-    o.add_atoms(x, v)
-    o.distribute()
-    o.forces()
-    o.thermo()
+    start = threading.Barrier(threads + 1)
+    ends, errs = [0.0] * threads, []
+
+    def worker(k):
+        try:
+            o = Oracle([L] * 3, [1, 1, 1], [63.546], ["Cu"])
+            o.load_eam(setfl)
+            o.apply_eam(0, 0)
+            o.make_grid()
+            o.add_atoms(x, v)
+            o.distribute()
+            o.forces()
+            o.thermo()
+        except Exception as e:   # the barrier still has to be met
+            errs.append(e)
+            o = None
+        start.wait()
+        if o is not None:
+            for _ in range(steps):
+                o.vv_start(2e-3, thermostat=1, tau=0.1, T0=600.0)
+                o.forces()
+                o.vv_finish(2e-3)
+            o.close()
+        ends[k] = time.perf_counter()
+
+    ts = [threading.Thread(target=worker, args=(k,)) for k in range(threads)]
+    for t in ts:
+        t.start()
+    start.wait()
     t0 = time.perf_counter()
-    for _ in range(steps):
-        o.vv_start(2e-3, thermostat=1, tau=0.1, T0=600.0)
-        o.forces()
-        o.vv_finish(2e-3)
-    dt = time.perf_counter() - t0
-    return dict(value=len(x) * steps / dt, natoms=len(x), steps=steps, seconds=dt)
+    for t in ts:
+        t.join()
+    if errs:
+        raise RuntimeError("cpu baseline: %r" % errs[0])
+    dt = max(ends) - t0
+    return dict(value=threads * len(x) * steps / dt, natoms=len(x), steps=steps, seconds=dt, threads=threads)
 
 
 def batched_ensemble(nsys, steps, plain_sample, seed=3):
@@ -446,8 +494,11 @@ def main():
             log("bench: served collisions %s" % srv)
         cpu = None
         if not args.no_cpu_baseline:
-            cpu = cpu_baseline(setfl)
+            cpu = cpu_baseline(setfl, threads=host_threads())
             log("bench: cpu baseline %s" % cpu)
+            cpu1 = cpu_baseline(setfl, steps=10)
+            log("bench: cpu baseline, one thread %s" % cpu1)
+            cpu["one_thread"] = cpu1["value"]
     n, K = r["natoms"], r["steps"]
     ms_step = r["gpu_ms"] / K
     value = n * K / (r["gpu_ms"] * 1e-3)
@@ -492,10 +543,14 @@ def main():
     if srv is not None:
         out["served_collisions"] = srv
     if cpu is not None:
-        out["cpu_baseline"] = {"value": cpu["value"], "unit": "atom-steps/s", "cores": 1, "kind": "port",
-                               "sample": "oracle (FP64 reference restatement, one thread) on EAM Cu %d atoms "
-                                         "(32^3 FCC, same setfl and lattice recipe), %d Berendsen VV steps, %.1f s"
-                                         % (cpu["natoms"], cpu["steps"], cpu["seconds"])}
+        out["cpu_baseline"] = {"value": cpu["value"], "unit": "atom-steps/s", "cores": cpu["threads"], "kind": "port",
+                               "sample": "oracle (FP64 reference restatement) on EAM Cu: %d concurrent copies, one "
+                                         "per host thread (%s), of a %d-atom sample (32^3 FCC, same setfl and "
+                                         "lattice recipe), %d Berendsen VV steps each, %.1f s; one copy alone: "
+                                         "%.3g atom-steps/s"
+                                         % (cpu["threads"], cpu_model(), cpu["natoms"], cpu["steps"], cpu["seconds"],
+                                            cpu["one_thread"]),
+                               "one_thread": cpu["one_thread"]}
     print(json.dumps(out), flush=True)
 
 
