@@ -96,3 +96,41 @@ def rel_err(a, b, scale=None):
     b = np.asarray(b, dtype=np.float64)
     s = scale if scale is not None else max(np.max(np.abs(b)), 1e-300)
     return float(np.max(np.abs(a - b)) / s)
+
+
+# second element of the two-element setfl: the same analytic forms as the
+# synthetic Cu table, other amplitudes and decay lengths
+XX = dict(re=2.62, fe=1.21, beta=4.6, A=0.251, alpha=7.1, B=0.402, kappa=4.6, F0=1.83, a=3.70, cutoff=5.5)
+
+
+def write_setfl_two(path, nr=3000, nrho=3000):
+    """A two-element eam/alloy setfl ("Cu" with the synthetic Cu forms, "Xx" with XX):
+    per element F(rho) then rho(r), then r*phi for the pairs (0,0), (1,0), (1,1) with
+    phi_XxCu = (phi_CuCu + phi_XxXx) / 2 -- the layout eam.c:422-474 reads."""
+    from fmd_amd import workloads as W
+    ps = [W.CU, XX]
+    rc = W.CU["cutoff"]
+    dr = rc / (nr - 1)
+    r = np.arange(nr) * dr
+    rho_e = [W.fcc_shell_density(p) for p in ps]
+    drho = 2.0 * max(rho_e) / (nrho - 1)
+    rho = np.arange(nrho) * drho
+    f = [W.eam_analytic(r, p)[0] for p in ps]
+    phi = [W.eam_analytic(r, p)[1] for p in ps]
+    rphi = {(0, 0): r * phi[0], (1, 0): r * 0.5 * (phi[0] + phi[1]), (1, 1): r * phi[1]}
+
+    def block(fp, arr):
+        for i in range(0, len(arr), 5):
+            fp.write(" ".join("%.16e" % v for v in arr[i:i + 5]) + "\n")
+
+    with open(path, "w") as fp:
+        fp.write("two-element synthetic eam/alloy (test fixture)\n\n\n")
+        fp.write("2 Cu Xx\n")
+        fp.write("%d %.16e %d %.16e %.16e\n" % (nrho, drho, nr, dr, rc))
+        for e, (name, mass) in enumerate((("29", 63.546), ("99", 58.69))):
+            fp.write("%s %.3f %.4f fcc\n" % (name, mass, ps[e]["a"]))
+            block(fp, -ps[e]["F0"] * np.sqrt(rho / rho_e[e]))
+            block(fp, f[e])
+        for key in ((0, 0), (1, 0), (1, 1)):
+            block(fp, rphi[key])
+    return path

@@ -83,6 +83,44 @@ def test_equilibrate_eam_through_product_loader(rng, cu01):
     f.free()
 
 
+def test_integrate_eam_two_elements_through_product_loader(rng, tmp_path):
+    from tests.pairsys import write_setfl_two
+    setfl = write_setfl_two(str(tmp_path / "two.eam.alloy"))
+    a = 3.65
+    x = lattice_fcc((6, 6, 6), a, rng, jitter=0.03)
+    kind = (rng.random(len(x)) < 0.4).astype(np.int32)
+    masses = [63.546, 58.69]
+    v = np.where(kind[:, None] == 0, maxwell_velocities(len(x), masses[0], 400.0, rng),
+                 maxwell_velocities(len(x), masses[1], 400.0, rng))
+    L = [6 * a] * 3
+    f = api.Fmd()
+    f.fmd_box_setSize(*L)
+    f.fmd_box_setPBC(True, True, True)
+    f.set_kinds(["Cu", "Xx"], masses)
+    pot = f.fmd_pot_eam_alloy_load(setfl.encode())
+    assert pot
+    for k1, k2 in ((0, 0), (0, 1), (1, 1)):
+        f.fmd_pot_apply(k1, k2, pot)
+    f.load_atoms(x, v, kind=kind)
+    f.fmd_dync_integrate(api.GROUP_ALL, 0.02, 1e-3)
+    o = Oracle(L, [1, 1, 1], masses, ["Cu", "Xx"])
+    o.load_eam(setfl)
+    for k1, k2 in ((0, 0), (0, 1), (1, 1)):
+        o.apply_eam(k1, k2)
+    o.make_grid(); o.add_atoms(x, v, kind=kind); o.distribute()
+    steps, _ = o.integrate_vv(0.0, 0.02, 1e-3)
+    assert steps == 20
+    ga, ra = f.atoms(), o.atoms()
+    assert np.array_equal(ga["atomid"], ra["atomid"])
+    assert np.array_equal(ga["kind"], ra["kind"])
+    assert rel_err(ga["x"], ra["x"], scale=L[0]) < 1e-12
+    assert rel_err(ga["f"], ra["f"]) < 1e-9
+    e = o.energies()
+    assert f.fmd_ext_getPotentialEnergy() == pytest.approx(e["pe"], rel=1e-10)
+    assert f.fmd_matt_getTemperature() == pytest.approx(e["T"], rel=1e-10)
+    f.free()
+
+
 def test_timer_events_and_state_roundtrip(rng, tmp_path):
     x, v, L = _lj_pair(rng, n=(6, 6, 6))
     f = api.Fmd()

@@ -76,6 +76,24 @@ def test_hybrid_eam_lj_two_kinds_with_group(rng, cu01):
     _compare(o, d, 1e-10, 1e-10, ag=1)
 
 
+@pytest.mark.parametrize("precision", [F.FMDX_PREC_FP64, F.FMDX_PREC_MIXED])
+def test_eam_two_elements(rng, tmp_path, precision):
+    """eam/alloy with two elements (general-category kernel: per-element density functions,
+    shared unlike-pair phi) on a random 40/60 mixture."""
+    from tests.pairsys import write_setfl_two
+    setfl = write_setfl_two(str(tmp_path / "two.eam.alloy"))
+    x = lattice_fcc((7, 7, 7), 3.65, rng, jitter=0.04)
+    kind = (rng.random(len(x)) < 0.4).astype(np.int32)
+    L = [7 * 3.65] * 3
+    o, d = build(L, [1, 1, 1], [63.546, 58.69], ["Cu", "Xx"], [("eam", setfl, [(0, 0), (0, 1), (1, 1)])], x,
+                 kind=kind, precision=precision)
+    if precision == F.FMDX_PREC_FP64:
+        got, ref = _compare(o, d, 1e-10, 1e-10)
+        assert rel_err(got["aux"], ref["aux"]) < 1e-10
+    else:
+        _compare(o, d, 1e-5, 1e-6)
+
+
 def test_mixed_precision_eam(rng, cu01):
     x = lattice_fcc((8, 8, 8), 3.615, rng, jitter=0.04)
     L = [8 * 3.615] * 3

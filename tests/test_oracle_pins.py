@@ -186,6 +186,64 @@ def test_eam_brute_force(rng, which, synth_setfl, cu01):
     assert o.energies()["pe"] == pytest.approx(e, rel=1e-11)
 
 
+def test_eam_two_elements_brute_force(rng, tmp_path):
+    """eam/alloy with two elements: rho_i sums the density function of each neighbour's element,
+    the pair force uses F'_i rho'_e(j) + F'_j rho'_e(i) + phi'_e(i)e(j) (eam.h:176-228, 270-310),
+    phi of the unlike pair is the one table both orders share (eam.c:455-474)."""
+    from tests.pairsys import write_setfl_two
+    setfl = write_setfl_two(str(tmp_path / "two.eam.alloy"))
+    x = lattice_fcc((5, 5, 5), 3.65, rng, jitter=0.05)
+    kind = (rng.random(len(x)) < 0.4).astype(np.int32)
+    L = [5 * 3.65] * 3
+    o, _ = build(L, [1, 1, 1], [63.546, 58.69], ["Cu", "Xx"], [("eam", setfl, [(0, 0), (0, 1), (1, 1)])], x,
+                 kind=kind, device=False)
+    o.forces()
+    a = o.atoms()
+    t = o.eam_tables()
+    assert t["nelem"] == 2
+    rr = np.arange(t["nr2"]) * t["dr2"]
+    nat = ((2, 0.0), (2, 0.0))
+    rho_s = [CubicSpline(rr, t["rho"][e], bc_type=nat) for e in range(2)]
+    phi_s = [[CubicSpline(rr, t["phi"][e, g], bc_type=nat) for g in range(2)] for e in range(2)]
+    F_s = [CubicSpline(np.arange(t["nrho"]) * t["drho"], t["F"][e], bc_type=nat) for e in range(2)]
+    # the tables themselves: one phi for both orders of the unlike pair, the mixing rule of the file
+    assert np.array_equal(t["phi"][0, 1], t["phi"][1, 0])
+    q = np.linspace(2.3 ** 2, 5.4 ** 2, 50)
+    assert rel_err(phi_s[0][1](q), 0.5 * (phi_s[0][0](q) + phi_s[1][1](q))) < 1e-9
+    rc2 = t["rc2"]
+    xs, ks = a["x"], a["kind"].astype(int)
+    n = len(xs)
+    rho = np.zeros(n)
+    pairs = []
+    for i in range(n):
+        d = minimg(xs[i] - xs, L, (1, 1, 1))
+        r2 = np.einsum("ij,ij->i", d, d)
+        m = np.nonzero((r2 > 0) & (r2 < rc2))[0]
+        rho[i] = sum(np.sum(rho_s[e](r2[m][ks[m] == e])) for e in range(2))
+        pairs.append((m, d[m], r2[m]))
+    Fp = np.array([F_s[ks[i]](rho[i], 1) for i in range(n)])
+    e = sum(F_s[ks[i]](rho[i]) for i in range(n))
+    f = np.zeros_like(xs)
+    for i in range(n):
+        m, d, r2 = pairs[i]
+        ei, ej = ks[i], ks[m]
+        rho_j = np.where(ej == 0, rho_s[0](r2, 1), rho_s[1](r2, 1))   # d rho_e(j) / d r2
+        rho_i = rho_s[ei](r2, 1)
+        phid = np.where(ej == 0, phi_s[ei][0](r2, 1), phi_s[ei][1](r2, 1))
+        mag = 2 * (Fp[i] * rho_j + Fp[m] * rho_i + phid)
+        f[i] = -np.sum(mag[:, None] * d, axis=0)
+        e += 0.5 * np.sum(np.where(ej == 0, phi_s[ei][0](r2), phi_s[ei][1](r2)))
+    assert rel_err(a["aux"], Fp) < 1e-9
+    assert rel_err(a["f"], f) < 1e-9
+    assert o.energies()["pe"] == pytest.approx(e, rel=1e-11)
+    # a brute force that swapped the density functions of the pair (rho_e(i) into rho_i) must fail
+    rho_bad = np.zeros(n)
+    for i in range(n):
+        m, d, r2 = pairs[i]
+        rho_bad[i] = np.sum(rho_s[ks[i]](r2))
+    assert rel_err(np.array([F_s[ks[i]](rho_bad[i], 1) for i in range(n)]), Fp) > 1e-3
+
+
 def test_perfect_lattice_symmetry(synth_setfl):
     x = lattice_fcc((6, 6, 6), 3.615)
     L = [6 * 3.615] * 3
