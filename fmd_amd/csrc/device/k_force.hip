@@ -52,10 +52,24 @@ enum
     CAT_MORSE = 3    /* one kind, Morse */
 };
 
+/* Candidates per LDS tile.  The tile sets a workgroup's LDS and with it the
+   workgroups per CU; a cell's candidates beyond one tile take further rounds.
+   A round stops staging once fewer than 64 places are left, and about 49 of
+   every 64 staged candidates survive the culling, so 224 (FP64) still takes
+   the ~300 survivors of a copper cell in two rounds of ~196, like 256 did,
+   while the FP64 force pass needs 11.0 KB instead of 12.1 KB of LDS: 14
+   workgroups per CU instead of 13 (240 already falls back to 13).  Measured
+   on the 4M-atom EAM step: 224/384 take 4.38/3.33 ms, 256/512 4.62/3.65 ms. */
+#ifndef FMDX_TILE64
+#define FMDX_TILE64 224
+#endif
+#ifndef FMDX_TILE32
+#define FMDX_TILE32 384
+#endif
 template <typename R> struct Prec;
 template <> struct Prec<double>
 {
-    enum { CT = 256 };   /* candidates per LDS tile */
+    enum { CT = FMDX_TILE64 };
     typedef double4 Q;
     static __device__ __forceinline__ const double4 *rho(const EamDev &E) { return E.rhoc; }
     static __device__ __forceinline__ const double4 *phi(const EamDev &E) { return E.phic; }
@@ -66,7 +80,7 @@ template <> struct Prec<double>
 };
 template <> struct Prec<float>
 {
-    enum { CT = 512 };
+    enum { CT = FMDX_TILE32 };
     typedef float4 Q;
     static __device__ __forceinline__ const float4 *rho(const EamDev &E) { return E.rhoc32; }
     static __device__ __forceinline__ const float4 *phi(const EamDev &E) { return E.phic32; }
