@@ -56,10 +56,11 @@ enum
    workgroups per CU; a cell's candidates beyond one tile take further rounds.
    A round stops staging once fewer than 64 places are left, and about 49 of
    every 64 staged candidates survive the culling, so 224 (FP64) still takes
-   the ~300 survivors of a copper cell in two rounds of ~196, like 256 did,
-   while the FP64 force pass needs 11.0 KB instead of 12.1 KB of LDS: 14
-   workgroups per CU instead of 13 (240 already falls back to 13).  Measured
-   on the 4M-atom EAM step: 224/384 take 4.38/3.33 ms, 256/512 4.62/3.65 ms. */
+   the ~300 survivors of a copper cell in two rounds of ~196, as 256 does,
+   with less LDS: the FP64 force pass needs 9.7 KB, which lets 16 workgroups
+   share a CU, as many as its registers allow.  Measured on the 4M-atom EAM
+   step (FP64 / mixed): 224/384 4.27/3.33 ms, 240/416 4.34/3.34, 256/512
+   4.39/3.41. */
 #ifndef FMDX_TILE64
 #define FMDX_TILE64 224
 #endif
@@ -90,22 +91,23 @@ template <> struct Prec<float>
     static __device__ __forceinline__ const float *pz(const KDesc &D) { return D.zr; }
 };
 
-/* LDS candidate tile (positions in the home frame, F' of the candidate, kind);
-   N covers the tile plus the padding of the last scan trip */
-template <typename R, bool FP> struct Tile;
+/* LDS candidate tile (positions in the home frame, F' of the candidate, and
+   the kind where the generic kernel dispatches on it: KD); N covers the tile
+   plus the padding of the last scan trip */
+template <typename R, bool FP, bool KD> struct Tile;
 struct Rec3 { double x, y, z; };
 template <bool FP> struct RecOf { typedef double4 T; };
 template <> struct RecOf<false> { typedef Rec3 T; };
-template <bool FP> struct Tile<double, FP>
+template <bool FP, bool KD> struct Tile<double, FP, KD>
 {
     enum { N = Prec<double>::CT + WG };
     typename RecOf<FP>::T p[N];   /* x, y, z (, F') */
-    uint8_t k[N];
+    uint8_t k[KD ? N : 1];
     __device__ __forceinline__ void put(int m, double x, double y, double zz, double f, int kk)
     {
         p[m].x = x; p[m].y = y; p[m].z = zz;
         if constexpr (FP) p[m].w = f;
-        k[m] = (uint8_t)kk;
+        if constexpr (KD) k[m] = (uint8_t)kk;
     }
     __device__ __forceinline__ void pos(int m, double &x, double &y, double &zz) const
     {
@@ -118,15 +120,15 @@ template <bool FP> struct Tile<double, FP>
         else f = 0.0;
     }
 };
-template <bool FP> struct Tile<float, FP>
+template <bool FP, bool KD> struct Tile<float, FP, KD>
 {
     enum { N = Prec<float>::CT + WG };
     float4 p[N];
-    uint8_t k[N];
+    uint8_t k[KD ? N : 1];
     __device__ __forceinline__ void put(int m, float x, float y, float zz, float f, int kk)
     {
         p[m] = make_float4(x, y, zz, f);
-        k[m] = (uint8_t)kk;
+        if constexpr (KD) k[m] = (uint8_t)kk;
     }
     __device__ __forceinline__ void pos(int m, float &x, float &y, float &zz) const
     {
@@ -138,6 +140,38 @@ template <bool FP> struct Tile<float, FP>
         const float4 a = p[m];
         x = a.x; y = a.y; zz = a.z; f = a.w;
     }
+};
+
+/* Image shift of each of the 27 neighbour cells, added to the staged
+   positions.  FP64 keeps the wrap (-1, 0 or +1 per dimension) and multiplies
+   it by the box length when staging, which gives the same x_j - L, x_j or
+   x_j + L as a stored shift; FP32 keeps the origin of the neighbour's image
+   relative to the home cell origin, formed in FP64 and rounded once. */
+template <typename R> struct Shifts;
+template <> struct Shifts<double>
+{
+    signed char w[3][27];
+    __device__ __forceinline__ void set(const KDesc &D, int n, const int (&wr)[3], const int (&dj)[3], const int (&hc)[3])
+    {
+        (void)D; (void)dj; (void)hc;
+        w[0][n] = (signed char)wr[0]; w[1][n] = (signed char)wr[1]; w[2][n] = (signed char)wr[2];
+    }
+    __device__ __forceinline__ double sx(const KDesc &D, int n) const { return (double)w[0][n] * D.L0; }
+    __device__ __forceinline__ double sy(const KDesc &D, int n) const { return (double)w[1][n] * D.L1; }
+    __device__ __forceinline__ double sz(const KDesc &D, int n) const { return (double)w[2][n] * D.L2; }
+};
+template <> struct Shifts<float>
+{
+    float s[3][27];
+    __device__ __forceinline__ void set(const KDesc &D, int n, const int (&wr)[3], const int (&dj)[3], const int (&hc)[3])
+    {
+        s[0][n] = (float)((dj[0] * D.h0 + wr[0] * D.L0) - hc[0] * D.h0);
+        s[1][n] = (float)((dj[1] * D.h1 + wr[1] * D.L1) - hc[1] * D.h1);
+        s[2][n] = (float)((dj[2] * D.h2 + wr[2] * D.L2) - hc[2] * D.h2);
+    }
+    __device__ __forceinline__ float sx(const KDesc &D, int n) const { (void)D; return s[0][n]; }
+    __device__ __forceinline__ float sy(const KDesc &D, int n) const { (void)D; return s[1][n]; }
+    __device__ __forceinline__ float sz(const KDesc &D, int n) const { (void)D; return s[2][n]; }
 };
 
 /* accumulators: plain FP64, or Kahan-compensated FP32 */
@@ -343,13 +377,12 @@ static __device__ void b_cell(const KDesc &D, const int c)
     }
     __shared__ long nb_base[27];
     __shared__ int nb_pre[28];
-    __shared__ R nb_sx[27], nb_sy[27], nb_sz[27];
-    __shared__ Tile<R, WITH_FP> T;
-    __shared__ short selfpos[256];   /* tile position of home atom j, -1 when not in the tile */
+    __shared__ Shifts<R> SH;
+    __shared__ Tile<R, WITH_FP, CAT == CAT_GEN> T;
     extern __shared__ PairParam pp[];   /* nkinds^2 entries, sized at launch */
     /* the stripe sums reuse the tile once the scan is over */
     double(*red)[WG] = reinterpret_cast<double(*)[WG]>(&T);
-    static_assert(sizeof(Tile<R, WITH_FP>) >= 4 * WG * sizeof(double), "tile too small for the stripe sums");
+    static_assert(sizeof(T) >= 4 * WG * sizeof(double), "tile too small for the stripe sums");
 
     const int lane = threadIdx.x;
     const int nkk = D.nkinds * D.nkinds;
@@ -359,29 +392,19 @@ static __device__ void b_cell(const KDesc &D, const int c)
     if (lane < 27)
     {
         int jx = cx + lane / 9 - 1, jy = cy + (lane / 3) % 3 - 1, jz = cz + lane % 3 - 1;
-        double sx = 0, sy = 0, sz = 0;
+        int wx = 0, wy = 0, wz = 0;
         bool ok = true;
-        if (jx < 0) { if (D.pbc0) { jx += D.nc0; sx = -D.L0; } else ok = false; }
-        else if (jx >= D.nc0) { if (D.pbc0) { jx -= D.nc0; sx = D.L0; } else ok = false; }
-        if (jy < 0) { if (D.pbc1) { jy += D.nc1; sy = -D.L1; } else ok = false; }
-        else if (jy >= D.nc1) { if (D.pbc1) { jy -= D.nc1; sy = D.L1; } else ok = false; }
-        if (jz < 0) { if (D.pbc2) { jz += D.nc2; sz = -D.L2; } else ok = false; }
-        else if (jz >= D.nc2) { if (D.pbc2) { jz -= D.nc2; sz = D.L2; } else ok = false; }
+        if (jx < 0) { if (D.pbc0) { jx += D.nc0; wx = -1; } else ok = false; }
+        else if (jx >= D.nc0) { if (D.pbc0) { jx -= D.nc0; wx = 1; } else ok = false; }
+        if (jy < 0) { if (D.pbc1) { jy += D.nc1; wy = -1; } else ok = false; }
+        else if (jy >= D.nc1) { if (D.pbc1) { jy -= D.nc1; wy = 1; } else ok = false; }
+        if (jz < 0) { if (D.pbc2) { jz += D.nc2; wz = -1; } else ok = false; }
+        else if (jz >= D.nc2) { if (D.pbc2) { jz -= D.nc2; wz = 1; } else ok = false; }
         const long jc = ok ? ((long)jx * D.nc1 + jy) * D.nc2 + jz : 0;
         nb_base[lane] = jc * D.cap;
         my_cnt = ok ? D.cnt[jc] : 0;
-        if (sizeof(R) == sizeof(double))
-        {
-            /* image shift of the neighbour cell: staged x = x_j + shift */
-            nb_sx[lane] = (R)sx; nb_sy[lane] = (R)sy; nb_sz[lane] = (R)sz;
-        }
-        else
-        {
-            /* origin of the image of cell jc relative to the home cell origin (FP64, then rounded) */
-            nb_sx[lane] = (R)((jx * D.h0 + sx) - cx * D.h0);
-            nb_sy[lane] = (R)((jy * D.h1 + sy) - cy * D.h1);
-            nb_sz[lane] = (R)((jz * D.h2 + sz) - cz * D.h2);
-        }
+        const int wr[3] = {wx, wy, wz}, dj[3] = {jx, jy, jz}, hc[3] = {cx, cy, cz};
+        SH.set(D, lane, wr, dj, hc);
     }
     /* inclusive prefix of the neighbour counts over lanes 0..26 */
     int incl = my_cnt;
@@ -450,12 +473,13 @@ static __device__ void b_cell(const KDesc &D, const int c)
         while (gpos < ncand)
         {
             __syncthreads();   /* the previous tile is no longer read */
-            for (int j = lane; j < nh; j += WG) selfpos[j] = -1;
-            __syncthreads();
             /* fill the tile with the candidates that can reach the home cell: a
                candidate farther than the cutoff from the cell's box cannot pair
                with any home atom; survivors are compacted in list order */
             int nt = 0;
+            /* tile position of this lane's home atom in this round, -1 while it is not in the tile */
+            int selft = -1;
+            const int gself = nb_pre[13] + h0 + a;   /* its place in the 27-cell candidate list */
             while (gpos < ncand && nt <= (int)CT - WG)
             {
                 const int g = gpos + lane;
@@ -472,9 +496,9 @@ static __device__ void b_cell(const KDesc &D, const int c)
                         else hi = mid;
                     }
                     const long js = nb_base[lo] + (g - nb_pre[lo]);
-                    x = P::px(D)[js] + nb_sx[lo];
-                    y = P::py(D)[js] + nb_sy[lo];
-                    z = P::pz(D)[js] + nb_sz[lo];
+                    x = P::px(D)[js] + SH.sx(D, lo);
+                    y = P::py(D)[js] + SH.sy(D, lo);
+                    z = P::pz(D)[js] + SH.sz(D, lo);
                     const R ex = fmax(fmax(bx0 - x, x - bx1), (R)0);
                     const R ey = fmax(fmax(by0 - y, y - by1), (R)0);
                     const R ez = fmax(fmax(bz0 - z, z - bz1), (R)0);
@@ -487,21 +511,22 @@ static __device__ void b_cell(const KDesc &D, const int c)
                     }
                 }
                 const unsigned long long bal = __ballot(keep);
-                if (keep)
-                {
-                    const int off = nt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
-                                                                       __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
-                    T.put(off, x, y, z, fpv, kj);
-                    if (lo == 13) selfpos[g - nb_pre[13]] = (short)off;
-                }
+                const int off = nt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+                if (keep) T.put(off, x, y, z, fpv, kj);
+                /* the lane that staged this lane's home atom (list entry 13 is the home
+                   cell itself, not an image) hands over the tile position it wrote */
+                const int src = gself - gpos;
+                const int got = __shfl((keep && lo == 13) ? off : -1, src & (WG - 1));
+                if (src >= 0 && src < WG) selft = got;
                 nt += __popcll(bal);
                 gpos += WG;
             }
+            if (s >= S) selft = -1;
             const int trips = (nt + S - 1) / S;
             const int npad = trips * S;
             for (int m = nt + lane; m < npad; m += WG) T.put(m, qnan, qnan, qnan, (R)0, NOKIND);
             __syncthreads();
-            const int selft = (s < S) ? selfpos[h0 + a] : -1;
             for (int g0 = 0; g0 < trips; g0 += 64)
             {
                 const int ng = min(64, trips - g0);
