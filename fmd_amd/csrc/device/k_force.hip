@@ -140,6 +140,7 @@ template <bool FP, bool KD> struct Tile<float, FP, KD>
         const float4 a = p[m];
         x = a.x; y = a.y; zz = a.z; f = a.w;
     }
+    __device__ __forceinline__ const float4 *at(int m) const { return p + m; }
 };
 
 /* Image shift of each of the 27 neighbour cells, added to the staged
@@ -421,6 +422,19 @@ static __device__ void b_cell(const KDesc &D, const int c)
     const int nh = __builtin_amdgcn_readfirstlane(D.cnt[c]);
     const int ag = D.ag;
     const R rcut2 = (R)D.rcmax2;
+    /* FP32 tiles without F' keep |x_j|^2 in their fourth word, and the scan forms
+       d = |x_j|^2 + (|x_i|^2 - rc^2 (1 + 2.5e-4)) - 2 x_i.x_j with one add and three
+       FMAs and shifts the sign of d into the hit word (v_alignbit), where the
+       difference form takes six operations, a compare and a select.  The scan only
+       selects candidates: every selected pair is tested again with the differences
+       (flush_eam1, pair_hit), so the margin only has to exceed the cancellation
+       error; cell-relative coordinates stay below 4 sqrt(3) rc (cells are narrower
+       than 2 rc), which bounds that error by ~2e-5 rc^2.  No NaN may reach d, whose
+       sign bit would be arbitrary: a lane without a home atom scans with zero
+       multipliers and |x_i|^2 = +inf, and padding entries are (0, 0, 0, +inf), so
+       both give d = +inf. */
+    constexpr bool NSCAN = sizeof(R) == sizeof(float) && !WITH_FP;
+    const R rscan2 = rcut2 * (R)(1.0 + 2.5e-4);
     const R qnan = (R)__builtin_nan("");
     const long hbase = (long)c * D.cap;
     const bool one_block = nh > 0 && nh <= WG;
@@ -513,7 +527,7 @@ static __device__ void b_cell(const KDesc &D, const int c)
                 const unsigned long long bal = __ballot(keep);
                 const int off = nt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
                                                                    __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
-                if (keep) T.put(off, x, y, z, fpv, kj);
+                if (keep) T.put(off, x, y, z, NSCAN ? x * x + y * y + z * z : fpv, kj);
                 /* the lane that staged this lane's home atom (list entry 13 is the home
                    cell itself, not an image) hands over the tile position it wrote */
                 const int src = gself - gpos;
@@ -525,7 +539,11 @@ static __device__ void b_cell(const KDesc &D, const int c)
             if (s >= S) selft = -1;
             const int trips = (nt + S - 1) / S;
             const int npad = trips * S;
-            for (int m = nt + lane; m < npad; m += WG) T.put(m, qnan, qnan, qnan, (R)0, NOKIND);
+            for (int m = nt + lane; m < npad; m += WG)
+            {
+                if constexpr (NSCAN) T.put(m, (R)0, (R)0, (R)0, (R)__builtin_inf(), NOKIND);
+                else T.put(m, qnan, qnan, qnan, (R)0, NOKIND);
+            }
             __syncthreads();
             for (int g0 = 0; g0 < trips; g0 += 64)
             {
@@ -539,23 +557,48 @@ static __device__ void b_cell(const KDesc &D, const int c)
                     unsigned lo = 0, hi = 0;
                     const int n0 = min(ng, 32);
                     int m = s + (g0 + ng - 1) * S;
-#pragma unroll 4
-                    for (int t = ng - 1; t >= 32; t--, m -= S)
+                    if constexpr (NSCAN)
                     {
-                        R x, y, z;
-                        T.pos(m, x, y, z);
-                        const R dx = xi - x, dy = yi - y, dz = zi - z;
-                        const R r2 = dx * dx + dy * dy + dz * dz;
-                        hi = (hi << 1) | (r2 < rcut2 ? 1u : 0u);
+                        const bool live = xi == xi;   /* a home atom on this lane */
+                        const float mx = live ? -2.0f * xi : 0.0f, my = live ? -2.0f * yi : 0.0f;
+                        const float mz = live ? -2.0f * zi : 0.0f;
+                        const float nim = live ? (xi * xi + yi * yi + zi * zi) - rscan2 : __builtin_inff();
+                        const float4 *q = T.at(m);
+#pragma unroll 4
+                        for (int t = ng - 1; t >= 32; t--, q -= S)
+                        {
+                            const float4 v = *q;
+                            const float d = __builtin_fmaf(v.z, mz, __builtin_fmaf(v.y, my, __builtin_fmaf(v.x, mx, v.w + nim)));
+                            hi = __builtin_amdgcn_alignbit(hi, __builtin_bit_cast(unsigned, d), 31);
+                        }
+#pragma unroll 4
+                        for (int t = n0 - 1; t >= 0; t--, q -= S)
+                        {
+                            const float4 v = *q;
+                            const float d = __builtin_fmaf(v.z, mz, __builtin_fmaf(v.y, my, __builtin_fmaf(v.x, mx, v.w + nim)));
+                            lo = __builtin_amdgcn_alignbit(lo, __builtin_bit_cast(unsigned, d), 31);
+                        }
                     }
-#pragma unroll 4
-                    for (int t = n0 - 1; t >= 0; t--, m -= S)
+                    else
                     {
-                        R x, y, z;
-                        T.pos(m, x, y, z);
-                        const R dx = xi - x, dy = yi - y, dz = zi - z;
-                        const R r2 = dx * dx + dy * dy + dz * dz;
-                        lo = (lo << 1) | (r2 < rcut2 ? 1u : 0u);
+#pragma unroll 4
+                        for (int t = ng - 1; t >= 32; t--, m -= S)
+                        {
+                            R x, y, z;
+                            T.pos(m, x, y, z);
+                            const R dx = xi - x, dy = yi - y, dz = zi - z;
+                            const R r2 = dx * dx + dy * dy + dz * dz;
+                            hi = (hi << 1) | (r2 < rcut2 ? 1u : 0u);
+                        }
+#pragma unroll 4
+                        for (int t = n0 - 1; t >= 0; t--, m -= S)
+                        {
+                            R x, y, z;
+                            T.pos(m, x, y, z);
+                            const R dx = xi - x, dy = yi - y, dz = zi - z;
+                            const R r2 = dx * dx + dy * dy + dz * dz;
+                            lo = (lo << 1) | (r2 < rcut2 ? 1u : 0u);
+                        }
                     }
                     word = ((unsigned long long)hi << 32) | lo;
                     /* the home atom meets itself on one trip of one stripe: clear that bit */
