@@ -113,6 +113,7 @@ template <bool FP, bool KD> struct Tile<double, FP, KD>
     {
         x = p[m].x; y = p[m].y; zz = p[m].z;
     }
+    __device__ __forceinline__ const typename RecOf<FP>::T *at(int m) const { return p + m; }
     __device__ __forceinline__ void load(int m, double &x, double &y, double &zz, double &f) const
     {
         x = p[m].x; y = p[m].y; zz = p[m].z;
@@ -174,6 +175,13 @@ template <> struct Shifts<float>
     __device__ __forceinline__ float sy(const KDesc &D, int n) const { (void)D; return s[1][n]; }
     __device__ __forceinline__ float sz(const KDesc &D, int n) const { (void)D; return s[2][n]; }
 };
+
+/* the 32-bit word holding the sign bit of a value, in bit 31 */
+static __device__ __forceinline__ unsigned sign_word(float v) { return __builtin_bit_cast(unsigned, v); }
+static __device__ __forceinline__ unsigned sign_word(double v)
+{
+    return (unsigned)(__builtin_bit_cast(unsigned long long, v) >> 32);
+}
 
 /* accumulators: plain FP64, or Kahan-compensated FP32 */
 struct AccD
@@ -432,9 +440,13 @@ static __device__ void b_cell(const KDesc &D, const int c)
        than 2 rc), which bounds that error by ~2e-5 rc^2.  No NaN may reach d, whose
        sign bit would be arbitrary: a lane without a home atom scans with zero
        multipliers and |x_i|^2 = +inf, and padding entries are (0, 0, 0, +inf), so
-       both give d = +inf. */
+       both give d = +inf.  The other tiles keep the difference form, with the
+       margin folded into the first FMA, d = dz^2 + (dy^2 + (dx^2 - rc^2 (1 + m))),
+       and the same sign shift (m = 1e-12 in FP64); there a lane without a home atom
+       scans from x_i = +inf and padding sits at -inf, so that d = +inf again. */
     constexpr bool NSCAN = sizeof(R) == sizeof(float) && !WITH_FP;
-    const R rscan2 = rcut2 * (R)(1.0 + 2.5e-4);
+    const R rscan2 = rcut2 * (R)(sizeof(R) == sizeof(double) ? 1.0 + 1e-12 : 1.0 + 2.5e-4);
+    const R pinf = (R)__builtin_inf();
     const R qnan = (R)__builtin_nan("");
     const long hbase = (long)c * D.cap;
     const bool one_block = nh > 0 && nh <= WG;
@@ -541,8 +553,8 @@ static __device__ void b_cell(const KDesc &D, const int c)
             const int npad = trips * S;
             for (int m = nt + lane; m < npad; m += WG)
             {
-                if constexpr (NSCAN) T.put(m, (R)0, (R)0, (R)0, (R)__builtin_inf(), NOKIND);
-                else T.put(m, qnan, qnan, qnan, (R)0, NOKIND);
+                if constexpr (NSCAN) T.put(m, (R)0, (R)0, (R)0, pinf, NOKIND);
+                else T.put(m, -pinf, -pinf, -pinf, (R)0, NOKIND);
             }
             __syncthreads();
             for (int g0 = 0; g0 < trips; g0 += 64)
@@ -581,23 +593,22 @@ static __device__ void b_cell(const KDesc &D, const int c)
                     }
                     else
                     {
+                        const bool live = xi == xi;   /* a home atom on this lane */
+                        const R sx = live ? xi : pinf, sy = live ? yi : pinf, sz = live ? zi : pinf;
+                        const auto *q = T.at(m);
 #pragma unroll 4
-                        for (int t = ng - 1; t >= 32; t--, m -= S)
+                        for (int t = ng - 1; t >= 32; t--, q -= S)
                         {
-                            R x, y, z;
-                            T.pos(m, x, y, z);
-                            const R dx = xi - x, dy = yi - y, dz = zi - z;
-                            const R r2 = dx * dx + dy * dy + dz * dz;
-                            hi = (hi << 1) | (r2 < rcut2 ? 1u : 0u);
+                            const R dx = sx - q->x, dy = sy - q->y, dz = sz - q->z;
+                            const R d = fma(dz, dz, fma(dy, dy, fma(dx, dx, -rscan2)));
+                            hi = __builtin_amdgcn_alignbit(hi, sign_word(d), 31);
                         }
 #pragma unroll 4
-                        for (int t = n0 - 1; t >= 0; t--, m -= S)
+                        for (int t = n0 - 1; t >= 0; t--, q -= S)
                         {
-                            R x, y, z;
-                            T.pos(m, x, y, z);
-                            const R dx = xi - x, dy = yi - y, dz = zi - z;
-                            const R r2 = dx * dx + dy * dy + dz * dz;
-                            lo = (lo << 1) | (r2 < rcut2 ? 1u : 0u);
+                            const R dx = sx - q->x, dy = sy - q->y, dz = sz - q->z;
+                            const R d = fma(dz, dz, fma(dy, dy, fma(dx, dx, -rscan2)));
+                            lo = __builtin_amdgcn_alignbit(lo, sign_word(d), 31);
                         }
                     }
                     word = ((unsigned long long)hi << 32) | lo;
