@@ -94,6 +94,51 @@ def test_eam_two_elements(rng, tmp_path, precision):
         _compare(o, d, 1e-5, 1e-6)
 
 
+@pytest.mark.parametrize("pot", ["lj", "eam"])
+@pytest.mark.parametrize("precision", [F.FMDX_PREC_FP64, F.FMDX_PREC_MIXED])
+def test_pairs_at_the_cutoff(pot, precision, synth_setfl):
+    """Dimers just inside and just outside the cutoff.  The candidate scans select with a
+    margin above rc^2 (1e-12 relative in FP64, 2.5e-4 in FP32) and the pair evaluation
+    tests r^2 < rc^2 again; a dimer in that band outside rc must contribute nothing,
+    and LJ and the synthetic table are both nonzero at rc.
+
+    The mixed EAM bar is looser: the synthetic table ends at rc without a taper, so
+    its last r^2 interval (0.006 A^2 wide) falls from rho(rc) = 0.014 to the zero
+    knots the reference appends (eam.c:445), and there the FP32 rounding of the
+    positions (~2e-7 A) moves the interval fraction by ~2e-3; an isolated dimer
+    at rc gets all of its force from that interval."""
+    rel = 2e-13 if precision == F.FMDX_PREC_FP64 else 2e-5
+    if pot == "lj":
+        rc, pots, names, mass = AR["rc"], [("lj", 0, 0, AR["sig"], AR["eps"], AR["rc"])], ["Ar"], [AR["m"]]
+    else:
+        rc, pots, names, mass = 5.5, [("eam", synth_setfl, [(0, 0)])], ["Cu"], [63.546]
+    L = 6.0 * rc
+    x = []
+    # along x, y, z and a diagonal, inside and outside by rel
+    dirs = [np.array(v, dtype=np.float64) / np.linalg.norm(v) for v in ((1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 1))]
+    k = 0
+    for dvec in dirs:
+        for sgn in (-1.0, 1.0):
+            a = np.array([0.7 + 1.5 * rc * (k % 4), 0.9 + 3.0 * rc * (k // 4), 1.1 + 1.5 * rc * (k % 2)]) % L
+            x += [a, (a + dvec * rc * (1.0 + sgn * rel)) % L]
+            k += 1
+    x = np.array(x)
+    o, d = build([L] * 3, [1, 1, 1], mass, names, pots, x, precision=precision)
+    o.forces()
+    d.forces()
+    got, ref = d.download(), o.atoms()
+    assert np.array_equal(got["atomid"], ref["atomid"])
+    fmax = np.max(np.abs(ref["f"]))
+    assert fmax > 0
+    # the outside dimers carry exactly zero force in the oracle
+    ids = ref["atomid"]
+    outside = np.isin(ids, [2 * j + b for j in range(8) if j % 2 == 1 for b in (0, 1)])
+    assert np.all(ref["f"][outside] == 0.0)
+    assert np.all(got["f"][outside] == 0.0)
+    tol = 1e-10 if precision == F.FMDX_PREC_FP64 else (1e-5 if pot == "lj" else 5e-3)
+    assert rel_err(got["f"], ref["f"], scale=fmax) < tol
+
+
 def test_mixed_precision_eam(rng, cu01):
     x = lattice_fcc((8, 8, 8), 3.615, rng, jitter=0.04)
     L = [8 * 3.615] * 3
