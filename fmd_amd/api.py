@@ -16,6 +16,7 @@ EVENT_TIMER_TICK, EVENT_FIELD_UPDATE, EVENT_ERROR = 0, 1, 2
 TURI_CUSTOM, TURI_TTM_TYPE1, TURI_TTM_TYPE2 = 0, 1, 2
 FIELD_MASS, FIELD_VCM, FIELD_TEMPERATURE, FIELD_NUMBER, FIELD_NUMBER_DENSITY, FIELD_TTM_TE, FIELD_TTM_XI = range(7)
 GROUP_ALL = -1
+SCM_XYZ_ATOMSNUM, SCM_XYZ_SEPARATE, SCM_CSV, SCM_VTF = range(4)   # fmd_SaveConfigMode_t, fmd.h:31-37
 
 EventHandler = C.CFUNCTYPE(None, P, C.c_int, P, P)
 

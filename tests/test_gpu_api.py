@@ -121,6 +121,31 @@ def test_integrate_eam_two_elements_through_product_loader(rng, tmp_path):
     f.free()
 
 
+def test_save_configuration_after_dynamics(rng, tmp_path):
+    """fmd_matt_saveConfiguration from the device after 20 steps, in all four modes, against
+    the oracle's atoms after the same steps (formats as in test_config_output.py)."""
+    from tests import test_config_output as T
+    x, L, kind, group = T.system(rng)
+    v = maxwell_velocities(len(x), 60.0, 80.0, rng)
+    f = T.api_system(x, L, kind, group, tmp_path, v)
+    f.fmd_dync_integrate(api.GROUP_ALL, 0.02, 1e-3)
+    o = Oracle(L, [1, 1, 1], T.MASSES, T.NAMES)
+    for (a, b), (sg, ep) in T.LJ.items():
+        o.set_lj(a, b, sg, ep, T.RC)
+    o.make_grid(); o.add_atoms(x, v, group, kind); o.distribute()
+    steps, _ = o.integrate_vv(0.0, 0.02, 1e-3)
+    assert steps == 20
+    ref = o.atoms()
+    for mode in (T.api.SCM_XYZ_SEPARATE, T.api.SCM_CSV, T.api.SCM_VTF, T.api.SCM_XYZ_ATOMSNUM):
+        f.fmd_io_setSaveConfigMode(mode)
+        f.fmd_matt_saveConfiguration()
+    f.free()
+    assert (tmp_path / "00000.xyz").read_text() == T.expected(ref, api.SCM_XYZ_SEPARATE)
+    assert (tmp_path / "00001.csv").read_text() == T.expected(ref, api.SCM_CSV)
+    assert (tmp_path / "00002.vtf").read_text() == T.expected(ref, api.SCM_VTF)
+    assert (tmp_path / ("%d.xyz" % len(x))).read_text() == T.expected(ref, api.SCM_XYZ_ATOMSNUM)
+
+
 def test_timer_events_and_state_roundtrip(rng, tmp_path):
     x, v, L = _lj_pair(rng, n=(6, 6, 6))
     f = api.Fmd()
