@@ -51,6 +51,12 @@ _sigs = {
     "fmd_pot_eam_alloy_load": ([P, C.c_char_p], P), "fmd_pot_apply": ([P, C.c_uint, C.c_uint, P], None),
     "fmd_pot_eam_getCutoffRadius": ([P, P], D), "fmd_pot_eam_getLatticeParameter": ([P, P, C.c_char_p], D),
     "fmd_matt_makeCuboidFCC": ([P, D, D, D, C.c_int, C.c_int, C.c_int, D, C.c_uint, C.c_int, D], None),
+    "fmd_matt_makeCuboidBCC": ([P, D, D, D, C.c_int, C.c_int, C.c_int, D, C.c_uint, C.c_int, D], None),
+    "fmd_matt_makeCuboidSC": ([P, D, D, D, C.c_int, C.c_int, C.c_int, D, C.c_uint, C.c_int, D], None),
+    "fmd_matt_makeCuboidFCC_mix": ([P, D, D, D, C.c_int, C.c_int, C.c_int, D, P, C.c_int, D], None),
+    "fmd_matt_makeCuboidBCC_mix": ([P, D, D, D, C.c_int, C.c_int, C.c_int, D, P, C.c_int, D], None),
+    "fmd_matt_makeCuboidSC_mix": ([P, D, D, D, C.c_int, C.c_int, C.c_int, D, P, C.c_int, D], None),
+    "fmd_matt_translate": ([P, C.c_int, D, D, D], None),
     "fmd_matt_addVelocity": ([P, C.c_int, D, D, D], None),
     "fmd_matt_getTotalEnergy": ([P], D), "fmd_matt_getKineticEnergy": ([P], D),
     "fmd_matt_getTemperature": ([P], D), "fmd_matt_getMomentum": ([P, P], None),

@@ -146,6 +146,46 @@ def test_save_configuration_after_dynamics(rng, tmp_path):
     assert (tmp_path / ("%d.xyz" % len(x))).read_text() == T.expected(ref, api.SCM_XYZ_ATOMSNUM)
 
 
+def test_matter_functions_after_distribution():
+    """The matter functions once the atoms live on the device: findLimits, changeGroupID,
+    addVelocity and giveMaxwellDistribution act there and the getters read the device's
+    sums; translate is refused after distribution (matter.c:500-505)."""
+    from tests import test_matter_host as H
+    f = H.new_fmd([40.0] * 3)
+    errs = H.errors_of(f)
+    f.fmd_matt_makeCuboidFCC(0.0, 0.0, 0.0, 5, 5, 5, 5.6, 0, 1, 80.0)
+    f.fmd_matt_makeCuboidBCC(28.5, 1.0, 2.0, 2, 2, 2, 5.0, 1, 2, 80.0)
+    f.fmd_dync_integrate(api.GROUP_ALL, 0.005, 1e-3)        # distributes and steps
+    a = f.atoms()
+    lo, hi = (C.c_double * 3)(), (C.c_double * 3)()
+    for g in (1, 2, -1):
+        f.fmd_matt_findLimits(g, lo, hi)
+        sel = a["x"] if g == -1 else a["x"][a["group"] == g]
+        assert list(lo) == list(sel.min(axis=0)) and list(hi) == list(sel.max(axis=0))
+    f.fmd_matt_changeGroupID(2, 5)
+    b = f.atoms()
+    assert np.array_equal(b["group"], np.where(a["group"] == 2, 5, a["group"]))
+    f.fmd_matt_addVelocity(5, 1.0, -2.0, 0.5)
+    c = f.atoms()
+    dv = c["v"] - b["v"]
+    assert np.allclose(dv[b["group"] == 5], [1.0, -2.0, 0.5], rtol=0, atol=1e-12)
+    assert np.all(dv[b["group"] != 5] == 0.0)
+    ke, m = H.kinetic(c)
+    mom = (C.c_double * 3)()
+    f.fmd_matt_getMomentum(mom)
+    assert np.allclose(list(mom), np.sum(m[:, None] * c["v"], axis=0) * MASS_UNIT, rtol=1e-12, atol=1e-9)
+    assert f.fmd_matt_getKineticEnergy() == pytest.approx(ke, rel=1e-12)
+    f.fmd_matt_giveMaxwellDistribution(1, 900.0)
+    d = f.atoms()
+    assert np.array_equal(d["v"][d["group"] == 5], c["v"][c["group"] == 5])
+    assert np.std(d["v"][d["group"] == 1]) == pytest.approx(np.sqrt(KB * 900.0 * MASS_UNIT / H.MASSES[0]), rel=0.06)
+    n_err = len(errs)
+    f.fmd_matt_translate(1, 1.0, 0.0, 0.0)
+    assert len(errs) == n_err + 1
+    assert np.array_equal(f.atoms()["x"], d["x"])
+    f.free()
+
+
 def test_timer_events_and_state_roundtrip(rng, tmp_path):
     x, v, L = _lj_pair(rng, n=(6, 6, 6))
     f = api.Fmd()
