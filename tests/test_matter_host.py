@@ -245,3 +245,38 @@ def test_limits_groups_velocities_and_getters():
     v1 = d["v"][d["group"] == 1]
     assert np.std(v1) == pytest.approx(math.sqrt(KB * 900.0 * MASS_UNIT / MASSES[0]), rel=0.06)
     f.free()
+
+
+def test_version_process_and_printf(capfd):
+    """version.c:25-61, misc.c:571-617, subdomain.c:325-341: one process that is root and
+    owns the whole box; the cell increment (the reference's cell-memory growth step)
+    defaults to 3 and refuses values below 1; fmd_io_printf prints on the root."""
+    L = api.lib()
+    for name, res in (("fmd_version_getMajor", C.c_int), ("fmd_version_getMinor", C.c_int),
+                      ("fmd_version_getRevision", C.c_int), ("fmd_version_getType", C.c_char)):
+        getattr(L, name).restype = res
+        getattr(L, name).argtypes = []
+    assert (L.fmd_version_getMajor(), L.fmd_version_getMinor(), L.fmd_version_getRevision()) == (0, 4, 0)
+    assert L.fmd_version_getType() == b"r"
+    assert L.fmd_version_getString() == b"0.4.0"
+    f = new_fmd([30.0] * 3)
+    errs = errors_of(f)
+    for name, res in (("fmd_proc_isRoot", C.c_bool), ("fmd_proc_hasSubdomain", C.c_bool),
+                      ("fmd_proc_getWallTime", C.c_double), ("fmd_proc_getCellIncrement", C.c_int)):
+        getattr(L, name).restype = res
+        getattr(L, name).argtypes = [C.c_void_p]
+    L.fmd_proc_setCellIncrement.argtypes = [C.c_void_p, C.c_int]
+    L.fmd_proc_setNumThreads.argtypes = [C.c_void_p, C.c_int]
+    assert L.fmd_proc_isRoot(f.md) and L.fmd_proc_hasSubdomain(f.md)
+    t0 = L.fmd_proc_getWallTime(f.md)
+    t1 = L.fmd_proc_getWallTime(f.md)
+    assert 0.0 <= t0 <= t1
+    assert L.fmd_proc_getCellIncrement(f.md) == 3
+    L.fmd_proc_setCellIncrement(f.md, 5)
+    L.fmd_proc_setCellIncrement(f.md, 0)
+    assert L.fmd_proc_getCellIncrement(f.md) == 5 and len(errs) == 1
+    L.fmd_proc_setNumThreads(f.md, 4)
+    L.fmd_io_printf(f.md, b"step %d of %s: %.3f\n", C.c_int(7), b"run", C.c_double(2.5))
+    C.CDLL(None).fflush(None)   # vprintf leaves the line in the C library's buffer
+    assert capfd.readouterr().out.endswith("step 7 of run: 2.500\n")
+    f.free()
