@@ -426,6 +426,16 @@ def copy_calibration(mib):
     log("bench: copied %d bytes x3" % n)
 
 
+def guarded(name, fn, *a, **k):
+    """Run an optional leg; a failure is logged and reported in the JSON line instead of
+    ending the run, so that the headline figure is still printed."""
+    try:
+        return fn(*a, **k)
+    except Exception as e:
+        log("bench: %s failed: %r" % (name, e))
+        return {"error": repr(e)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -469,36 +479,41 @@ def main():
         other = None
         if not args.no_mixed:
             p2 = F.FMDX_PREC_MIXED if prec == F.FMDX_PREC_FP64 else F.FMDX_PREC_FP64
-            other = gpu_bench(args.cells, args.steps, args.warmup, p2, setfl)
+            other = guarded("other precision", gpu_bench, args.cells, args.steps, args.warmup, p2, setfl)
             log("bench: other precision %s" % other)
         ens = srv = ttm = None
         drift = {}
         if not args.no_drift:
             cu01 = os.path.join(ROOT, "tests", "golden", "data", "Cu01.eam.alloy")
             for pname, pv in (("fp64", F.FMDX_PREC_FP64), ("mixed", F.FMDX_PREC_MIXED)):
-                drift["cu01_" + pname] = nve_drift(min(args.cells, 40), args.drift_steps, pv, cu01)
-                drift["synthetic_" + pname] = nve_drift(min(args.cells, 40), args.drift_steps, pv, setfl)
-            drift["cu01_mixed_minus_fp64_eV_per_atom_per_ns"] = (drift["cu01_mixed"]["drift_eV_per_atom_per_ns"] -
-                                                                drift["cu01_fp64"]["drift_eV_per_atom_per_ns"])
+                drift["cu01_" + pname] = guarded("NVE drift Cu01 " + pname, nve_drift, min(args.cells, 40),
+                                                 args.drift_steps, pv, cu01)
+                drift["synthetic_" + pname] = guarded("NVE drift synthetic " + pname, nve_drift,
+                                                      min(args.cells, 40), args.drift_steps, pv, setfl)
+            if "error" not in drift["cu01_mixed"] and "error" not in drift["cu01_fp64"]:
+                drift["cu01_mixed_minus_fp64_eV_per_atom_per_ns"] = (
+                    drift["cu01_mixed"]["drift_eV_per_atom_per_ns"] - drift["cu01_fp64"]["drift_eV_per_atom_per_ns"])
             drift["note"] = ("Cu01 is the smooth Mishin setfl; the synthetic table is truncated at 5.5 A "
                              "without a taper, so its drift measures the cutoff jumps")
             log("bench: NVE drift %s" % drift)
         if not args.no_ttm:
-            ttm = ttm_slab(args.ttm_steps, F.FMDX_PREC_MIXED, setfl)
+            ttm = guarded("TTM slab", ttm_slab, args.ttm_steps, F.FMDX_PREC_MIXED, setfl)
             log("bench: TTM slab %s" % ttm)
         if not args.no_batch:
-            ens = batched_ensemble(args.batch_systems, args.batch_steps, args.plain_sample)
+            ens = guarded("batched ensemble", batched_ensemble, args.batch_systems, args.batch_steps,
+                          args.plain_sample)
             log("bench: batched ensemble %s" % ens)
         if not args.no_serve:
-            srv = served_collisions(args.serve_jobs, args.serve_rate)
+            srv = guarded("served collisions", served_collisions, args.serve_jobs, args.serve_rate)
             log("bench: served collisions %s" % srv)
         cpu = None
         if not args.no_cpu_baseline:
-            cpu = cpu_baseline(setfl, threads=host_threads())
+            cpu = guarded("cpu baseline", cpu_baseline, setfl, threads=host_threads())
             log("bench: cpu baseline %s" % cpu)
-            cpu1 = cpu_baseline(setfl, steps=10)
+            cpu1 = guarded("cpu baseline, one thread", cpu_baseline, setfl, steps=10)
             log("bench: cpu baseline, one thread %s" % cpu1)
-            cpu["one_thread"] = cpu1["value"]
+            if "error" not in cpu:
+                cpu["one_thread"] = cpu1.get("value")
     n, K = r["natoms"], r["steps"]
     ms_step = r["gpu_ms"] / K
     value = n * K / (r["gpu_ms"] * 1e-3)
@@ -529,7 +544,9 @@ def main():
         "step_roofline_frac": STEP_BYTES[prec] * value / (HBM_PEAK_GBS * 1e9),
         "pass1_ms": r["pass1_ms"],
     }
-    if other is not None:
+    if other is not None and "error" in other:
+        out["other_precision"] = other
+    elif other is not None:
         out["other_precision"] = {"precision": "mixed" if prec == F.FMDX_PREC_FP64 else "fp64",
                                   "value": other["natoms"] * other["steps"] / (other["gpu_ms"] * 1e-3),
                                   "ms_per_step": other["gpu_ms"] / other["steps"],
@@ -542,14 +559,17 @@ def main():
         out["batched_ensemble"] = ens
     if srv is not None:
         out["served_collisions"] = srv
-    if cpu is not None:
+    if cpu is not None and "error" in cpu:
+        out["cpu_baseline"] = {"value": None, "unit": "atom-steps/s", "cores": host_threads(), "kind": "port",
+                               "sample": None, "error": cpu["error"]}
+    elif cpu is not None:
         out["cpu_baseline"] = {"value": cpu["value"], "unit": "atom-steps/s", "cores": cpu["threads"], "kind": "port",
                                "sample": "oracle (FP64 reference restatement) on EAM Cu: %d concurrent copies, one "
                                          "per host thread (%s), of a %d-atom sample (32^3 FCC, same setfl and "
                                          "lattice recipe), %d Berendsen VV steps each, %.1f s; one copy alone: "
                                          "%.3g atom-steps/s"
                                          % (cpu["threads"], cpu_model(), cpu["natoms"], cpu["steps"], cpu["seconds"],
-                                            cpu["one_thread"]),
+                                            cpu["one_thread"] or float("nan")),
                                "one_thread": cpu["one_thread"]}
     print(json.dumps(out), flush=True)
 
