@@ -283,13 +283,18 @@ extern "C" int fmdx_set_eam(fmdx_system_t *s, int nelem, int nrho, double drho, 
     return FMDX_OK;
 }
 
+/* Slot capacities are multiples of 32, so that every cell's run of 4- and
+   8-byte values starts on a 128-byte cache line: a copper cell's ~15 atoms
+   then sit in one line of each array instead of straddling two. */
+static unsigned line_cap(unsigned c)
+{
+    c = (c + 31) & ~31u;
+    return c > 256 ? 256 : c;
+}
+
 static int choose_cap(unsigned maxc)
 {
-    unsigned c = maxc + maxc / 2 + 8;
-    if (c < 16) c = 16;
-    c = (c + 7) & ~7u;
-    if (c > 256) c = 256;
-    return (int)c;
+    return (int)line_cap(maxc + maxc / 2 + 8);
 }
 
 extern "C" int fmdx_upload(fmdx_system_t *s, unsigned natoms, const unsigned *cell_count, const double *x,

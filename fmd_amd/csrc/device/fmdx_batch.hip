@@ -215,7 +215,7 @@ static int grow(fmdx_batch_t *b)
         const unsigned cap = (unsigned)s->D.cap;
         if (cap >= 256 || maxc + 8 + maxc / 4 <= cap) continue;
         unsigned nc = maxc + maxc / 2 + 8;
-        nc = (nc + 7) & ~7u;
+        nc = (nc + 31) & ~31u;   /* multiples of 32 slots: each cell starts on a cache line */
         if (nc > 256) nc = 256;
         int r = fmdx_set_capacity(s, nc);
         if (r) return r;

@@ -26,11 +26,12 @@ fmd_real_t fmd_dync_getTime(fmd_t *md) { return md->time; }
 
 /* ---------------------------------------------------------------- device */
 
+/* the same rule as the device library's first choice: 1.5 x the fullest cell + 8,
+   in multiples of 32 slots so that each cell's values start on a 128-byte line */
 static unsigned choose_cap(unsigned maxc)
 {
     unsigned c = maxc + maxc / 2 + 8;
-    if (c < 16) c = 16;
-    c = (c + 7) & ~7u;
+    c = (c + 31) & ~31u;
     return c > 256 ? 256 : c;
 }
 

@@ -159,7 +159,7 @@ int fmdx_add_velocity(fmdx_system_t *s, int group, double vx, double vy, double 
 int fmdx_read_thermo(fmdx_system_t *s, fmdx_thermo_t *out);
 int fmdx_clear_errors(fmdx_system_t *s);
 int fmdx_sync(fmdx_system_t *s);
-/* grow the per-cell slot capacity (synchronises) */
+/* set the per-cell slot capacity (synchronises); the library's own choices are multiples of 32 */
 int fmdx_set_capacity(fmdx_system_t *s, unsigned cap);
 
 /* 1-D TTM on the device */

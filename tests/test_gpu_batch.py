@@ -139,6 +139,7 @@ def test_batch_grows_capacity(rng):
     _, e = _make(spec, F.FMDX_PREC_FP64)
     mx = int(np.max(d.download()["cell_count"]))
     d.set_capacity(mx + 4)
+    assert d.read().cap == mx + 4
     b = F.Batch(F.FMDX_PREC_FP64)
     b.set_check_interval(1)
     b.add(d, 2e-3)
@@ -146,7 +147,7 @@ def test_batch_grows_capacity(rng):
     _run_single(e, 40, 2e-3)
     t = b.status()[0]
     assert t.err == 0
-    assert t.cap > mx + 4
+    assert t.cap > mx + 4 and t.cap % 32 == 0   # grown by the batch, to whole cache lines
     _same(d, e)
     b.close()
 
