@@ -378,27 +378,102 @@ static __device__ __forceinline__ void copy_staged(const KDesc &D, int mid, long
     D.grp[to] = D.mvgrp[mid]; D.aid[to] = D.mvaid[mid]; D.kind[to] = D.mvkind[mid];
 }
 
-/* replay of the swap-remove scan (misc.c:279-303) to rank and stage leavers */
+/* The replays below walk a cell's move flags in a data-dependent order, one
+   dependent read per step.  A cell of up to MVL atoms first copies its flags
+   into an LDS row with independent 16-byte loads (the library's slot
+   capacities are multiples of 32, so the row never reads past the cell and
+   the loads are aligned); larger cells, and capacities set by a caller to
+   other values, read HBM directly. */
+#define MVL 32
+struct MvRow
+{
+    const int *g;
+    int *row;
+    bool local;
+    __device__ __forceinline__ int operator[](int k) const { return local ? row[k] : g[k]; }
+};
+
+static __device__ __forceinline__ MvRow mv_row(const KDesc &D, long base, int nc, int (*rows)[MVL + 1])
+{
+    MvRow r;
+    r.g = D.mv + base;
+    r.row = rows[threadIdx.x];
+    r.local = nc <= MVL && D.cap >= MVL && (D.cap & 3) == 0;
+    if (r.local)
+    {
+        const int4 *src = reinterpret_cast<const int4 *>(D.mv + base);
+        int4 q[MVL / 4];
+#pragma unroll
+        for (int k = 0; k < MVL / 4; k++) q[k] = src[k];
+#pragma unroll
+        for (int k = 0; k < MVL / 4; k++)
+        {
+            r.row[4 * k] = q[k].x; r.row[4 * k + 1] = q[k].y;
+            r.row[4 * k + 2] = q[k].z; r.row[4 * k + 3] = q[k].w;
+        }
+    }
+    return r;
+}
+
+#define WAVE 64   /* lanes per wavefront on gfx950 */
+
+/* exclusive prefix of v over the block, and the block total, through the waves' shuffles */
+static __device__ __forceinline__ unsigned block_prefix(unsigned v, unsigned &total)
+{
+    __shared__ unsigned wsum[TPB / WAVE];
+    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+    unsigned incl = v;
+    for (int o = 1; o < WAVE; o <<= 1)
+    {
+        const unsigned t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    if (lane == WAVE - 1) wsum[w] = incl;
+    __syncthreads();
+    unsigned before = 0;
+    total = 0;
+    for (int k = 0; k < TPB / WAVE; k++)
+    {
+        if (k < w) before += wsum[k];
+        total += wsum[k];
+    }
+    __syncthreads();
+    return before + incl - v;
+}
+
+/* replay of the swap-remove scan (misc.c:279-303) to rank and stage leavers.
+   Each block reserves one run of staging entries for all its cells with a
+   single atomic (a cell takes one entry per leaver, deletions included, so
+   the staging buffer of natoms entries always suffices); one atomic per
+   leaver on the shared counter serialised across the whole device. */
 static __device__ void b_bin_rank(const KDesc &D, int blk)
 {
 #pragma clang fp contract(off)
-    int c = blk * TPB + threadIdx.x;
-    if (c >= D.ncell) return;
-    if (D.nleave[c] == 0) return;
+    __shared__ int rows[TPB][MVL + 1];
+    __shared__ unsigned run0;
+    const int c = blk * TPB + threadIdx.x;
+    const int nl = c < D.ncell ? D.nleave[c] : 0;
+    unsigned total;
+    const unsigned off = block_prefix((unsigned)nl, total);
+    if (threadIdx.x == 0) run0 = total ? atomicAdd(&D.counters[0], total) : 0u;
+    __syncthreads();
+    if (nl == 0) return;
+    const unsigned mid0 = run0 + off;
     const int nc = D.cnt[c];
     const long base = (long)c * D.cap;
+    const MvRow mv = mv_row(D, base, nc, rows);
     int i = 0, n = nc + D.nearly[c], cur = 0, rank = 0;
     while (i < n)
     {
-        if (cur < nc && D.mv[base + cur] != MV_STAY)
+        if (cur < nc && mv[cur] != MV_STAY)
         {
             long s = base + cur;
-            int dst = D.mv[s];
+            int dst = mv[cur];
             if (dst == MV_DELETE)
                 atomicAdd(&D.counters[1], 1u);
             else
             {
-                int mid = (int)atomicAdd(&D.counters[0], 1u);
+                const int mid = (int)(mid0 + rank);
                 int code = D.code[s];
                 double x = D.x[s], y = D.y[s], z = D.z[s];
                 /* ccopy_for_migrate, md-ghost.c:132-140: x += L (lower face) or x += -L (upper) */
@@ -436,15 +511,35 @@ static __device__ void b_bin_rank(const KDesc &D, int blk)
 DEF_KERNELS(bin_rank)
 
 /* place stayers, early and late arrivals of every affected cell */
+static __device__ int bin_apply_cell(const KDesc &D, int c, int nl, int na, int (*rows)[MVL + 1]);
+
 static __device__ void b_bin_apply(const KDesc &D, int blk)
 {
-    int c = blk * TPB + threadIdx.x;
-    if (c >= D.ncell) return;
-    const int nl = D.nleave[c], na = D.narr[c];
-    if (nl == 0 && na == 0) return;
+    __shared__ int rows[TPB][MVL + 1];
+    const int c = blk * TPB + threadIdx.x;
+    const int nl = c < D.ncell ? D.nleave[c] : 0, na = c < D.ncell ? D.narr[c] : 0;
+    int total = 0;
+    if (nl != 0 || na != 0) total = bin_apply_cell(D, c, nl, na, rows);
+    /* the fullest cell: one atomic per block (a per-cell atomic on the shared word serialises) */
+    unsigned m = (unsigned)total;
+    for (int o = WAVE / 2; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor(m, o));
+    __shared__ unsigned wmax[TPB / WAVE];
+    if (threadIdx.x % WAVE == 0) wmax[threadIdx.x / WAVE] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        unsigned bm = 0;
+        for (int k = 0; k < TPB / WAVE; k++) bm = max(bm, wmax[k]);
+        if (bm) atomicMax(&D.counters[2], bm);
+    }
+}
+
+static __device__ int bin_apply_cell(const KDesc &D, int c, int nl, int na, int (*rows)[MVL + 1])
+{
     const int nc = D.cnt[c];
     const int A = D.nearly[c];
     const long base = (long)c * D.cap;
+    const MvRow mv = mv_row(D, base, nc, rows);
     const int nb = na < D.cap ? na : D.cap;
     /* insertion sort of the arrival bucket by key */
     for (int a = 1; a < nb; a++)
@@ -464,7 +559,7 @@ static __device__ void b_bin_apply(const KDesc &D, int blk)
     int i = 0, n = nc + A, cur = 0;
     while (i < n)
     {
-        if (cur < nc && D.mv[base + cur] != MV_STAY)
+        if (cur < nc && mv[cur] != MV_STAY)
         {
             n--;
             if (n != i) cur = n;
@@ -494,11 +589,11 @@ static __device__ void b_bin_apply(const KDesc &D, int blk)
         total = D.cap;
     }
     D.cnt[c] = total;
-    atomicMax(&D.counters[2], (unsigned)total);
     D.nleave[c] = 0;
     D.narr[c] = 0;
     D.nearly[c] = 0;
     D.fill[c] = 0;
+    return total;
 }
 DEF_KERNELS(bin_apply)
 
