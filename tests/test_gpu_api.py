@@ -258,6 +258,69 @@ def _ttm_cfg_for_oracle(gamma, K, G, ratio, laser, ncz):
     return c
 
 
+def test_ttm_type2_matches_oracle(rng, cu01, data_dir):
+    """examples/04_ttm2.c's type-2 turi (Ce and G read from its tables, Zhigilei
+    conductivity) on a small slab, through the public API on the GPU and through the
+    oracle's 1-D type-2 solver (ttm.c:155-192, 300-384), same inputs and steps."""
+    import os
+    from tests.test_oracle_ttm3d import _table, J_M3K2, W_M3K
+    lp = 3.6316
+    L = [6 * lp, 6 * lp, 40 * lp]
+    x = lattice_fcc((6, 6, 30), lp, rng, jitter=0.02, origin=(0.0, 0.0, 5.0 * lp))
+    v = maxwell_velocities(len(x), 63.546, 600.0, rng)
+    laser = dict(fluence=2e4, reflectance=0.85, duration=100e-15, t0=5e-15, AbsorptionDepth=14e-9)
+    zh = api.Zhigilei(v=1.57e6, A=2.66e6, B=2.41e11)   # fields in the order of fmd.h: v, A, B
+    f = api.Fmd()
+    f.fmd_box_setSize(*L)
+    f.fmd_box_setPBC(True, True, False)
+    f.set_kinds(["Cu"], [63.546])
+    pot = f.fmd_pot_eam_alloy_load(cu01.encode())
+    f.fmd_pot_apply(0, 0, pot)
+    f.load_atoms(x, v)
+    f.fmd_ext_distribute()
+    turi = f.fmd_turi_add(api.TURI_TTM_TYPE2, 1, 1, 20, 0.0, -1.0)
+    fTe, fn = f.fmd_field_find(turi, api.FIELD_TTM_TE), f.fmd_field_find(turi, api.FIELD_NUMBER)
+    f._fmd_ttm_setHeatCapacity_file(turi, os.path.join(data_dir, "Ce_Cu.dat").encode())
+    f._fmd_ttm_setHeatConductivity_zhigilei(turi, zh)
+    f._fmd_ttm_setCouplingFactor_file(turi, os.path.join(data_dir, "G_Cu.dat").encode())
+    f.fmd_ttm_setElectronTemperature(turi, 303.0)
+    f.fmd_ttm_setTimestepRatio(turi, 20)
+    f.fmd_ttm_setCellActivationFraction(turi, 0.1)
+    f._fmd_ttm_setLaserSource_gaussian(turi, api.LaserGaussian(laser["fluence"], laser["reflectance"],
+                                                                laser["t0"], laser["duration"],
+                                                                laser["AbsorptionDepth"]))
+    f.fmd_dync_integrate(api.GROUP_ALL, 0.01, 1e-3)
+    c = _ttm_cfg_for_oracle(1.0, 0.0, 0.0, 20, laser, 20)
+    c.type = 2
+    T_C, Ct = _table(os.path.join(data_dir, "Ce_Cu.dat"), 1e4, 1e5 * J_M3K2)
+    T_G, Gt = _table(os.path.join(data_dir, "G_Cu.dat"), 1e4, 1e17 * W_M3K)
+    c.nC, c.nG = len(T_C), len(T_G)
+    c.T_C, c.Ct, c.T_G, c.Gt = T_C.ctypes.data, Ct.ctypes.data, T_G.ctypes.data, Gt.ctypes.data
+    c.Kzh_A, c.Kzh_B, c.Kzh_v2 = 2.66e6 / 1e12, 2.41e11 / 1e12, (1.57e6 * 1e-2) ** 2
+    o = Oracle(L, [1, 1, 0], [63.546], ["Cu"])
+    o.load_eam(cu01); o.apply_eam(0, 0)
+    o.make_grid(); o.add_atoms(x, v); o.distribute()
+    o.thermo()
+    o.ttm_init(c, 303.0)
+    steps, _ = o.integrate_se(0.0, 0.01, 1e-3)
+    assert steps == 10
+    assert o.errors()[0] == 0
+    r = o.ttm_arrays(22)
+    te = f.field_array(turi, fTe)
+    n = f.field_array(turi, fn, np.uint32)
+    assert np.array_equal(n, r["num"][1:21])
+    act = r["Te"][1:21] >= 0
+    assert np.array_equal(te < 0, ~act)
+    assert np.max(te[act]) > 320.0                       # the pulse heated the electrons
+    assert rel_err(te[act], r["Te"][1:21][act]) < 1e-10
+    ga, ra = f.atoms(), o.atoms()
+    assert np.array_equal(ga["atomid"], ra["atomid"])
+    assert rel_err(ga["x"], ra["x"], scale=L[2]) < 1e-12
+    assert rel_err(ga["v"], ra["v"]) < 1e-9
+    del T_C, Ct, T_G, Gt
+    f.free()
+
+
 def test_ttm_type1_matches_oracle(rng, cu01):
     # examples/03_ttm1.c geometry scaled down: Cu slab, open z faces, 1 x 1 x 20 turi
     lp = 3.6316
