@@ -305,10 +305,37 @@ static __device__ __forceinline__ void pair_hit(const KDesc &D, const PairParam 
     }
 }
 
+/* Coefficient records through a buffer resource over the rp table: a lane
+   without a pair passes an offset beyond the table, which the buffer unit
+   answers with zeros and no memory access, so the gathers stay branch-free
+   while only the lanes with a pair fetch. */
+/* The 128-bit result is reinterpreted as a whole vector: converting its
+   elements one by one (bit_cast of each lane of the returned vector) has been
+   compiled into a single 32-bit load whose value fills every component. */
+typedef unsigned U4 __attribute__((ext_vector_type(4)));
+typedef float F4 __attribute__((ext_vector_type(4)));
+typedef double D2 __attribute__((ext_vector_type(2)));
+static __device__ __forceinline__ __amdgpu_buffer_rsrc_t rp_rsrc(const void *base, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, (int)bytes, 0x00020000);
+}
+static __device__ __forceinline__ double4 rec_at(__amdgpu_buffer_rsrc_t r, unsigned off, double4 *)
+{
+    const D2 a = __builtin_bit_cast(D2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    const D2 b = __builtin_bit_cast(D2, __builtin_amdgcn_raw_buffer_load_b128(r, off + 16u, 0, 0));
+    return make_double4(a.x, a.y, b.x, b.y);
+}
+static __device__ __forceinline__ float4 rec_at(__amdgpu_buffer_rsrc_t r, unsigned off, float4 *)
+{
+    const F4 a = __builtin_bit_cast(F4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    return make_float4(a.x, a.y, a.z, a.w);
+}
+#define RP_NONE 0x80000000u   /* an offset beyond any table */
+
 /* Single-element EAM: the set bits of a hit word, two per iteration and
    without branches, so that the coefficient gathers of both pairs are in
    flight together.  A slot without a pair (or outside the pair cutoff)
-   evaluates interval 0 and contributes an exact zero. */
+   reads zeros (no memory access) and contributes an exact zero. */
 template <typename R, int PASS, typename TileT, typename A>
 static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T, unsigned long long word, int s,
                                                   int g0, int S, R xi, R yi, R zi, R Fpi, R rc2, A *acc)
@@ -317,7 +344,8 @@ static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T
     const EamDev &E = D.eam;
     const R inv = (R)E.inv_dr2;
     const int kmax = E.nr2 - 2;
-    const Q *rp = Prec<R>::rp(E);
+    const unsigned rec = (unsigned)sizeof(Q);   /* one of the two halves of an interval's record */
+    const __amdgpu_buffer_rsrc_t rb = rp_rsrc(Prec<R>::rp(E), (unsigned)(E.nr2 - 1) * 2u * rec);
     while (__ballot(word != 0ull))
     {
         const bool v1 = word != 0ull;
@@ -342,15 +370,18 @@ static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T
         const R b1 = tt1 - (R)k1, b2 = tt2 - (R)k2;
         if (PASS == 1)
         {
-            const Q q1 = rp[2 * (long)k1], q2 = rp[2 * (long)k2];
+            const Q q1 = rec_at(rb, u1 ? (unsigned)k1 * 2u * rec : RP_NONE, (Q *)nullptr);
+            const Q q2 = rec_at(rb, u2 ? (unsigned)k2 * 2u * rec : RP_NONE, (Q *)nullptr);
             const R c1 = sp_val<R, Q>(q1, b1), c2 = sp_val<R, Q>(q2, b2);
             acc[3].add(u1 ? c1 : (R)0);
             acc[3].add(u2 ? c2 : (R)0);
         }
         else
         {
-            const Q qr1 = rp[2 * (long)k1], qp1 = rp[2 * (long)k1 + 1];
-            const Q qr2 = rp[2 * (long)k2], qp2 = rp[2 * (long)k2 + 1];
+            const unsigned o1 = u1 ? (unsigned)k1 * 2u * rec : RP_NONE;
+            const unsigned o2 = u2 ? (unsigned)k2 * 2u * rec : RP_NONE;
+            const Q qr1 = rec_at(rb, o1, (Q *)nullptr), qp1 = rec_at(rb, o1 + rec, (Q *)nullptr);
+            const Q qr2 = rec_at(rb, o2, (Q *)nullptr), qp2 = rec_at(rb, o2 + rec, (Q *)nullptr);
             const R rho1 = sp_der<R, Q>(qr1, b1, inv), rho2 = sp_der<R, Q>(qr2, b2, inv);
             const R ph1 = sp_der<R, Q>(qp1, b1, inv), ph2 = sp_der<R, Q>(qp2, b2, inv);
             const R mag1 = (R)2 * (Fpi * rho1 + f1 * rho1 + ph1);
