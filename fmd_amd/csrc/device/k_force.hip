@@ -220,10 +220,18 @@ static __device__ __forceinline__ R sp_der(const Q &q, R b, R inv_h)
     return (q.y + b * ((R)2 * q.z + (R)3 * b * q.w)) * inv_h;
 }
 
+/* FP64 keeps the library's exp and sqrt and true divisions.  FP32 (the mixed
+   build, whose bar is 1e-5 of max|F|) uses the hardware's single instructions:
+   v_sqrt_f32 and v_rcp_f32 (about 1 ulp) and v_exp_f32 on x log2(e), whose
+   relative error grows as |x| eps, below 1e-6 for the Morse exponents met
+   within a cutoff (|x| < 10) -- the library sequences are several times
+   longer, and the Morse pass is bound by them. */
 static __device__ __forceinline__ double fexp(double x) { return exp(x); }
-static __device__ __forceinline__ float fexp(float x) { return expf(x); }
+static __device__ __forceinline__ float fexp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
 static __device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
-static __device__ __forceinline__ float fsqrt(float x) { return sqrtf(x); }
+static __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+static __device__ __forceinline__ double frcp(double x) { return 1.0 / x; }
+static __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 /* one candidate pair inside the search radius; acc = {fx, fy, fz, energy or density} */
 template <typename R, int PASS, int CAT, typename A>
@@ -282,7 +290,7 @@ static __device__ __forceinline__ void pair_hit(const KDesc &D, const PairParam 
     }
     else if (cat == FMDX_POT_LJ)
     {
-        const R inv_r2 = (R)1 / r2;
+        const R inv_r2 = frcp(r2);
         const R inv_rs2 = (R)p.p1 * inv_r2;
         const R inv_rs6 = inv_rs2 * inv_rs2 * inv_rs2;
         const R inv_rs12 = inv_rs6 * inv_rs6;
@@ -297,7 +305,8 @@ static __device__ __forceinline__ void pair_hit(const KDesc &D, const PairParam 
         const R r = fsqrt(r2);
         const R e1 = fexp(-(R)p.p1 * (r - (R)p.p2));
         const R e2 = e1 * e1;
-        const R factor = (R)2 * (R)p.p1 * (R)p.p0 / r * (e2 - e1);
+        const R factor = sizeof(R) == sizeof(double) ? (R)2 * (R)p.p1 * (R)p.p0 / r * (e2 - e1)
+                                                     : (R)2 * (R)p.p1 * (R)p.p0 * frcp(r) * (e2 - e1);
         acc[0].add(factor * dx);
         acc[1].add(factor * dy);
         acc[2].add(factor * dz);
