@@ -233,6 +233,29 @@ static __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn
 static __device__ __forceinline__ double frcp(double x) { return 1.0 / x; }
 static __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
+/* LJ (lj.h:39-57) and Morse (morse.h:42-56) for one pair at r^2 inside the
+   cutoff: the force on atom i is fac * (x_i - x_j), the energy share en */
+template <typename R>
+static __device__ __forceinline__ void lj_terms(const PairParam &p, R r2, R &fac, R &en)
+{
+    const R inv_r2 = frcp(r2);
+    const R inv_rs2 = (R)p.p1 * inv_r2;
+    const R inv_rs6 = inv_rs2 * inv_rs2 * inv_rs2;
+    const R inv_rs12 = inv_rs6 * inv_rs6;
+    fac = (R)12 * (R)p.p0 * inv_r2 * (inv_rs12 - (R)0.5 * inv_rs6);
+    en = (R)0.5 * (R)p.p0 * (inv_rs12 - inv_rs6);
+}
+template <typename R>
+static __device__ __forceinline__ void morse_terms(const PairParam &p, R r2, R &fac, R &en)
+{
+    const R r = fsqrt(r2);
+    const R e1 = fexp(-(R)p.p1 * (r - (R)p.p2));
+    const R e2 = e1 * e1;
+    fac = sizeof(R) == sizeof(double) ? (R)2 * (R)p.p1 * (R)p.p0 / r * (e2 - e1)
+                                      : (R)2 * (R)p.p1 * (R)p.p0 * frcp(r) * (e2 - e1);
+    en = (R)0.5 * (R)p.p0 * (e2 - (R)2 * e1);
+}
+
 /* one candidate pair inside the search radius; acc = {fx, fy, fz, energy or density} */
 template <typename R, int PASS, int CAT, typename A>
 static __device__ __forceinline__ void pair_hit(const KDesc &D, const PairParam &p, R Fpi, R Fpj, R dx, R dy,
@@ -288,29 +311,15 @@ static __device__ __forceinline__ void pair_hit(const KDesc &D, const PairParam 
         acc[2].add(-mag * dz);
         acc[3].add((R)0.5 * sp_val<R, Q>(q, b));
     }
-    else if (cat == FMDX_POT_LJ)
+    else if (cat == FMDX_POT_LJ || cat == FMDX_POT_MORSE)
     {
-        const R inv_r2 = frcp(r2);
-        const R inv_rs2 = (R)p.p1 * inv_r2;
-        const R inv_rs6 = inv_rs2 * inv_rs2 * inv_rs2;
-        const R inv_rs12 = inv_rs6 * inv_rs6;
-        const R factor = (R)12 * (R)p.p0 * inv_r2 * (inv_rs12 - (R)0.5 * inv_rs6);
-        acc[0].add(factor * dx);
-        acc[1].add(factor * dy);
-        acc[2].add(factor * dz);
-        acc[3].add((R)0.5 * (R)p.p0 * (inv_rs12 - inv_rs6));
-    }
-    else if (cat == FMDX_POT_MORSE)
-    {
-        const R r = fsqrt(r2);
-        const R e1 = fexp(-(R)p.p1 * (r - (R)p.p2));
-        const R e2 = e1 * e1;
-        const R factor = sizeof(R) == sizeof(double) ? (R)2 * (R)p.p1 * (R)p.p0 / r * (e2 - e1)
-                                                     : (R)2 * (R)p.p1 * (R)p.p0 * frcp(r) * (e2 - e1);
-        acc[0].add(factor * dx);
-        acc[1].add(factor * dy);
-        acc[2].add(factor * dz);
-        acc[3].add((R)0.5 * (R)p.p0 * (e2 - (R)2 * e1));
+        R fac, en;
+        if (cat == FMDX_POT_LJ) lj_terms<R>(p, r2, fac, en);
+        else morse_terms<R>(p, r2, fac, en);
+        acc[0].add(fac * dx);
+        acc[1].add(fac * dy);
+        acc[2].add(fac * dz);
+        acc[3].add(en);
     }
 }
 
@@ -405,6 +414,46 @@ static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T
             acc[2].add(u2 ? -mag2 * dz2 : (R)0);
             acc[3].add(u2 ? e2 : (R)0);
         }
+    }
+}
+
+/* Single-kind LJ or Morse: the set bits of a hit word two per iteration
+   without branches, as flush_eam1 does; a slot without a pair evaluates the
+   formulas at r^2 = rc^2 and adds exact zeros. */
+template <typename R, int CAT, typename TileT, typename A>
+static __device__ __forceinline__ void flush_pair2(const TileT &T, unsigned long long word, int s, int g0, int S,
+                                                   R xi, R yi, R zi, const PairParam &p, A *acc)
+{
+    const R rc2 = (R)p.rc2;
+    while (__ballot(word != 0ull))
+    {
+        const bool v1 = word != 0ull;
+        const int t1 = v1 ? __builtin_ctzll(word) : 0;
+        word &= word - 1ull;
+        const bool v2 = word != 0ull;
+        const int t2 = v2 ? __builtin_ctzll(word) : 0;
+        word &= word - 1ull;
+        R x1, y1, z1, f1, x2, y2, z2, f2;
+        T.load(s + (g0 + t1) * S, x1, y1, z1, f1);
+        T.load(s + (g0 + t2) * S, x2, y2, z2, f2);
+        const R dx1 = xi - x1, dy1 = yi - y1, dz1 = zi - z1;
+        const R dx2 = xi - x2, dy2 = yi - y2, dz2 = zi - z2;
+        R r21 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
+        R r22 = dx2 * dx2 + dy2 * dy2 + dz2 * dz2;
+        const bool u1 = v1 && r21 < rc2, u2 = v2 && r22 < rc2;
+        r21 = u1 ? r21 : rc2;
+        r22 = u2 ? r22 : rc2;
+        R fac1, en1, fac2, en2;
+        if (CAT == CAT_LJ) { lj_terms<R>(p, r21, fac1, en1); lj_terms<R>(p, r22, fac2, en2); }
+        else { morse_terms<R>(p, r21, fac1, en1); morse_terms<R>(p, r22, fac2, en2); }
+        acc[0].add(u1 ? fac1 * dx1 : (R)0);
+        acc[1].add(u1 ? fac1 * dy1 : (R)0);
+        acc[2].add(u1 ? fac1 * dz1 : (R)0);
+        acc[3].add(u1 ? en1 : (R)0);
+        acc[0].add(u2 ? fac2 * dx2 : (R)0);
+        acc[1].add(u2 ? fac2 * dy2 : (R)0);
+        acc[2].add(u2 ? fac2 * dz2 : (R)0);
+        acc[3].add(u2 ? en2 : (R)0);
     }
 }
 
@@ -659,6 +708,8 @@ static __device__ void b_cell(const KDesc &D, const int c)
                 }
                 if constexpr (CAT == CAT_EAM1)
                     flush_eam1<R, PASS>(D, T, word, s, g0, S, xi, yi, zi, Fpi, (R)p1.rc2, acc);
+                else if constexpr ((CAT == CAT_LJ || CAT == CAT_MORSE) && PASS == 0)
+                    flush_pair2<R, CAT>(T, word, s, g0, S, xi, yi, zi, p1, acc);
                 else
                     while (__ballot(word != 0ull))
                     {
