@@ -222,7 +222,7 @@ static __device__ __forceinline__ R sp_der(const Q &q, R b, R inv_h)
 
 /* FP64 keeps the library's exp and sqrt and true divisions.  FP32 (the mixed
    build, whose bar is 1e-5 of max|F|) uses the hardware's single instructions:
-   v_sqrt_f32 and v_rcp_f32 (about 1 ulp) and v_exp_f32 on x log2(e), whose
+   v_sqrt_f32, v_rsq_f32 and v_rcp_f32 (about 1 ulp) and v_exp_f32 on x log2(e), whose
    relative error grows as |x| eps, below 1e-6 for the Morse exponents met
    within a cutoff (|x| < 10) -- the library sequences are several times
    longer, and the Morse pass is bound by them. */
@@ -248,12 +248,26 @@ static __device__ __forceinline__ void lj_terms(const PairParam &p, R r2, R &fac
 template <typename R>
 static __device__ __forceinline__ void morse_terms(const PairParam &p, R r2, R &fac, R &en)
 {
-    const R r = fsqrt(r2);
-    const R e1 = fexp(-(R)p.p1 * (r - (R)p.p2));
-    const R e2 = e1 * e1;
-    fac = sizeof(R) == sizeof(double) ? (R)2 * (R)p.p1 * (R)p.p0 / r * (e2 - e1)
-                                      : (R)2 * (R)p.p1 * (R)p.p0 * frcp(r) * (e2 - e1);
-    en = (R)0.5 * (R)p.p0 * (e2 - (R)2 * e1);
+    if constexpr (sizeof(R) == sizeof(double))
+    {
+        const R r = fsqrt(r2);
+        const R e1 = fexp(-(R)p.p1 * (r - (R)p.p2));
+        const R e2 = e1 * e1;
+        fac = (R)2 * (R)p.p1 * (R)p.p0 / r * (e2 - e1);
+        en = (R)0.5 * (R)p.p0 * (e2 - (R)2 * e1);
+    }
+    else
+    {
+        /* one v_rsq_f32 gives both 1/r and r = r^2 / r; the exponent
+           -alpha (r - r0) log2(e) is one FMA with per-pair-type constants */
+        const R ir = __builtin_amdgcn_rsqf(r2);
+        const R r = r2 * ir;
+        const R ka = -(R)p.p1 * 1.44269504088896341f, kb = (R)p.p1 * (R)p.p2 * 1.44269504088896341f;
+        const R e1 = __builtin_amdgcn_exp2f(__builtin_fmaf(ka, r, kb));
+        const R e2 = e1 * e1;
+        fac = (R)2 * (R)p.p1 * (R)p.p0 * ir * (e2 - e1);
+        en = (R)0.5 * (R)p.p0 * (e2 - (R)2 * e1);
+    }
 }
 
 /* one candidate pair inside the search radius; acc = {fx, fy, fz, energy or density} */
