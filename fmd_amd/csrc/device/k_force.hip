@@ -364,15 +364,36 @@ static __device__ __forceinline__ float4 rec_at(__amdgpu_buffer_rsrc_t r, unsign
 }
 #define RP_NONE 0x80000000u   /* an offset beyond any table */
 
-/* Single-element EAM: the set bits of a hit word, two per iteration and
-   without branches, so that the coefficient gathers of both pairs are in
+/* Single-element EAM: the set bits of a hit word, NP per iteration and
+   without branches, so that the coefficient gathers of NP pairs are in
    flight together.  A slot without a pair (or outside the pair cutoff)
-   reads zeros (no memory access) and contributes an exact zero. */
+   reads zeros (no memory access) and contributes an exact zero.  Each
+   accumulator takes the pairs in bit order whatever NP is.  The passes wait
+   on the gathers' latency, so more pairs per iteration pay until the
+   registers cost a wave per SIMD: measured on the 4M-atom copper step, FP64
+   keeps 2 in both passes (3 in the force pass drops to 3 waves per SIMD and
+   is 8 % slower; 3 or 4 held at 4 waves, by re-reading the tile after the
+   gathers, gain nothing: the FP64 passes are bound by the gathers'
+   throughput), FP32 takes 3 in the force pass and 4 in the density pass. */
+#ifndef FMDX_EAM_NP64_0
+#define FMDX_EAM_NP64_0 2
+#endif
+#ifndef FMDX_EAM_NP64_1
+#define FMDX_EAM_NP64_1 2
+#endif
+#ifndef FMDX_EAM_NP32_0
+#define FMDX_EAM_NP32_0 3
+#endif
+#ifndef FMDX_EAM_NP32_1
+#define FMDX_EAM_NP32_1 4
+#endif
 template <typename R, int PASS, typename TileT, typename A>
 static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T, unsigned long long word, int s,
                                                   int g0, int S, R xi, R yi, R zi, R Fpi, R rc2, A *acc)
 {
     typedef typename Prec<R>::Q Q;
+    constexpr int NP = sizeof(R) == sizeof(double) ? (PASS == 0 ? FMDX_EAM_NP64_0 : FMDX_EAM_NP64_1)
+                                                   : (PASS == 0 ? FMDX_EAM_NP32_0 : FMDX_EAM_NP32_1);
     const EamDev &E = D.eam;
     const R inv = (R)E.inv_dr2;
     const int kmax = E.nr2 - 2;
@@ -380,53 +401,61 @@ static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T
     const __amdgpu_buffer_rsrc_t rb = rp_rsrc(Prec<R>::rp(E), (unsigned)(E.nr2 - 1) * 2u * rec);
     while (__ballot(word != 0ull))
     {
-        const bool v1 = word != 0ull;
-        const int t1 = v1 ? __builtin_ctzll(word) : 0;
-        word &= word - 1ull;
-        const bool v2 = word != 0ull;
-        const int t2 = v2 ? __builtin_ctzll(word) : 0;
-        word &= word - 1ull;
-        const int m1 = s + (g0 + t1) * S, m2 = s + (g0 + t2) * S;
-        R x1, y1, z1, f1, x2, y2, z2, f2;
-        T.load(m1, x1, y1, z1, f1);
-        T.load(m2, x2, y2, z2, f2);
-        const R dx1 = xi - x1, dy1 = yi - y1, dz1 = zi - z1;
-        const R dx2 = xi - x2, dy2 = yi - y2, dz2 = zi - z2;
-        R r21 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
-        R r22 = dx2 * dx2 + dy2 * dy2 + dz2 * dz2;
-        const bool u1 = v1 && r21 < rc2, u2 = v2 && r22 < rc2;
-        r21 = u1 ? r21 : (R)0;
-        r22 = u2 ? r22 : (R)0;
-        const R tt1 = r21 * inv, tt2 = r22 * inv;
-        const int k1 = min((int)tt1, kmax), k2 = min((int)tt2, kmax);
-        const R b1 = tt1 - (R)k1, b2 = tt2 - (R)k2;
+        R dx[NP], dy[NP], dz[NP], fj[NP], b[NP];
+        bool u[NP];
+        unsigned o[NP];
+#pragma unroll
+        for (int j = 0; j < NP; j++)
+        {
+            const bool v = word != 0ull;
+            const int t = v ? __builtin_ctzll(word) : 0;
+            word &= word - 1ull;
+            R x, y, z;
+            T.load(s + (g0 + t) * S, x, y, z, fj[j]);
+            dx[j] = xi - x;
+            dy[j] = yi - y;
+            dz[j] = zi - z;
+            R r2 = dx[j] * dx[j] + dy[j] * dy[j] + dz[j] * dz[j];
+            u[j] = v && r2 < rc2;
+            r2 = u[j] ? r2 : (R)0;
+            const R tt = r2 * inv;
+            const int k = min((int)tt, kmax);
+            b[j] = tt - (R)k;
+            o[j] = u[j] ? (unsigned)k * 2u * rec : RP_NONE;
+        }
         if (PASS == 1)
         {
-            const Q q1 = rec_at(rb, u1 ? (unsigned)k1 * 2u * rec : RP_NONE, (Q *)nullptr);
-            const Q q2 = rec_at(rb, u2 ? (unsigned)k2 * 2u * rec : RP_NONE, (Q *)nullptr);
-            const R c1 = sp_val<R, Q>(q1, b1), c2 = sp_val<R, Q>(q2, b2);
-            acc[3].add(u1 ? c1 : (R)0);
-            acc[3].add(u2 ? c2 : (R)0);
+            Q q[NP];
+#pragma unroll
+            for (int j = 0; j < NP; j++) q[j] = rec_at(rb, o[j], (Q *)nullptr);
+#pragma unroll
+            for (int j = 0; j < NP; j++)
+            {
+                const R c = sp_val<R, Q>(q[j], b[j]);
+                acc[3].add(u[j] ? c : (R)0);
+            }
         }
         else
         {
-            const unsigned o1 = u1 ? (unsigned)k1 * 2u * rec : RP_NONE;
-            const unsigned o2 = u2 ? (unsigned)k2 * 2u * rec : RP_NONE;
-            const Q qr1 = rec_at(rb, o1, (Q *)nullptr), qp1 = rec_at(rb, o1 + rec, (Q *)nullptr);
-            const Q qr2 = rec_at(rb, o2, (Q *)nullptr), qp2 = rec_at(rb, o2 + rec, (Q *)nullptr);
-            const R rho1 = sp_der<R, Q>(qr1, b1, inv), rho2 = sp_der<R, Q>(qr2, b2, inv);
-            const R ph1 = sp_der<R, Q>(qp1, b1, inv), ph2 = sp_der<R, Q>(qp2, b2, inv);
-            const R mag1 = (R)2 * (Fpi * rho1 + f1 * rho1 + ph1);
-            const R mag2 = (R)2 * (Fpi * rho2 + f2 * rho2 + ph2);
-            const R e1 = (R)0.5 * sp_val<R, Q>(qp1, b1), e2 = (R)0.5 * sp_val<R, Q>(qp2, b2);
-            acc[0].add(u1 ? -mag1 * dx1 : (R)0);
-            acc[1].add(u1 ? -mag1 * dy1 : (R)0);
-            acc[2].add(u1 ? -mag1 * dz1 : (R)0);
-            acc[3].add(u1 ? e1 : (R)0);
-            acc[0].add(u2 ? -mag2 * dx2 : (R)0);
-            acc[1].add(u2 ? -mag2 * dy2 : (R)0);
-            acc[2].add(u2 ? -mag2 * dz2 : (R)0);
-            acc[3].add(u2 ? e2 : (R)0);
+            Q qr[NP], qp[NP];
+#pragma unroll
+            for (int j = 0; j < NP; j++)
+            {
+                qr[j] = rec_at(rb, o[j], (Q *)nullptr);
+                qp[j] = rec_at(rb, o[j] + rec, (Q *)nullptr);
+            }
+#pragma unroll
+            for (int j = 0; j < NP; j++)
+            {
+                const R rho = sp_der<R, Q>(qr[j], b[j], inv);
+                const R ph = sp_der<R, Q>(qp[j], b[j], inv);
+                const R mag = (R)2 * (Fpi * rho + fj[j] * rho + ph);
+                const R e = (R)0.5 * sp_val<R, Q>(qp[j], b[j]);
+                acc[0].add(u[j] ? -mag * dx[j] : (R)0);
+                acc[1].add(u[j] ? -mag * dy[j] : (R)0);
+                acc[2].add(u[j] ? -mag * dz[j] : (R)0);
+                acc[3].add(u[j] ? e : (R)0);
+            }
         }
     }
 }
