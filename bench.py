@@ -47,92 +47,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def fmd_lib():
-    lib = F.load_fmd()
-    P = C.c_void_p
-    lib.fmd_create.restype = P
-    lib.fmd_free.argtypes = [P]
-    lib.fmd_box_setSize.argtypes = [P, C.c_double, C.c_double, C.c_double]
-    lib.fmd_box_setPBC.argtypes = [P, C.c_bool, C.c_bool, C.c_bool]
-    lib.fmd_matt_setAtomKinds.argtypes = [P, C.c_uint, P, P]
-    lib.fmd_pot_eam_alloy_load.restype = P
-    lib.fmd_pot_eam_alloy_load.argtypes = [P, C.c_char_p]
-    lib.fmd_pot_apply.argtypes = [P, C.c_uint, C.c_uint, P]
-    lib.fmd_ext_loadAtoms.argtypes = [P, C.c_uint, P, P, P, P]
-    lib.fmd_ext_distribute.restype = C.c_bool
-    lib.fmd_ext_distribute.argtypes = [P]
-    lib.fmd_ext_getDevice.restype = P
-    lib.fmd_ext_getDevice.argtypes = [P]
-    lib.fmd_ext_setPrecision.argtypes = [P, C.c_int]
-    return lib
-
-
-def make_system(cells, seed, setfl, precision):
-    """The EAM Cu workload on the device through the public API; returns (md, dev handle, natoms)."""
-    lib = fmd_lib()
-    a = W.CU["a"]
-    rng = np.random.default_rng(seed)
-    x = W.fcc_positions(cells, cells, cells, a)
-    x = x + rng.normal(0.0, 0.02, size=x.shape)
-    L = cells * a
-    x = np.mod(x, L)
-    v = W.maxwell_velocities(len(x), 63.546, 600.0, rng)
-    md = lib.fmd_create()
-    lib.fmd_box_setSize(md, L, L, L)
-    lib.fmd_box_setPBC(md, True, True, True)
-    names = (C.c_char_p * 1)(b"Cu")
-    mass = (C.c_double * 1)(63.546)
-    lib.fmd_matt_setAtomKinds(md, 1, C.cast(names, C.c_void_p), C.cast(mass, C.c_void_p))
-    pot = lib.fmd_pot_eam_alloy_load(md, setfl.encode())
-    if not pot:
-        raise RuntimeError("setfl load failed")
-    lib.fmd_pot_apply(md, 0, 0, pot)
-    x = np.ascontiguousarray(x)
-    v = np.ascontiguousarray(v)
-    lib.fmd_ext_loadAtoms(md, len(x), x.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p), None, None)
-    lib.fmd_ext_setPrecision(md, precision)
-    if not lib.fmd_ext_distribute(md):
-        raise RuntimeError("upload failed")
-    return lib, md, lib.fmd_ext_getDevice(md), len(x)
-
-
-def run_steps(fx, dev, nsteps, dt, tau, T0):
-    for _ in range(nsteps):
-        F._ck(fx.fmdx_vv_start(dev, -1, dt, 1, tau, T0), "vv_start")
-        F._ck(fx.fmdx_update_forces(dev, -1, 0), "update_forces")
-        F._ck(fx.fmdx_vv_finish(dev, -1, dt), "vv_finish")
-
-
-def gpu_bench(cells, steps, warmup, precision, setfl, seed=1):
-    fx = F.load_fmdx()
-    lib, md, dev, n = make_system(cells, seed, setfl, precision)
-    dt, tau, T0 = 2e-3, 0.1, 600.0
-    F._ck(fx.fmdx_update_forces(dev, -1, 0), "update_forces")
-    F._ck(fx.fmdx_thermo(dev, -1), "thermo")
-    run_steps(fx, dev, warmup, dt, tau, T0)
-    th = F.Thermo()
-    F._ck(fx.fmdx_read_thermo(dev, C.byref(th)), "read_thermo")
-    F._ck(fx.fmdx_profile_enable(dev, 1), "profile")
-    F._ck(fx.fmdx_event_record(dev, 0), "event")
-    t0 = time.perf_counter()
-    run_steps(fx, dev, steps, dt, tau, T0)
-    F._ck(fx.fmdx_event_record(dev, 1), "event")
-    F._ck(fx.fmdx_sync(dev), "sync")
-    wall = time.perf_counter() - t0
-    ms = C.c_float(0)
-    F._ck(fx.fmdx_event_elapsed_ms(dev, 0, 1, C.byref(ms)), "elapsed")
-    prof = (C.c_double * 8)()
-    cnt = C.c_uint(0)
-    F._ck(fx.fmdx_profile_read(dev, prof, C.byref(cnt)), "profile_read")
-    F._ck(fx.fmdx_profile_enable(dev, 0), "profile")
-    F._ck(fx.fmdx_read_thermo(dev, C.byref(th)), "read_thermo")
-    if th.err:
-        raise RuntimeError("device error word %d" % th.err)
-    res = dict(natoms=n, steps=steps, wall_s=wall, gpu_ms=ms.value,
-               pass1_ms=prof[0] / max(cnt.value, 1), pass0_ms=prof[1] / max(cnt.value, 1),
-               T=th.temperature, pe=th.pe, cap=th.cap, max_cell=th.max_cell_count)
-    lib.fmd_free(md)
-    return res
+fmd_lib = W.fmd_lib
+make_system = W.eam_cu_system
+run_steps = W.vv_berendsen_steps
+gpu_bench = W.time_eam_cu_steps
 
 
 def host_threads():

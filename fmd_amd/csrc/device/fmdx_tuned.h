@@ -1,0 +1,41 @@
+/* Compile-time shape of the pair kernels (k_force.hip), as chosen by
+ * fmd_amd/autotune.py on one MI355X for the 4,000,000-atom EAM Cu step
+ * (FCC 100^3, synthetic setfl, rc 5.5 A).  Each value can be overridden
+ * with -D; the tuner builds its candidates that way.
+ *
+ *   FMDX_TILE64 / FMDX_TILE32   candidates per LDS tile, FP64 / FP32 kernels
+ *   FMDX_EAM_NP64_0 / _1        pairs per flush iteration, FP64 force / density pass
+ *   FMDX_EAM_NP32_0 / _1        the same for the FP32 (mixed) passes
+ *
+ * Fixed by design, not tuned: the cell size (the binning grid, cellh >= rc,
+ * whose exact atom order the kernels share) and one wave per home cell (its
+ * lanes split over (atom, stripe) pairs for any cell population).
+ * Last tuning run: macro, choice, then step time per candidate:
+ *   FMDX_TILE64        224  192: 3.986 ms  224: 3.899 ms  256: 4.053 ms
+ *   FMDX_EAM_NP64_0      2  2: 3.893 ms  3: 4.083 ms
+ *   FMDX_EAM_NP64_1      2  2: 3.897 ms  3: 3.902 ms
+ *   FMDX_TILE32        384  320: 3.009 ms  384: 2.941 ms  448: 2.942 ms
+ *   FMDX_EAM_NP32_0      3  2: 3.004 ms  3: 2.943 ms  4: 2.967 ms
+ *   FMDX_EAM_NP32_1      4  2: 2.983 ms  4: 2.947 ms  6: 2.936 ms
+ */
+#ifndef FMDX_TUNED_H
+#define FMDX_TUNED_H
+#ifndef FMDX_TILE64
+#define FMDX_TILE64 224
+#endif
+#ifndef FMDX_EAM_NP64_0
+#define FMDX_EAM_NP64_0 2
+#endif
+#ifndef FMDX_EAM_NP64_1
+#define FMDX_EAM_NP64_1 2
+#endif
+#ifndef FMDX_TILE32
+#define FMDX_TILE32 384
+#endif
+#ifndef FMDX_EAM_NP32_0
+#define FMDX_EAM_NP32_0 3
+#endif
+#ifndef FMDX_EAM_NP32_1
+#define FMDX_EAM_NP32_1 4
+#endif
+#endif

@@ -60,13 +60,8 @@ enum
    with less LDS: the FP64 force pass needs 9.7 KB, which lets 16 workgroups
    share a CU, as many as its registers allow.  Measured on the 4M-atom EAM
    step (FP64 / mixed): 224/384 4.27/3.33 ms, 240/416 4.34/3.34, 256/512
-   4.39/3.41. */
-#ifndef FMDX_TILE64
-#define FMDX_TILE64 224
-#endif
-#ifndef FMDX_TILE32
-#define FMDX_TILE32 384
-#endif
+   4.39/3.41.  The values in use come from fmdx_tuned.h. */
+#include "fmdx_tuned.h"
 template <typename R> struct Prec;
 template <> struct Prec<double>
 {
@@ -374,19 +369,8 @@ static __device__ __forceinline__ float4 rec_at(__amdgpu_buffer_rsrc_t r, unsign
    keeps 2 in both passes (3 in the force pass drops to 3 waves per SIMD and
    is 8 % slower; 3 or 4 held at 4 waves, by re-reading the tile after the
    gathers, gain nothing: the FP64 passes are bound by the gathers'
-   throughput), FP32 takes 3 in the force pass and 4 in the density pass. */
-#ifndef FMDX_EAM_NP64_0
-#define FMDX_EAM_NP64_0 2
-#endif
-#ifndef FMDX_EAM_NP64_1
-#define FMDX_EAM_NP64_1 2
-#endif
-#ifndef FMDX_EAM_NP32_0
-#define FMDX_EAM_NP32_0 3
-#endif
-#ifndef FMDX_EAM_NP32_1
-#define FMDX_EAM_NP32_1 4
-#endif
+   throughput), FP32 takes 3 in the force pass and 4 in the density pass
+   (FMDX_EAM_NP*, fmdx_tuned.h). */
 template <typename R, int PASS, typename TileT, typename A>
 static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T, unsigned long long word, int s,
                                                   int g0, int S, R xi, R yi, R zi, R Fpi, R rc2, A *acc)

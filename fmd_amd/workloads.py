@@ -10,10 +10,14 @@
       F(rho) = -F0 * sqrt(rho / rho_e)
   with rho_e the FCC equilibrium density at a = 3.615 A inside the cutoff.
 """
+import ctypes as C
 import math
 import os
+import time
 
 import numpy as np
+
+import fmd_amd as F
 
 MASS_UNIT = 9.6485332907e+03
 KB = 8.6173303e-5
@@ -84,3 +88,97 @@ def write_setfl(path, nr=5000, nrho=5000, p=CU):
 
 def data_dir():
     return os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests", "golden", "data")
+
+
+# --- the EAM Cu step through the public API (used by bench.py and fmd_amd.autotune) ---
+
+def fmd_lib():
+    """libfmd.so with the argument types of the calls below."""
+    lib = F.load_fmd()
+    P = C.c_void_p
+    lib.fmd_create.restype = P
+    lib.fmd_free.argtypes = [P]
+    lib.fmd_box_setSize.argtypes = [P, C.c_double, C.c_double, C.c_double]
+    lib.fmd_box_setPBC.argtypes = [P, C.c_bool, C.c_bool, C.c_bool]
+    lib.fmd_matt_setAtomKinds.argtypes = [P, C.c_uint, P, P]
+    lib.fmd_pot_eam_alloy_load.restype = P
+    lib.fmd_pot_eam_alloy_load.argtypes = [P, C.c_char_p]
+    lib.fmd_pot_apply.argtypes = [P, C.c_uint, C.c_uint, P]
+    lib.fmd_ext_loadAtoms.argtypes = [P, C.c_uint, P, P, P, P]
+    lib.fmd_ext_distribute.restype = C.c_bool
+    lib.fmd_ext_distribute.argtypes = [P]
+    lib.fmd_ext_getDevice.restype = P
+    lib.fmd_ext_getDevice.argtypes = [P]
+    lib.fmd_ext_setPrecision.argtypes = [P, C.c_int]
+    return lib
+
+
+def eam_cu_system(cells, seed, setfl, precision):
+    """The EAM Cu workload (FCC cells^3, a = 3.615 A, jitter 0.02 A, 600 K, periodic) on the device
+    through the public API; returns (libfmd, fmd_t handle, device handle, natoms)."""
+    lib = fmd_lib()
+    a = CU["a"]
+    rng = np.random.default_rng(seed)
+    x = fcc_positions(cells, cells, cells, a)
+    x = x + rng.normal(0.0, 0.02, size=x.shape)
+    L = cells * a
+    x = np.mod(x, L)
+    v = maxwell_velocities(len(x), 63.546, 600.0, rng)
+    md = lib.fmd_create()
+    lib.fmd_box_setSize(md, L, L, L)
+    lib.fmd_box_setPBC(md, True, True, True)
+    names = (C.c_char_p * 1)(b"Cu")
+    mass = (C.c_double * 1)(63.546)
+    lib.fmd_matt_setAtomKinds(md, 1, C.cast(names, C.c_void_p), C.cast(mass, C.c_void_p))
+    pot = lib.fmd_pot_eam_alloy_load(md, setfl.encode())
+    if not pot:
+        raise RuntimeError("setfl load failed")
+    lib.fmd_pot_apply(md, 0, 0, pot)
+    x = np.ascontiguousarray(x)
+    v = np.ascontiguousarray(v)
+    lib.fmd_ext_loadAtoms(md, len(x), x.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p), None, None)
+    lib.fmd_ext_setPrecision(md, precision)
+    if not lib.fmd_ext_distribute(md):
+        raise RuntimeError("upload failed")
+    return lib, md, lib.fmd_ext_getDevice(md), len(x)
+
+
+def vv_berendsen_steps(fx, dev, nsteps, dt, tau, T0):
+    """nsteps velocity-Verlet steps with Berendsen rescaling through the C-ABI seams."""
+    for _ in range(nsteps):
+        F._ck(fx.fmdx_vv_start(dev, -1, dt, 1, tau, T0), "vv_start")
+        F._ck(fx.fmdx_update_forces(dev, -1, 0), "update_forces")
+        F._ck(fx.fmdx_vv_finish(dev, -1, dt), "vv_finish")
+
+
+def time_eam_cu_steps(cells, steps, warmup, precision, setfl, seed=1):
+    """HIP-event time of `steps` EAM Cu steps after `warmup`, with the two force passes' own times."""
+    fx = F.load_fmdx()
+    lib, md, dev, n = eam_cu_system(cells, seed, setfl, precision)
+    dt, tau, T0 = 2e-3, 0.1, 600.0
+    F._ck(fx.fmdx_update_forces(dev, -1, 0), "update_forces")
+    F._ck(fx.fmdx_thermo(dev, -1), "thermo")
+    vv_berendsen_steps(fx, dev, warmup, dt, tau, T0)
+    th = F.Thermo()
+    F._ck(fx.fmdx_read_thermo(dev, C.byref(th)), "read_thermo")
+    F._ck(fx.fmdx_profile_enable(dev, 1), "profile")
+    F._ck(fx.fmdx_event_record(dev, 0), "event")
+    t0 = time.perf_counter()
+    vv_berendsen_steps(fx, dev, steps, dt, tau, T0)
+    F._ck(fx.fmdx_event_record(dev, 1), "event")
+    F._ck(fx.fmdx_sync(dev), "sync")
+    wall = time.perf_counter() - t0
+    ms = C.c_float(0)
+    F._ck(fx.fmdx_event_elapsed_ms(dev, 0, 1, C.byref(ms)), "elapsed")
+    prof = (C.c_double * 8)()
+    cnt = C.c_uint(0)
+    F._ck(fx.fmdx_profile_read(dev, prof, C.byref(cnt)), "profile_read")
+    F._ck(fx.fmdx_profile_enable(dev, 0), "profile")
+    F._ck(fx.fmdx_read_thermo(dev, C.byref(th)), "read_thermo")
+    if th.err:
+        raise RuntimeError("device error word %d" % th.err)
+    res = dict(natoms=n, steps=steps, wall_s=wall, gpu_ms=ms.value,
+               pass1_ms=prof[0] / max(cnt.value, 1), pass0_ms=prof[1] / max(cnt.value, 1),
+               T=th.temperature, pe=th.pe, cap=th.cap, max_cell=th.max_cell_count)
+    lib.fmd_free(md)
+    return res

@@ -1,0 +1,44 @@
+"""Host side of the build-time tuner: the tuned header it reads and rewrites, and
+the build that compiles it in (no GPU needed)."""
+import os
+import shutil
+
+from fmd_amd import autotune as A
+from fmd_amd import build as B
+
+
+def test_tuned_header_defines_every_parameter():
+    vals = A.read_tuned()
+    assert set(vals) == {m for m, _, _ in A.PARAMS}
+    for m, cands, _ in A.PARAMS:
+        assert vals[m] > 0
+        assert len(cands) >= 2
+
+
+def test_write_then_read_round_trip(tmp_path):
+    p = str(tmp_path / "fmdx_tuned.h")
+    shutil.copy(A.TUNED_H, p)
+    vals = A.read_tuned(p)
+    new = dict(vals, FMDX_TILE64=256, FMDX_EAM_NP32_1=6)
+    A.write_tuned(new, [("FMDX_TILE64", 256, [(192, 4.1), (224, 3.9), (256, 3.8)])], p)
+    assert A.read_tuned(p) == new
+    text = open(p).read()
+    assert "256: 3.800 ms" in text
+    # a second run replaces the table instead of stacking it
+    A.write_tuned(vals, [("FMDX_TILE32", 384, [(320, 3.0), (384, 2.9)])], p)
+    assert A.read_tuned(p) == vals
+    text = open(p).read()
+    assert text.count("Last tuning run") == 1 and "256: 3.800 ms" not in text
+
+
+def test_force_kernels_take_the_tuned_header():
+    src = open(os.path.join(B.DEV_SRC, A.TUNED_FILE)).read()
+    assert '#include "fmdx_tuned.h"' in src
+    for m, _, _ in A.PARAMS:
+        assert m in src, m
+
+
+def test_builder_key_names_every_parameter():
+    key = A.Builder.key(A.read_tuned())
+    assert key.count("_") >= len(A.PARAMS) - 1
+    assert "tile64" in key and "np32_1" in key
