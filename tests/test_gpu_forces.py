@@ -154,6 +154,17 @@ def test_mixed_precision_lj(rng):
     _compare(o, d, 1e-5, 1e-6)
 
 
+def test_mixed_precision_morse(rng):
+    # the FP32 Morse pass: r and 1/r from one reciprocal square root, the
+    # exponential from the hardware's base-2 instruction; pairs from the 1st to
+    # the 7th neighbour shell inside rc = 7 A, open along z
+    x = lattice_fcc((8, 8, 8), 3.615, rng, jitter=0.1)
+    L = [8 * 3.615, 8 * 3.615, 8 * 3.615 + 10.0]
+    o, d = build(L, [1, 1, 0], [63.546], ["Cu"], [("morse", 0, 0, 0.3429, 1.3588, 2.866, 7.0)], x,
+                 precision=F.FMDX_PREC_MIXED)
+    _compare(o, d, 1e-5, 1e-6)
+
+
 @pytest.mark.parametrize("precision", [F.FMDX_PREC_FP64, F.FMDX_PREC_MIXED])
 def test_degenerate_boxes_1_and_2_cells(rng, precision):
     # 1, 2 and 3 cells along x, y, z: the stencil meets the same cell through
