@@ -61,7 +61,8 @@ def test_eam_synthetic_setfl(rng, synth_setfl):
     _compare(o, d, 1e-10, 1e-10)
 
 
-def test_hybrid_eam_lj_two_kinds_with_group(rng, cu01):
+@pytest.mark.parametrize("precision", [F.FMDX_PREC_FP64, F.FMDX_PREC_MIXED])
+def test_hybrid_eam_lj_two_kinds_with_group(rng, cu01, precision):
     # Cu (EAM) and Ar (LJ Ar-Ar, LJ Cu-Ar) as in examples/02_collision.c
     xcu = lattice_fcc((5, 5, 5), 3.6316, rng, jitter=0.03, origin=(2.0, 2.0, 2.0))
     xar = lattice_fcc((4, 4, 4), 5.26, rng, jitter=0.05, origin=(22.0, 2.0, 2.0))
@@ -70,10 +71,12 @@ def test_hybrid_eam_lj_two_kinds_with_group(rng, cu01):
     group = np.r_[np.zeros(len(xcu)), np.ones(len(xar))].astype(np.int32)
     L = [48.0, 30.0, 30.0]
     pots = [("eam", cu01, [(0, 0)]), ("lj", 1, 1, 3.40, 0.0104, 2.5 * 3.40), ("lj", 0, 1, 2.87, 0.0652, 2.5 * 2.87)]
-    o, d = build(L, [1, 1, 1], [63.546, 39.948], ["Cu", "Ar"], pots, x, group=group, kind=kind)
-    _compare(o, d, 1e-10, 1e-10)
-    _compare(o, d, 1e-10, 1e-10, ag=0)
-    _compare(o, d, 1e-10, 1e-10, ag=1)
+    o, d = build(L, [1, 1, 1], [63.546, 39.948], ["Cu", "Ar"], pots, x, group=group, kind=kind,
+                 precision=precision)
+    tol = (1e-10, 1e-10) if precision == F.FMDX_PREC_FP64 else (1e-5, 1e-6)
+    _compare(o, d, *tol)
+    _compare(o, d, *tol, ag=0)
+    _compare(o, d, *tol, ag=1)
 
 
 @pytest.mark.parametrize("precision", [F.FMDX_PREC_FP64, F.FMDX_PREC_MIXED])
