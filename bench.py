@@ -277,8 +277,27 @@ def nve_drift(cells, steps, precision, setfl, seed=1, dt=1e-3, equil_steps=1000,
     jittered 600 K lattice is first thermalised (Berendsen, tau 0.1 ps, `equil_steps` steps) so that
     the equipartition transient is over, then E = PE + KE is sampled every 10 NVE steps and the drift
     is the least-squares slope over the whole window; the spread of E about that line is reported."""
-    fx = F.load_fmdx()
     lib, md, dev, n = make_system(cells, seed, setfl, precision)
+    try:
+        return drift_on(dev, n, steps, dt, equil_steps, T0)
+    finally:
+        lib.fmd_free(md)
+
+
+def pair_drift(pot, cells, steps, precision, T0, seed=5, dt=1e-3, equil_steps=1000):
+    """The same measurement on a periodic LJ-argon or Morse-copper FCC crystal of cells^3 unit
+    cells, the systems of the batched ensemble (rc 2.5 sigma and 7 A, neither shifted at rc)."""
+    from fmd_amd import ensemble as E
+    job = E.ensemble_job(dict(pot=pot, cells=cells, T=T0, seed=seed, index=0), precision, dt=dt)
+    try:
+        return drift_on(job.dev, job.natoms, steps, dt, equil_steps, T0)
+    finally:
+        job.free()
+
+
+def drift_on(dev, n, steps, dt, equil_steps, T0):
+    """Berendsen to T0 for equil_steps, then NVE for `steps`: least-squares slope of (PE + KE) / n."""
+    fx = F.load_fmdx()
     F._ck(fx.fmdx_update_forces(dev, -1, 0), "forces")
     F._ck(fx.fmdx_thermo(dev, -1), "thermo")   # the first Berendsen factor needs the temperature
     for _ in range(equil_steps):
@@ -298,7 +317,6 @@ def nve_drift(cells, steps, precision, setfl, seed=1, dt=1e-3, equil_steps=1000,
             es.append(th.pe + th.ke)
     T_end = th.temperature
     err = th.err
-    lib.fmd_free(md)
     if err:
         raise RuntimeError("NVE drift run: device error word %d" % err)
     t = np.arange(len(es)) * 10 * dt * 1e-3   # ns
@@ -408,11 +426,18 @@ def main():
                                                  args.drift_steps, pv, cu01)
                 drift["synthetic_" + pname] = guarded("NVE drift synthetic " + pname, nve_drift,
                                                       min(args.cells, 40), args.drift_steps, pv, setfl)
-            if "error" not in drift["cu01_mixed"] and "error" not in drift["cu01_fp64"]:
-                drift["cu01_mixed_minus_fp64_eV_per_atom_per_ns"] = (
-                    drift["cu01_mixed"]["drift_eV_per_atom_per_ns"] - drift["cu01_fp64"]["drift_eV_per_atom_per_ns"])
+                drift["lj_ar_" + pname] = guarded("NVE drift LJ Ar " + pname, pair_drift, "lj", 20,
+                                                  args.drift_steps, pv, 100.0)
+                drift["morse_cu_" + pname] = guarded("NVE drift Morse Cu " + pname, pair_drift, "morse", 20,
+                                                     args.drift_steps, pv, 300.0)
+            for key in ("cu01", "synthetic", "lj_ar", "morse_cu"):
+                a, b = drift[key + "_mixed"], drift[key + "_fp64"]
+                if "error" not in a and "error" not in b:
+                    drift[key + "_mixed_minus_fp64_eV_per_atom_per_ns"] = (
+                        a["drift_eV_per_atom_per_ns"] - b["drift_eV_per_atom_per_ns"])
             drift["note"] = ("Cu01 is the smooth Mishin setfl; the synthetic table is truncated at 5.5 A "
-                             "without a taper, so its drift measures the cutoff jumps")
+                             "without a taper, and LJ (rc 2.5 sigma, 100 K) and Morse (rc 7 A, 300 K) are "
+                             "not shifted at rc, so their drifts include the cutoff jumps")
             log("bench: NVE drift %s" % drift)
         if not args.no_ttm:
             ttm = guarded("TTM slab", ttm_slab, args.ttm_steps, F.FMDX_PREC_MIXED, setfl)
