@@ -127,6 +127,48 @@ def cpu_baseline(setfl, cells=32, steps=20, seed=1, threads=1):
     return dict(value=threads * len(x) * steps / dt, natoms=len(x), steps=steps, seconds=dt, threads=threads)
 
 
+def mixed_force_error(f, restore):
+    """Max and RMS of |F_mixed - F_fp64| over all force components, each over max|F_fp64|, and the
+    fraction of atoms whose largest component error exceeds 1e-5 of max|F_fp64|, on the current
+    state of the distributed system `f` (fmd_amd.api.Fmd); leaves it at precision `restore`."""
+    fx = F.load_fmdx()
+    dev = C.c_void_p(f.fmd_ext_getDevice())
+    f.fmd_ext_setPrecision(F.FMDX_PREC_FP64)
+    F._ck(fx.fmdx_update_forces(dev, -1, 0), "forces")
+    f64 = f.atoms()["f"]
+    f.fmd_ext_setPrecision(F.FMDX_PREC_MIXED)
+    F._ck(fx.fmdx_update_forces(dev, -1, 0), "forces")
+    d = f.atoms()["f"] - f64
+    f.fmd_ext_setPrecision(restore)
+    F._ck(fx.fmdx_update_forces(dev, -1, 0), "forces")
+    scale = max(float(np.max(np.abs(f64))), 1e-300)
+    above = float(np.mean(np.max(np.abs(d), axis=1) > 1e-5 * scale))
+    return float(np.max(np.abs(d))) / scale, float(np.sqrt(np.mean(d * d))) / scale, above
+
+
+def eam_force_error(cells, setfl, seed=1):
+    """Mixed against FP64 forces on the EAM Cu benchmark state (the same recipe as the timed run)."""
+    from fmd_amd import api
+    a = W.CU["a"]
+    rng = np.random.default_rng(seed)
+    x = W.fcc_positions(cells, cells, cells, a)
+    x = np.mod(x + rng.normal(0.0, 0.02, size=x.shape), cells * a)
+    v = W.maxwell_velocities(len(x), 63.546, 600.0, rng)
+    f = api.Fmd()
+    try:
+        f.fmd_box_setSize(cells * a, cells * a, cells * a)
+        f.fmd_box_setPBC(True, True, True)
+        f.set_kinds(["Cu"], [63.546])
+        f.fmd_pot_apply(0, 0, f.fmd_pot_eam_alloy_load(setfl.encode()))
+        f.load_atoms(x, v)
+        if not f.fmd_ext_distribute():
+            raise RuntimeError("distribute failed")
+        emax, erms, above = mixed_force_error(f, F.FMDX_PREC_FP64)
+    finally:
+        f.free()
+    return dict(natoms=len(x), max_rel=emax, rms_rel=erms, frac_atoms_over_1em5=above)
+
+
 def batched_ensemble(nsys, steps, plain_sample, seed=3):
     """The batched-ensemble workload: nsys periodic systems (LJ Ar or Morse Cu, log-uniform
     500-50000 atoms, 50-800 K), all advanced together in mixed precision by the batch
@@ -149,7 +191,7 @@ def batched_ensemble(nsys, steps, plain_sample, seed=3):
         j.free()
     # plain FP64, one system at a time, and the mixed-vs-FP64 force error of the same systems
     sample = specs[:: max(1, nsys // plain_sample)]
-    per_step, ferr = [], []
+    per_step, ferr, frms = [], [], []
     for sp in sample:
         j = E.ensemble_job(sp, F.FMDX_PREC_FP64)
         f = j.f
@@ -157,12 +199,9 @@ def batched_ensemble(nsys, steps, plain_sample, seed=3):
         t1 = time.perf_counter()
         f.fmd_dync_integrate(-1, (steps - 0.5) * j.dt, j.dt)
         per_step.append((time.perf_counter() - t1) / steps)
-        a64 = f.atoms()
-        f.fmd_ext_setPrecision(F.FMDX_PREC_MIXED)
-        F._ck(F.load_fmdx().fmdx_update_forces(C.c_void_p(j.dev), -1, 0), "forces")
-        a32 = f.atoms()
-        scale = max(np.max(np.abs(a64["f"])), 1e-300)
-        ferr.append(float(np.max(np.abs(a32["f"] - a64["f"])) / scale))
+        emax, erms, _ = mixed_force_error(f, F.FMDX_PREC_FP64)
+        ferr.append(emax)
+        frms.append(erms)
         j.free()
     t_step = t / steps
     jobs_s = nsys / (2000 * t_step)
@@ -174,7 +213,8 @@ def batched_ensemble(nsys, steps, plain_sample, seed=3):
                 plain_fp64_jobs_per_s=plain_job_s, speedup_vs_plain=jobs_s / plain_job_s,
                 plain_sample="%d of %d systems, %d timed steps each through fmd_dync_integrate"
                              % (len(sample), nsys, steps),
-                max_rel_force_err_mixed_vs_fp64=max(ferr), members_with_errors=errs)
+                max_rel_force_err_mixed_vs_fp64=max(ferr), rms_rel_force_err_mixed_vs_fp64=max(frms),
+                members_with_errors=errs)
 
 
 def served_collisions(njobs, rate, seed=4):
@@ -187,24 +227,27 @@ def served_collisions(njobs, rate, seed=4):
     served = S.Server(E.collision_job, precision=F.FMDX_PREC_MIXED).run(specs)
     plain = S.run_plain(specs, E.collision_job)
     a, p = S.summarize(served), S.summarize(plain)
+    ferr = []
+    for sp in specs[:4]:   # mixed against FP64 forces on the first jobs' initial states
+        j = E.collision_job(sp, F.FMDX_PREC_FP64)
+        try:
+            ferr.append(mixed_force_error(j.f, F.FMDX_PREC_FP64))
+        finally:
+            j.free()
     return dict(workload="%d jobs, Poisson arrivals at %g jobs/s; each two Morse-Cu FCC spheres R=15-50 A, "
                          "20 A gap, 10-50 A/ps, impact parameter 0-(R1+R2), vacuum box, 1000-4000 steps of 1 fs"
                          % (njobs, rate),
                 served_mixed=a, plain_fp64=p,
                 jobs_per_s_ratio=a["jobs_per_s"] / p["jobs_per_s"],
-                p99_ratio_plain_over_served=p["p99_s"] / a["p99_s"])
+                p99_ratio_plain_over_served=p["p99_s"] / a["p99_s"],
+                max_rel_force_err_mixed_vs_fp64=max(e[0] for e in ferr),
+                rms_rel_force_err_mixed_vs_fp64=max(e[1] for e in ferr),
+                force_err_sample="initial states of the first %d jobs" % len(ferr))
 
 
-def ttm_slab(steps, precision, setfl, seed=5, pad=320.0, pulse_to_ps=3.0):
-    """The TTM-MD workload: EAM Cu slab of 40 x 40 x 400 FCC cells (2,560,000 atoms), periodic in
-    x and y, free surfaces in z with `pad` A of vacuum on each side; a 4 x 4 x 200 TTM turi with
-    Ce = gamma Te (96.6 J/m^3/K^2), kappa = kappa0 Te / Tl (401 W/m/K), G = 1e17 W/m^3/K and the
-    Gaussian source (absorbed fluence 100 J/m^2, 15 nm depth, t0 = 2 ps, FWHM 0.8326 ps, i.e.
-    exp(-((t - 2 ps) / 0.5 ps)^2)); 50 TTM sub-steps per 1 fs MD step; Te = Tl = 300 K at start:
-    the lattice gets 600 K velocities and is brought to 300 K by 0.4 ps of Berendsen dynamics
-    before the turi exists.  The vacuum keeps the explicit TTM step inside its stability limit
-    (kappa0 / (gamma Tl) grows as the lattice cools).  Runs through the public API
-    (fmd_dync_integrate: symplectic Euler with the TTM turi)."""
+def slab_system(setfl, precision, seed, pad):
+    """The 40 x 40 x 400 Cu slab of the TTM workload with the EAM table `setfl`, distributed and
+    brought from 600 K to 300 K by 0.4 ps of Berendsen dynamics; returns (fmd_amd.api.Fmd, box)."""
     from fmd_amd import api
     a = W.CU["a"]
     n = (40, 40, 400)
@@ -223,8 +266,31 @@ def ttm_slab(steps, precision, setfl, seed=5, pad=320.0, pulse_to_ps=3.0):
     f.load_atoms(x, v)
     f.fmd_ext_setPrecision(precision)
     if not f.fmd_ext_distribute():
+        f.free()
         raise RuntimeError("distribute failed")
     f.fmd_dync_equilibrate(-1, 0.4, 2e-3, 0.05, 300.0)
+    return f, L
+
+
+def ttm_slab(steps, precision, setfl, seed=5, pad=320.0, pulse_to_ps=3.0):
+    """The TTM-MD workload: EAM Cu slab of 40 x 40 x 400 FCC cells (2,560,000 atoms), periodic in
+    x and y, free surfaces in z with `pad` A of vacuum on each side; a 4 x 4 x 200 TTM turi with
+    Ce = gamma Te (96.6 J/m^3/K^2), kappa = kappa0 Te / Tl (401 W/m/K), G = 1e17 W/m^3/K and the
+    Gaussian source (absorbed fluence 100 J/m^2, 15 nm depth, t0 = 2 ps, FWHM 0.8326 ps, i.e.
+    exp(-((t - 2 ps) / 0.5 ps)^2)); 50 TTM sub-steps per 1 fs MD step; Te = Tl = 300 K at start:
+    the lattice gets 600 K velocities and is brought to 300 K by 0.4 ps of Berendsen dynamics
+    before the turi exists.  The vacuum keeps the explicit TTM step inside its stability limit
+    (kappa0 / (gamma Tl) grows as the lattice cools).  Runs through the public API
+    (fmd_dync_integrate: symplectic Euler with the TTM turi)."""
+    from fmd_amd import api
+    f, L = slab_system(setfl, precision, seed, pad)
+    ferr = mixed_force_error(f, precision)   # on the thermalised slab, before the turi exists
+    cu01 = os.path.join(ROOT, "tests", "golden", "data", "Cu01.eam.alloy")
+    g, _ = slab_system(cu01, precision, seed, pad)
+    try:
+        ferr_cu01 = mixed_force_error(g, precision)
+    finally:
+        g.free()
     T_start = f.fmd_matt_getTemperature()
     dims = (4, 4, 200)
     turi = f.fmd_turi_add(api.TURI_TTM_TYPE1, *dims, 0.0, -1.0)
@@ -248,7 +314,7 @@ def ttm_slab(steps, precision, setfl, seed=5, pad=320.0, pulse_to_ps=3.0):
     act = te >= 0
     tl_min = float(np.min(ti[act])) if np.any(act) else float("nan")
     te_max, n_active = float(np.max(te)), int(np.sum(act))
-    natoms = len(x)
+    natoms = 4 * 40 * 40 * 400
     # untimed: carry on through the pulse (peak at 2 ps) and sample the electrons and the lattice
     pulse = dict(t_ps=[], max_Te=[], mean_Te_active=[], lattice_T=[])
     while pulse_to_ps and f.fmd_dync_getTime() < pulse_to_ps - 0.5 * dt:
@@ -269,7 +335,13 @@ def ttm_slab(steps, precision, setfl, seed=5, pad=320.0, pulse_to_ps=3.0):
                 ms_per_step=wall / steps * 1e3, atom_steps_per_s=natoms * steps / wall,
                 cfl_number_at_300K=cfl, cfl_number_coldest_cell=cfl * 300.0 / tl_min,
                 lattice_T_start=T_start, lattice_T_after=T, max_Te_after=te_max,
-                active_ttm_cells=n_active, through_the_pulse=pulse if pulse["t_ps"] else None)
+                active_ttm_cells=n_active, through_the_pulse=pulse if pulse["t_ps"] else None,
+                max_rel_force_err_mixed_vs_fp64=ferr[0], rms_rel_force_err_mixed_vs_fp64=ferr[1],
+                frac_atoms_over_1em5_mixed_vs_fp64=ferr[2],
+                force_err_cu01_slab=dict(max_rel=ferr_cu01[0], rms_rel=ferr_cu01[1], frac_atoms_over_1em5=ferr_cu01[2]),
+                force_err_note="the synthetic setfl is untapered at rc: after thermalisation some pairs sit on "
+                               "its last r^2 interval, whose slope is extreme, or within FP32 rounding of rc; "
+                               "force_err_cu01_slab is the same thermalised slab with the smooth Cu01 table")
 
 
 def nve_drift(cells, steps, precision, setfl, seed=1, dt=1e-3, equil_steps=1000, T0=600.0):
@@ -419,6 +491,10 @@ def main():
             log("bench: other precision %s" % other)
         ens = srv = ttm = None
         drift = {}
+        ferr = None
+        if not args.no_mixed:
+            ferr = guarded("mixed force error", eam_force_error, args.cells, setfl)
+            log("bench: mixed force error %s" % ferr)
         if not args.no_drift:
             cu01 = os.path.join(ROOT, "tests", "golden", "data", "Cu01.eam.alloy")
             for pname, pv in (("fp64", F.FMDX_PREC_FP64), ("mixed", F.FMDX_PREC_MIXED)):
@@ -494,6 +570,8 @@ def main():
                                   "value": other["natoms"] * other["steps"] / (other["gpu_ms"] * 1e-3),
                                   "ms_per_step": other["gpu_ms"] / other["steps"],
                                   "pass0_ms": other["pass0_ms"], "pass1_ms": other["pass1_ms"]}
+    if ferr is not None:
+        out["force_err_mixed_vs_fp64"] = ferr
     if drift:
         out["nve_drift"] = drift
     if ttm is not None:
