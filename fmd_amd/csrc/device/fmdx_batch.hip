@@ -81,6 +81,8 @@ static int upload_descs(fmdx_batch_t *b)
     std::vector<KDesc> h(n > 0 ? n : 1);
     for (int i = 0; i < n; i++) h[i] = b->sys[i]->D;
     if (n) CK(hipMemcpyAsync(b->descs, h.data(), sizeof(KDesc) * n, hipMemcpyHostToDevice, b->st));
+    /* the copy reads h, which ends with this call */
+    CK(hipStreamSynchronize(b->st));
     return FMDX_OK;
 }
 
