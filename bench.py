@@ -372,25 +372,28 @@ def drift_on(dev, n, steps, dt, equil_steps, T0):
     fx = F.load_fmdx()
     F._ck(fx.fmdx_update_forces(dev, -1, 0), "forces")
     F._ck(fx.fmdx_thermo(dev, -1), "thermo")   # the first Berendsen factor needs the temperature
-    for _ in range(equil_steps):
+    # the product loop's periodic check (audit, error word, slot growth) every 32 steps
+    for k in range(equil_steps):
         F._ck(fx.fmdx_vv_start(dev, -1, dt, 1, 0.1, T0), "vv_start")
         F._ck(fx.fmdx_update_forces(dev, -1, 0), "forces")
         F._ck(fx.fmdx_vv_finish(dev, -1, dt), "vv_finish")
+        if (k + 1) % 32 == 0:
+            W.check_device(fx, dev)
     F._ck(fx.fmdx_thermo(dev, -1), "thermo")
-    th = F.Thermo()
-    F._ck(fx.fmdx_read_thermo(dev, C.byref(th)), "read_thermo")
+    th = W.check_device(fx, dev)
     es = [th.pe + th.ke]
     for k in range(steps):
         F._ck(fx.fmdx_vv_start(dev, -1, dt, 0, 1.0, 0.0), "vv_start")
         F._ck(fx.fmdx_update_forces(dev, -1, 0), "forces")
         F._ck(fx.fmdx_vv_finish(dev, -1, dt), "vv_finish")
         if (k + 1) % 10 == 0:
+            th = F.Thermo()
             F._ck(fx.fmdx_read_thermo(dev, C.byref(th)), "read_thermo")
             es.append(th.pe + th.ke)
+        if (k + 1) % 32 == 0:
+            W.check_device(fx, dev)
+    th = W.check_device(fx, dev)
     T_end = th.temperature
-    err = th.err
-    if err:
-        raise RuntimeError("NVE drift run: device error word %d" % err)
     t = np.arange(len(es)) * 10 * dt * 1e-3   # ns
     e = np.array(es) / n
     slope, icpt = np.polyfit(t, e, 1)

@@ -18,13 +18,14 @@ LIB_DIR = os.path.join(HERE, "lib")
 ROOT = os.path.dirname(HERE)
 
 FMDX_OK = 0
+FMDX_EHIP, FMDX_EARG, FMDX_ENOMEM, FMDX_ESTATE, FMDX_ENODEV = -1, -2, -3, -4, -5
 FMDX_PREC_FP64 = 0
 FMDX_PREC_MIXED = 1
 POT_LJ, POT_EAM, POT_MORSE = 1, 2, 3
 MASS_UNIT = 9.6485332907e+03
 KB = 8.6173303e-5
 
-ERR_POSITION, ERR_RHO_RANGE, ERR_OVERFLOW, ERR_TTM_CE, ERR_TTM_G = 1, 2, 4, 8, 16
+ERR_POSITION, ERR_RHO_RANGE, ERR_OVERFLOW, ERR_TTM_CE, ERR_TTM_G, ERR_FORCE = 1, 2, 4, 8, 16, 32
 
 
 class FmdxError(RuntimeError):
@@ -34,7 +35,9 @@ class FmdxError(RuntimeError):
 class Thermo(C.Structure):
     _fields_ = [("pe", C.c_double), ("embed", C.c_double), ("mv2", C.c_double), ("ke", C.c_double),
                 ("temperature", C.c_double), ("momentum", C.c_double * 3), ("n_active", C.c_double),
-                ("natoms", C.c_uint), ("max_cell_count", C.c_uint), ("err", C.c_uint), ("cap", C.c_uint)]
+                ("natoms", C.c_uint), ("max_cell_count", C.c_uint), ("err", C.c_uint), ("cap", C.c_uint),
+                ("bad_force_cell", C.c_uint), ("bad_position_cell", C.c_uint), ("replay_mismatch", C.c_uint),
+                ("audit_atoms", C.c_uint), ("audit_max_cell", C.c_uint), ("audit_nonfinite", C.c_uint)]
 
 
 class TtmCfg(C.Structure):
@@ -72,7 +75,7 @@ def load_fmdx():
         lib.fmdx_system_create.argtypes = [C.POINTER(P), d3, C.POINTER(C.c_int), C.POINTER(C.c_int), d3, C.c_int,
                                            d3, C.c_int]
         lib.fmdx_system_destroy.argtypes = [P]
-        lib.fmdx_system_destroy.restype = None
+        lib.fmdx_system_destroy.restype = C.c_int
         lib.fmdx_set_precision.argtypes = [P, C.c_int]
         lib.fmdx_set_pair.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
         lib.fmdx_set_eam.argtypes = [P, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double,
@@ -90,6 +93,7 @@ def load_fmdx():
         lib.fmdx_add_velocity.argtypes = [P, C.c_int, C.c_double, C.c_double, C.c_double]
         lib.fmdx_read_thermo.argtypes = [P, C.POINTER(Thermo)]
         lib.fmdx_clear_errors.argtypes = [P]
+        lib.fmdx_audit.argtypes = [P]
         lib.fmdx_sync.argtypes = [P]
         lib.fmdx_set_capacity.argtypes = [P, C.c_uint]
         lib.fmdx_ttm_setup.argtypes = [P, C.POINTER(TtmCfg), P]
@@ -174,7 +178,7 @@ class DeviceSystem:
 
     def close(self):
         if self.h:
-            self.lib.fmdx_system_destroy(self.h)
+            _ck(self.lib.fmdx_system_destroy(self.h), "fmdx_system_destroy")
             self.h = C.c_void_p()
 
     def __del__(self):
@@ -246,6 +250,11 @@ class DeviceSystem:
         t = Thermo()
         _ck(self.lib.fmdx_read_thermo(self.h, C.byref(t)), "fmdx_read_thermo")
         return t
+
+    def audit(self):
+        """the state check (atoms over the cells, fullest cell, non-finite x, v, F); returns the record"""
+        _ck(self.lib.fmdx_audit(self.h), "fmdx_audit")
+        return self.read()
 
     def sync(self):
         _ck(self.lib.fmdx_sync(self.h), "fmdx_sync")

@@ -103,9 +103,10 @@ extern "C" int fmdx_system_create(fmdx_system_t **out, const double l[3], const 
     r |= dalloc(&D.cnt, ncell);
     r |= dalloc(&D.nleave, ncell); r |= dalloc(&D.narr, ncell);
     r |= dalloc(&D.nearly, ncell); r |= dalloc(&D.fill, ncell);
-    r |= dalloc(&D.counters, 8);
+    r |= dalloc(&D.counters, NCOUNTERS);
     r |= dalloc(&D.cell_pe, ncell); r |= dalloc(&D.cell_emb, ncell);
     r |= dalloc(&D.htrips, ncell);
+    r |= dalloc(&D.htile, ncell);
     r |= dalloc(&D.epart, 2 * NPART_MAX);
     r |= dalloc(&D.thermo, 16);
     r |= dalloc(&s->dpairs, MAXK * MAXK);
@@ -115,10 +116,11 @@ extern "C" int fmdx_system_create(fmdx_system_t **out, const double l[3], const 
     hipMemsetAsync(D.narr, 0, sizeof(int) * ncell, s->st);
     hipMemsetAsync(D.nearly, 0, sizeof(int) * ncell, s->st);
     hipMemsetAsync(D.fill, 0, sizeof(int) * ncell, s->st);
-    hipMemsetAsync(D.counters, 0, sizeof(unsigned) * 8, s->st);
+    hipMemsetAsync(D.counters, 0, sizeof(unsigned) * NCOUNTERS, s->st);
     hipMemsetAsync(D.cell_pe, 0, sizeof(double) * ncell, s->st);
     hipMemsetAsync(D.cell_emb, 0, sizeof(double) * ncell, s->st);
     hipMemsetAsync(D.htrips, 0xff, sizeof(int) * ncell, s->st);   /* -1: no recorded hits */
+    hipMemsetAsync(D.htile, 0, sizeof(unsigned long long) * ncell, s->st);
     hipMemsetAsync(D.thermo, 0, sizeof(double) * 16, s->st);
     memset(s->hp, 0, sizeof s->hp);
     D.pairs = s->dpairs;
@@ -130,15 +132,16 @@ extern "C" int fmdx_system_create(fmdx_system_t **out, const double l[3], const 
     return FMDX_OK;
 }
 
-extern "C" void fmdx_system_destroy(fmdx_system_t *s)
+extern "C" int fmdx_system_destroy(fmdx_system_t *s)
 {
-    if (!s) return;
+    if (!s) return FMDX_OK;
+    if (s->member) return FMDX_ESTATE;   /* a batch still holds its device pointers */
     if (s->st) hipStreamSynchronize(s->st);
     KDesc &D = s->D;
     free_slots(D);
     dfree(D.cnt); dfree(D.nleave); dfree(D.narr); dfree(D.nearly); dfree(D.fill);
     dfree(D.counters); dfree(D.cell_pe); dfree(D.cell_emb); dfree(D.thermo);
-    dfree(D.hbits); dfree(D.htrips); dfree(D.epart);
+    dfree(D.hbits); dfree(D.htrips); dfree(D.htile); dfree(D.epart);
     dfree(s->dpairs);
     dfree(s->Fc); dfree(s->rhoc); dfree(s->phic);
     dfree(s->Fc32); dfree(s->rhoc32); dfree(s->phic32);
@@ -149,6 +152,7 @@ extern "C" void fmdx_system_destroy(fmdx_system_t *s)
         if (s->pev[i]) hipEventDestroy(s->pev[i]);
     if (s->st) hipStreamDestroy(s->st);
     free(s);
+    return FMDX_OK;
 }
 
 extern "C" int fmdx_set_precision(fmdx_system_t *s, int precision)
@@ -230,6 +234,7 @@ extern "C" int fmdx_set_eam(fmdx_system_t *s, int nelem, int nrho, double drho, 
                             const double *phi, const double *phiDD, const int *elem_of_kind)
 {
     if (!s || nelem < 1 || nelem > MAXE || nrho < 3 || nr2 < 3 || !(drho > 0) || !(dr2 > 0)) return FMDX_EARG;
+    if (s->member) return FMDX_ESTATE;
     const long mr = nrho - 1, m2 = nr2 - 1;
     std::vector<double4> hF((size_t)nelem * mr), hR((size_t)nelem * m2), hP((size_t)nelem * nelem * m2);
     for (int e = 0; e < nelem; e++)
@@ -278,7 +283,14 @@ extern "C" int fmdx_set_eam(fmdx_system_t *s, int nelem, int nrho, double drho, 
     /* the kernels run on the system stream: the tables have landed before any of them reads */
     CK(hipStreamSynchronize(s->st));
     /* hit words kept between the two EAM passes (one bit per lane and trip) */
-    if (!s->D.hbits && dalloc(&s->D.hbits, (size_t)s->D.ncell * HBITS_WORDS * 64)) return FMDX_ENOMEM;
+    if (!s->D.hbits)
+    {
+        const size_t nw = (size_t)s->D.ncell * HBITS_WORDS * 64;
+        if (dalloc(&s->D.hbits, nw)) return FMDX_ENOMEM;
+        /* never read before the density pass writes it; cleared all the same */
+        CK(hipMemsetAsync(s->D.hbits, 0, nw * sizeof(unsigned long long), s->st));
+        CK(hipStreamSynchronize(s->st));
+    }
     for (int k = 0; k < s->D.nkinds; k++) s->D.elem[k] = elem_of_kind ? elem_of_kind[k] : -1;
     return FMDX_OK;
 }
@@ -301,6 +313,7 @@ extern "C" int fmdx_upload(fmdx_system_t *s, unsigned natoms, const unsigned *ce
                            const double *v, const int *group, const unsigned *atomid, const unsigned *kind)
 {
     if (!s || !cell_count) return FMDX_EARG;
+    if (s->member) return FMDX_ESTATE;
     KDesc &D = s->D;
     std::vector<int> start(D.ncell + 1);
     unsigned maxc = 0;
@@ -362,7 +375,7 @@ extern "C" int fmdx_upload(fmdx_system_t *s, unsigned natoms, const unsigned *ce
     CK(launch_fill(D.fz, nslot, 0.0, s->st));
     CK(launch_fill(D.aux, nslot, 0.0, s->st));
     CK(launch_scatter(D, s->st, dstart, dx, dv, dg, da, dk));
-    unsigned ctr[8] = {0, 0, maxc, 0, 0, 0, 0, 0};
+    unsigned ctr[NCOUNTERS] = {0, 0, maxc};
     CK(hipMemcpyAsync(D.counters, ctr, sizeof ctr, hipMemcpyHostToDevice, s->st));
     CK(hipStreamSynchronize(s->st));
     dfree(dstart); dfree(dx); dfree(dv); dfree(dg); dfree(da); dfree(dk);
@@ -525,7 +538,7 @@ extern "C" int fmdx_read_thermo(fmdx_system_t *s, fmdx_thermo_t *out)
 {
     if (!s || !out) return FMDX_EARG;
     double th[16];
-    unsigned ctr[8];
+    unsigned ctr[NCOUNTERS];
     CK(hipMemcpyAsync(th, s->D.thermo, sizeof th, hipMemcpyDeviceToHost, s->st));
     CK(hipMemcpyAsync(ctr, s->D.counters, sizeof ctr, hipMemcpyDeviceToHost, s->st));
     CK(hipStreamSynchronize(s->st));
@@ -542,6 +555,19 @@ extern "C" int fmdx_read_thermo(fmdx_system_t *s, fmdx_thermo_t *out)
     out->max_cell_count = ctr[2];
     out->err = ctr[3];
     out->cap = (unsigned)s->D.cap;
+    out->bad_force_cell = ctr[4];
+    out->bad_position_cell = ctr[5];
+    out->replay_mismatch = ctr[6];
+    out->audit_atoms = ctr[8];
+    out->audit_max_cell = ctr[9];
+    out->audit_nonfinite = ctr[10];
+    return FMDX_OK;
+}
+
+extern "C" int fmdx_audit(fmdx_system_t *s)
+{
+    if (!s || !s->uploaded) return FMDX_ESTATE;
+    CK(launch_audit(s->D, s->st));
     return FMDX_OK;
 }
 
@@ -564,7 +590,7 @@ extern "C" int fmdx_memory(fmdx_system_t *s, size_t *bytes)
                             sizeof(unsigned long long) + sizeof(int);
     const size_t nmov = s->natoms_uploaded + 64;
     size_t b = s->uploaded ? ns * per_slot + nmov * (6 * sizeof(double) + 9) : 0;
-    b += (size_t)D.ncell * (6 * sizeof(int) + 2 * sizeof(double));
+    b += (size_t)D.ncell * (6 * sizeof(int) + 2 * sizeof(double) + sizeof(unsigned long long));
     if (D.hbits) b += (size_t)D.ncell * HBITS_WORDS * 64 * sizeof(unsigned long long);
     if (s->rhoc) b += (size_t)(D.eam.nrho + 6 * D.eam.nr2) * (sizeof(double4) + sizeof(float4));
     *bytes = b;
@@ -581,6 +607,7 @@ extern "C" int fmdx_sync(fmdx_system_t *s)
 extern "C" int fmdx_set_capacity(fmdx_system_t *s, unsigned cap)
 {
     if (!s || !s->uploaded || cap < 8 || cap > 256) return FMDX_EARG;
+    if (s->member) return FMDX_ESTATE;
     unsigned ctr[8];
     CK(hipMemcpyAsync(ctr, s->D.counters, sizeof ctr, hipMemcpyDeviceToHost, s->st));
     CK(hipStreamSynchronize(s->st));

@@ -219,7 +219,10 @@ static int grow(fmdx_batch_t *b)
         unsigned nc = maxc + maxc / 2 + 8;
         nc = (nc + 31) & ~31u;   /* multiples of 32 slots: each cell starts on a cache line */
         if (nc > 256) nc = 256;
+        /* the batch itself swaps the member's slot arrays: its descriptors are refreshed below */
+        s->member = 0;
         int r = fmdx_set_capacity(s, nc);
+        s->member = 1;
         if (r) return r;
         changed = 1;
     }
@@ -259,6 +262,7 @@ extern "C" void fmdx_batch_destroy(fmdx_batch_t *b)
 {
     if (!b) return;
     hipStreamSynchronize(b->st);
+    for (auto *s : b->sys) s->member = 0;
     graph_drop(b);
     dfree(b->descs);
     dfree(b->items);
@@ -278,6 +282,7 @@ extern "C" int fmdx_batch_add(fmdx_batch_t *b, fmdx_system_t *s, double dt, int 
     if (s->ttm_on) return FMDX_EARG;   /* TTM members are not batched */
     for (auto *t : b->sys)
         if (t == s) return FMDX_EARG;
+    if (s->member) return FMDX_ESTATE;   /* already in another batch */
     int r = fmdx_set_precision(s, b->precision);
     if (r) return r;
     r = fmdx_update_forces(s, -1, 0);
@@ -286,6 +291,7 @@ extern "C" int fmdx_batch_add(fmdx_batch_t *b, fmdx_system_t *s, double dt, int 
     if (r) return r;
     CK(hipStreamSynchronize(s->st));
     set_step(s->D, s->mass, -1, dt, thermostat, tau, T0);
+    s->member = 1;
     b->sys.push_back(s);
     b->dirty = 1;
     return FMDX_OK;
@@ -297,6 +303,7 @@ extern "C" int fmdx_batch_remove(fmdx_batch_t *b, fmdx_system_t *s)
     auto it = std::find(b->sys.begin(), b->sys.end(), s);
     if (it == b->sys.end()) return FMDX_EARG;
     CK(hipStreamSynchronize(b->st));
+    s->member = 0;
     b->sys.erase(it);
     b->dirty = 1;
     return FMDX_OK;
@@ -371,6 +378,10 @@ extern "C" int fmdx_batch_status(fmdx_batch_t *b, fmdx_thermo_t *out)
         o.max_cell_count = ct[2];
         o.err = ct[3];
         o.cap = (unsigned)b->sys[i]->D.cap;
+        o.bad_force_cell = ct[4];
+        o.bad_position_cell = ct[5];
+        o.replay_mismatch = ct[6];
+        o.audit_atoms = o.audit_max_cell = o.audit_nonfinite = 0;
     }
     return FMDX_OK;
 }

@@ -25,6 +25,7 @@
 #define MAXE FMDX_MAX_ELEMENTS
 #define NPART_MAX 4096         /* blocks reduced by the single-block reductions */
 #define HBITS_WORDS 4          /* 64-bit hit words per lane kept between the EAM passes */
+#define NCOUNTERS 16           /* status words per system */
 
 /* mover destination codes */
 #define MV_STAY (-1)
@@ -89,7 +90,10 @@ struct KDesc
     int *mvgrp;
     unsigned *mvaid;
     uint8_t *mvkind;
-    unsigned *counters;   /* [0] movers, [1] deleted, [2] max cell count, [3] error word */
+    unsigned *counters;   /* [0] movers, [1] deleted, [2] max cell count, [3] error word, [4] cell + 1 of
+                             the first non-finite force, [5] cell + 1 of the first unbinnable position,
+                             [6] force-pass rescans after a tile mismatch, [8..10] audit: atoms, fullest
+                             cell, atoms with a non-finite x, v or F (NCOUNTERS entries) */
     /* potentials */
     int nkinds;
     int single_cat;        /* category when one category is used everywhere, else -1 */
@@ -102,6 +106,7 @@ struct KDesc
     /* pair-candidate hits found by the density pass, replayed by the force pass */
     unsigned long long *hbits;   /* [ncell][HBITS_WORDS][64] one bit per (lane, scan trip) */
     int *htrips;                 /* [ncell] hit words recorded, -1 = none (the force pass rescans) */
+    unsigned long long *htile;   /* [ncell] candidates staged in each of the first 4 tile rounds, 16 bits each */
     /* reductions */
     double *cell_pe;       /* [ncell] pair energy per home cell */
     double *cell_emb;      /* [ncell] embedding energy per home cell */
@@ -150,6 +155,12 @@ enum
 static __device__ __forceinline__ void set_err(const KDesc &D, unsigned bits)
 {
     atomicOr(&D.counters[3], bits);
+}
+
+/* keep the first value written into a diagnostic word that starts at 0 */
+static __device__ __forceinline__ void note_first(unsigned *w, unsigned v)
+{
+    atomicCAS(w, 0u, v);
 }
 
 /* The O(N) kernels map a block of TPB = 256 threads onto CPB = 16 consecutive
@@ -222,5 +233,6 @@ hipError_t launch_ttm_fields(const KDesc &D, const TtmState &S, hipStream_t st);
 hipError_t launch_ttm_presolve(const TtmState &S, unsigned *err, hipStream_t st);
 hipError_t launch_ttm_solve(const TtmState &S, double time, double dt, unsigned *err, hipStream_t st);
 hipError_t launch_fill(double *p, long n, double v, hipStream_t st);
+hipError_t launch_audit(const KDesc &D, hipStream_t st);
 
 #endif /* FMDX_INTERNAL_H */

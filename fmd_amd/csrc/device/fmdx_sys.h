@@ -17,6 +17,7 @@ struct fmdx_system
     KDesc D;
     unsigned natoms_uploaded;
     int uploaded;
+    int member;            /* 1 while a batch holds this system's device pointers */
     double mass[MAXK];
     PairParam hp[MAXK * MAXK];
     PairParam *dpairs;

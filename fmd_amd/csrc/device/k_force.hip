@@ -570,7 +570,11 @@ static __device__ void b_cell(const KDesc &D, const int c)
     if (PASS == 0 && D.has_eam) hw = __builtin_amdgcn_readfirstlane(D.htrips[c]);
     if (PASS == 1 && lane == 0 && !one_block) D.htrips[c] = -1;
     const bool record = PASS == 1 && one_block;
-    const bool replay = PASS == 0 && one_block && hw >= 0;
+    bool replay = PASS == 0 && one_block && hw >= 0;
+    /* tile sizes of the density pass's first four rounds (16 bits each): the force pass replays
+       a round only if it staged as many candidates, and rescans from the first round that differs */
+    unsigned long long tiles = 0ull;
+    if (replay) tiles = D.htile[c];
     PairParam p1;
     if (CAT != CAT_GEN) p1 = pp[0];
     /* the home cell's box in the frame of the staged positions, and the culling
@@ -612,6 +616,7 @@ static __device__ void b_cell(const KDesc &D, const int c)
         for (int q = 0; q < 4; q++) acc[q].zero();
         int wi = 0;   /* hit words so far (wave-uniform) */
         int gpos = 0; /* next candidate of the 27-cell list to stage (wave-uniform) */
+        int round = 0;
         while (gpos < ncand)
         {
             __syncthreads();   /* the previous tile is no longer read */
@@ -630,6 +635,10 @@ static __device__ void b_cell(const KDesc &D, const int c)
                 int kj = 0, lo = 0;
                 if (g < ncand)
                 {
+                    /* staged position and culling test without FMA contraction, so that every
+                       instantiation keeps the same candidates: the force pass replays the
+                       density pass's hit words by tile position */
+#pragma clang fp contract(off)
                     int hi = 27;
                     while (hi - lo > 1)
                     {
@@ -665,6 +674,17 @@ static __device__ void b_cell(const KDesc &D, const int c)
                 gpos += WG;
             }
             if (s >= S) selft = -1;
+            {
+                const unsigned sh16 = 16u * (unsigned)min(round, 3);
+                if (record && round < 4) tiles |= (unsigned long long)nt << sh16;
+                const unsigned rec = round < 4 ? (unsigned)((tiles >> sh16) & 0xFFFFull) : 0u;
+                if (replay && rec != (unsigned)nt)
+                {
+                    replay = false;
+                    if (lane == 0) atomicAdd(&D.counters[6], 1u);
+                }
+                round++;
+            }
             const int trips = (nt + S - 1) / S;
             const int npad = trips * S;
             for (int m = nt + lane; m < npad; m += WG)
@@ -756,7 +776,11 @@ static __device__ void b_cell(const KDesc &D, const int c)
                 wi++;
             }
         }
-        if (record && lane == 0) D.htrips[c] = wi <= HBITS_WORDS ? wi : -1;
+        if (record && lane == 0)
+        {
+            D.htrips[c] = wi <= HBITS_WORDS ? wi : -1;
+            D.htile[c] = tiles;
+        }
         __syncthreads();   /* the tile is reused for the stripe sums */
         red[0][lane] = acc[0].get();
         red[1][lane] = acc[1].get();
@@ -799,8 +823,8 @@ static __device__ void b_cell(const KDesc &D, const int c)
                 if (act && TT.on)
                 {
                     /* add_ttm_term_to_forces, forces.c:292-330 (z only for a 1 x 1 x n turi) */
-                    const double zabs = sizeof(R) == sizeof(double) ? (double)zi : (double)zi + cz * D.h2;
-                    const int iz = (int)(zabs / TT.tcellh_z) + 1;
+                    /* the FP64 position, as k_ttm_index uses for the cell sums of the same atoms */
+                    const int iz = (int)(D.z[is] / TT.tcellh_z) + 1;
                     if (iz >= 0 && iz < TT.ncz + 2)
                     {
                         if (TT.ncx * TT.ncy > 1)
@@ -817,6 +841,11 @@ static __device__ void b_cell(const KDesc &D, const int c)
                         else
                             sfz += TT.xi[iz] * D.mass[ki] * (D.vz[is] - TT.vcm[3 * iz + 2]);
                     }
+                }
+                if (act && !(__builtin_isfinite(sfx) && __builtin_isfinite(sfy) && __builtin_isfinite(sfz)))
+                {
+                    set_err(D, FMDX_DEV_ERR_FORCE);
+                    note_first(&D.counters[4], (unsigned)c + 1u);
                 }
                 D.fx[is] = act ? sfx : 0.0;
                 D.fy[is] = act ? sfy : 0.0;

@@ -59,6 +59,7 @@ static __device__ void bin_flag(const KDesc &D, int c, long slot, double x, doub
     if (!(f0 >= -1.0 && f0 <= D.nc0 && f1 >= -1.0 && f1 <= D.nc1 && f2 >= -1.0 && f2 <= D.nc2))
     {
         set_err(D, FMDX_DEV_ERR_POSITION);
+        note_first(&D.counters[5], (unsigned)c + 1u);
         D.mv[slot] = MV_STAY;
         return;
     }
@@ -66,6 +67,7 @@ static __device__ void bin_flag(const KDesc &D, int c, long slot, double x, doub
     if (j0 < 0 || j0 >= D.nc0 + 2 || j1 < 0 || j1 >= D.nc1 + 2 || j2 < 0 || j2 >= D.nc2 + 2)
     {
         set_err(D, FMDX_DEV_ERR_POSITION);
+        note_first(&D.counters[5], (unsigned)c + 1u);
         D.mv[slot] = MV_STAY;
         return;
     }
@@ -653,6 +655,61 @@ __global__ __launch_bounds__(TPB) void k_gather(KDesc D, const int *cell_start, 
     });
 }
 
+/* state check between steps: atoms over the cells (counters[8]), the fullest cell ([9]) and the
+   atoms whose position, velocity or force is not finite ([10]); a grid-stride walk over groups of
+   CPB cells, one set of atomics per block */
+__global__ __launch_bounds__(TPB) void k_audit(KDesc D)
+{
+    unsigned n = 0, m = 0, bad = 0;
+    const int ng = (D.ncell + CPB - 1) / CPB;
+    for (int g = blockIdx.x; g < ng; g += gridDim.x)
+    {
+        const int c = g * CPB + (int)threadIdx.x / LPC;
+        if (c >= D.ncell) continue;
+        const int k0 = (int)threadIdx.x % LPC;
+        const int cnt = D.cnt[c];
+        if (k0 == 0)
+        {
+            n += (unsigned)cnt;
+            m = max(m, (unsigned)cnt);
+        }
+        const int lim = min(cnt, D.cap);
+        for (int k = k0; k < lim; k += LPC)
+        {
+            const long q = (long)c * D.cap + k;
+            /* the sum is finite only if every term is (or overflows, which is as bad) */
+            const double t = D.x[q] + D.y[q] + D.z[q] + D.vx[q] + D.vy[q] + D.vz[q] + D.fx[q] + D.fy[q] + D.fz[q];
+            if (!__builtin_isfinite(t)) bad++;
+        }
+    }
+    __shared__ unsigned sh[3][TPB];
+    sh[0][threadIdx.x] = n;
+    sh[1][threadIdx.x] = m;
+    sh[2][threadIdx.x] = bad;
+    __syncthreads();
+    for (int s = TPB / 2; s > 0; s >>= 1)
+    {
+        if ((int)threadIdx.x < s)
+        {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+            sh[1][threadIdx.x] = max(sh[1][threadIdx.x], sh[1][threadIdx.x + s]);
+            sh[2][threadIdx.x] += sh[2][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        atomicAdd(&D.counters[8], sh[0][0]);
+        atomicMax(&D.counters[9], sh[1][0]);
+        atomicAdd(&D.counters[10], sh[2][0]);
+    }
+}
+
+__global__ __launch_bounds__(64) void k_audit_clear(KDesc D)
+{
+    if (threadIdx.x >= 8 && threadIdx.x <= 10) D.counters[threadIdx.x] = 0u;
+}
+
 __global__ __launch_bounds__(TPB) void k_fill(double *p, long n, double v)
 {
     long i = (long)blockIdx.x * TPB + threadIdx.x;
@@ -736,6 +793,14 @@ hipError_t launch_gather(const KDesc &D, hipStream_t st, const int *cell_start, 
                          double *aux, int *grp, unsigned *aid, uint8_t *kind)
 {
     hipLaunchKernelGGL(k_gather, dim3(nblk_h(D)), dim3(TPB), 0, st, D, cell_start, x, v, f, aux, grp, aid, kind);
+    return hipGetLastError();
+}
+
+hipError_t launch_audit(const KDesc &D, hipStream_t st)
+{
+    const unsigned ng = nblk_h(D);
+    hipLaunchKernelGGL(k_audit_clear, dim3(1), dim3(64), 0, st, D);
+    hipLaunchKernelGGL(k_audit, dim3(ng < 1024 ? (ng ? ng : 1) : 1024), dim3(TPB), 0, st, D);
     return hipGetLastError();
 }
 

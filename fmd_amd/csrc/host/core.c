@@ -143,7 +143,8 @@ fmd_t *fmd_create()
 void fmd_free(fmd_t *md)
 {
     if (!md) return;
-    if (md->dev) fmdx_system_destroy(md->dev);
+    if (md->dev && fmdx_system_destroy(md->dev) != FMDX_OK)
+        fmdh_err_function_failed(md, true, HERE, "fmdx_system_destroy: the system is still in a batch");
     if (md->ConfigFilep && md->_OldNumberOfParticles != -1) fclose(md->ConfigFilep);
     fmdh_gg_free(md);
     fmdh_potsys_free(md);

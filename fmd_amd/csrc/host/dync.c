@@ -35,11 +35,19 @@ static unsigned choose_cap(unsigned maxc)
     return c > 256 ? 256 : c;
 }
 
-static void report_device_errors(fmd_t *md, unsigned err)
+static void report_device_errors(fmd_t *md, const fmdx_thermo_t *t)
 {
-    unsigned fresh = err & ~md->dev_err_reported;
+    unsigned fresh = t->err & ~md->dev_err_reported;
     if (!fresh) return;
     md->dev_err_reported |= fresh;
+    /* where the device first saw a bad position or force, for the record (the device checks every
+       step; the host learns of it at the next synchronisation) */
+    if (md->ShowErrorMessages && (fresh & (FMDX_DEV_ERR_POSITION | FMDX_DEV_ERR_FORCE)))
+        fprintf(stderr, "FMD: device error word %u by step %d: first non-finite force in cell %d, "
+                        "first unbinnable position in cell %d (-1: none)\n",
+                t->err, md->time_iteration, (int)t->bad_force_cell - 1, (int)t->bad_position_cell - 1);
+    if (fresh & FMDX_DEV_ERR_FORCE)
+        fmdh_err_outside_real_interval(md, true, HERE, "force on an atom", NAN, "of finite values");
     if (fresh & FMDX_DEV_ERR_POSITION) fmdh_err_unexpected_position(md, true, HERE);
     if (fresh & FMDX_DEV_ERR_RHO_RANGE)
         fmdh_err_outside_real_interval(md, true, HERE, "electron density", NAN, "related to EAM potential");
@@ -72,19 +80,32 @@ void fmdh_sync_thermo(fmd_t *md)
         return;
     }
     apply_thermo(md, &t);
-    report_device_errors(md, t.err);
+    report_device_errors(md, &t);
 }
 
+/* Periodic check of the resident state (every CHECK_EVERY steps and at the end of a loop): the
+   device error word, and an audit of the cells -- the atoms counted over the cells equal the atoms
+   resident, no cell holds more than its slots, every position, velocity and force is finite.  A
+   broken invariant is reported once, naming the step. */
 void fmdh_check_device(fmd_t *md)
 {
     fmdx_thermo_t t;
-    if (fmdx_read_thermo(md->dev, &t) != FMDX_OK)
+    if (fmdx_audit(md->dev) != FMDX_OK || fmdx_read_thermo(md->dev, &t) != FMDX_OK)
     {
         fmdh_err_function_failed(md, true, HERE, "fmdx_read_thermo");
         return;
     }
     apply_thermo(md, &t);
-    report_device_errors(md, t.err);
+    report_device_errors(md, &t);
+    if (!md->audit_reported && (t.audit_atoms != t.natoms || t.audit_max_cell > t.cap || t.audit_nonfinite))
+    {
+        char msg[200];
+        snprintf(msg, sizeof msg, "device state check at step %d: %u atoms in the cells for %u resident, "
+                                  "fullest cell %u of %u slots, %u atoms with non-finite x, v or F",
+                 md->time_iteration, t.audit_atoms, t.natoms, t.audit_max_cell, t.cap, t.audit_nonfinite);
+        md->audit_reported = true;
+        fmdh_err_function_failed(md, true, HERE, msg);
+    }
     if (t.max_cell_count + t.max_cell_count / 4 + 4 > t.cap && t.cap < 256)
     {
         if (fmdx_set_capacity(md->dev, choose_cap(t.max_cell_count)) != FMDX_OK)

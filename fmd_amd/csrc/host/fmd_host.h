@@ -169,6 +169,7 @@ struct _fmd
     fmdx_system_t *dev;
     int precision;
     unsigned dev_err_reported;
+    bool audit_reported;          /* a failed device state check has been reported */
     bool dev_pot_dirty;
     /* dynamics */
     double time, timestep;

@@ -143,6 +143,32 @@ def eam_cu_system(cells, seed, setfl, precision):
     return lib, md, lib.fmd_ext_getDevice(md), len(x)
 
 
+def choose_cap(maxc):
+    """Slot capacity for a fullest cell of maxc atoms: the rule of the product's time loop
+    (1.5 x + 8, in multiples of 32, at most 256)."""
+    c = maxc + maxc // 2 + 8
+    return min(256, (c + 31) & ~31)
+
+
+def check_device(fx, dev):
+    """The periodic check of the product's time loops: audit the resident state, read the thermo
+    record, raise on a device error word or a broken invariant (naming where the device first saw
+    it), and grow the slot capacity by the loop's rule when the fullest cell nears it."""
+    F._ck(fx.fmdx_audit(dev), "audit")
+    th = F.Thermo()
+    F._ck(fx.fmdx_read_thermo(dev, C.byref(th)), "read_thermo")
+    if th.err:
+        raise RuntimeError("device error word %d (first non-finite force in cell %d, first unbinnable "
+                           "position in cell %d)" % (th.err, th.bad_force_cell - 1, th.bad_position_cell - 1))
+    if th.audit_atoms != th.natoms or th.audit_nonfinite or th.audit_max_cell > th.cap:
+        raise RuntimeError("device state check: %d atoms in the cells for %d resident, fullest cell %d of %d "
+                           "slots, %d atoms with non-finite x, v or F"
+                           % (th.audit_atoms, th.natoms, th.audit_max_cell, th.cap, th.audit_nonfinite))
+    if th.max_cell_count + th.max_cell_count // 4 + 4 > th.cap and th.cap < 256:
+        F._ck(fx.fmdx_set_capacity(dev, choose_cap(th.max_cell_count)), "set_capacity")
+    return th
+
+
 def vv_berendsen_steps(fx, dev, nsteps, dt, tau, T0):
     """nsteps velocity-Verlet steps with Berendsen rescaling through the C-ABI seams."""
     for _ in range(nsteps):
@@ -174,9 +200,7 @@ def time_eam_cu_steps(cells, steps, warmup, precision, setfl, seed=1):
     cnt = C.c_uint(0)
     F._ck(fx.fmdx_profile_read(dev, prof, C.byref(cnt)), "profile_read")
     F._ck(fx.fmdx_profile_enable(dev, 0), "profile")
-    F._ck(fx.fmdx_read_thermo(dev, C.byref(th)), "read_thermo")
-    if th.err:
-        raise RuntimeError("device error word %d" % th.err)
+    th = check_device(fx, dev)
     res = dict(natoms=n, steps=steps, wall_s=wall, gpu_ms=ms.value,
                pass1_ms=prof[0] / max(cnt.value, 1), pass0_ms=prof[1] / max(cnt.value, 1),
                T=th.temperature, pe=th.pe, cap=th.cap, max_cell=th.max_cell_count)

@@ -57,7 +57,9 @@ enum
     FMDX_DEV_ERR_RHO_RANGE = 1u << 1,  /* rho beyond the F(rho) table, eam.c:168-171 */
     FMDX_DEV_ERR_OVERFLOW = 1u << 2,   /* a cell outgrew its slot capacity */
     FMDX_DEV_ERR_TTM_CE = 1u << 3,     /* Te outside the Ce table, ttm.c:163-166 */
-    FMDX_DEV_ERR_TTM_G = 1u << 4       /* Te outside the G table, ttm.c:179-182 */
+    FMDX_DEV_ERR_TTM_G = 1u << 4,      /* Te outside the G table, ttm.c:179-182 */
+    FMDX_DEV_ERR_FORCE = 1u << 5       /* a non-finite force; no counterpart in the reference, where such a
+                                          force shows up one step later as misc.c:281-291 */
 };
 
 /* force-kernel arithmetic */
@@ -89,6 +91,17 @@ typedef struct
     unsigned max_cell_count; /* largest cell population seen by the last re-binning */
     unsigned err;            /* sticky FMDX_DEV_ERR_* bits */
     unsigned cap;            /* slots per cell */
+    /* diagnostics: cell index + 1 of the first atom seen with a non-finite force or an unbinnable
+       position (0: none), and the cells whose density-pass hits were rescanned by the force pass
+       because its tile differed (always 0 unless the two passes disagree) */
+    unsigned bad_force_cell;
+    unsigned bad_position_cell;
+    unsigned replay_mismatch;
+    /* filled by fmdx_audit: atoms counted over the cells, the fullest cell, and the atoms whose
+       position, velocity or force is not finite */
+    unsigned audit_atoms;
+    unsigned audit_max_cell;
+    unsigned audit_nonfinite;
 } fmdx_thermo_t;
 
 /* two-temperature model description: the reference's 1 x 1 x ncz turi, and
@@ -124,7 +137,8 @@ const char *fmdx_strerror(int code);
 /* one system: box, cell grid (nc cells of size cellh per dimension) and kinds */
 int fmdx_system_create(fmdx_system_t **out, const double l[3], const int pbc[3], const int nc[3],
                        const double cellh[3], int nkinds, const double *mass, int precision);
-void fmdx_system_destroy(fmdx_system_t *s);
+/* refuses (FMDX_ESTATE) while the system is a member of a batch */
+int fmdx_system_destroy(fmdx_system_t *s);
 int fmdx_set_precision(fmdx_system_t *s, int precision);
 
 /* pair table: LJ p0 = 4 eps, p1 = sigma^2; Morse p0 = D0, p1 = alpha, p2 = r0; EAM p0 = iloc, p1 = jloc */
@@ -159,6 +173,9 @@ int fmdx_add_velocity(fmdx_system_t *s, int group, double vx, double vy, double 
 int fmdx_read_thermo(fmdx_system_t *s, fmdx_thermo_t *out);
 int fmdx_clear_errors(fmdx_system_t *s);
 int fmdx_sync(fmdx_system_t *s);
+/* state check (asynchronous): counts the atoms over the cells, the fullest cell and the atoms with a
+   non-finite position, velocity or force; fmdx_read_thermo returns them (audit_*) */
+int fmdx_audit(fmdx_system_t *s);
 /* set the per-cell slot capacity (synchronises); the library's own choices are multiples of 32 */
 int fmdx_set_capacity(fmdx_system_t *s, unsigned cap);
 
@@ -195,8 +212,11 @@ int fmdx_profile_read(fmdx_system_t *s, double *ms_by_phase /* 8 entries */, uns
  * advanced together, every kernel of a step covering all members through
  * (system, block) work lists, the step replayed from a captured HIP graph.
  * For ensembles of small and medium systems and for served jobs.  A member is
- * an uploaded system without TTM; while it is a member only the batch calls
- * act on it.  Its trajectory is the same alone and in any mix. */
+ * an uploaded system without TTM, in one batch at a time; while it is a
+ * member only the batch calls act on it, and fmdx_system_destroy,
+ * fmdx_set_capacity, fmdx_upload and fmdx_set_eam refuse it (FMDX_ESTATE),
+ * since the batch's captured step reads its device pointers.  Its trajectory
+ * is the same alone and in any mix. */
 typedef struct fmdx_batch fmdx_batch_t;
 
 int fmdx_batch_create(fmdx_batch_t **out, int precision);
