@@ -36,7 +36,7 @@ class Thermo(C.Structure):
     _fields_ = [("pe", C.c_double), ("embed", C.c_double), ("mv2", C.c_double), ("ke", C.c_double),
                 ("temperature", C.c_double), ("momentum", C.c_double * 3), ("n_active", C.c_double),
                 ("natoms", C.c_uint), ("max_cell_count", C.c_uint), ("err", C.c_uint), ("cap", C.c_uint),
-                ("bad_force_cell", C.c_uint), ("bad_position_cell", C.c_uint), ("replay_mismatch", C.c_uint),
+                ("bad_force_cell", C.c_uint), ("bad_position_cell", C.c_uint), ("replay_mismatch", C.c_uint), ("exact_unmatched", C.c_uint),
                 ("audit_atoms", C.c_uint), ("audit_max_cell", C.c_uint), ("audit_nonfinite", C.c_uint)]
 
 

@@ -1,6 +1,6 @@
 """Build-time tuner of the pair kernels' compile-time shape (fmdx_tuned.h).
 
-The LDS candidate tile (FMDX_TILE64 / FMDX_TILE32) and the pairs per iteration
+The LDS candidate tile (FMDX_TILE64 / FMDX_TILE32 / FMDX_TILE32_EAM) and the pairs per iteration
 of the single-element EAM passes (FMDX_EAM_NP{64,32}_{0,1}) are constants of
 k_force.hip.  They are chosen at build time, not per run: the tile decides how
 a cell's candidates split into staging rounds and stripes, and with it the
@@ -46,6 +46,7 @@ PARAMS = [
     ("FMDX_EAM_NP64_0", (2, 3), FP64),
     ("FMDX_EAM_NP64_1", (2, 3), FP64),
     ("FMDX_TILE32", (320, 384, 448), MIXED),
+    ("FMDX_TILE32_EAM", (280, 312, 344), MIXED),
     ("FMDX_EAM_NP32_0", (2, 3, 4), MIXED),
     ("FMDX_EAM_NP32_1", (2, 4, 6), MIXED),
 ]

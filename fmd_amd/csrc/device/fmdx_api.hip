@@ -8,6 +8,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <algorithm>
 #include <vector>
 #include "fmdx_sys.h"
 
@@ -229,6 +230,28 @@ static void spline_coeffs(const double *y, const double *ydd, int n, double h, d
     }
 }
 
+/* Start (in r^2) of the steep run that ends a table at the cutoff: walking down from the
+   interval holding rc^2, the intervals whose curvature |2 c2| + |6 c3| exceeds 10 x its median
+   over [rc^2/4, rc^2].  A setfl table that is not zero at its cutoff ends in such a run (the two
+   zero knots appended by eam.c:444-471 make a kink that the natural spline spreads over the
+   intervals below it, decaying about 3.7x per r interval); a smooth table has none (+inf). */
+static double steep_start(const double4 *c, long m, double dr2, double rc2)
+{
+    long kc = (long)(rc2 / dr2);
+    if (kc > m - 1) kc = m - 1;
+    long kq = (long)(0.25 * rc2 / dr2);
+    if (kq > kc) kq = kc;
+    std::vector<double> kap;
+    kap.reserve(kc - kq + 1);
+    for (long k = kq; k <= kc; k++) kap.push_back(fabs(2.0 * c[k].z) + fabs(6.0 * c[k].w));
+    std::vector<double> srt(kap);
+    std::nth_element(srt.begin(), srt.begin() + srt.size() / 2, srt.end());
+    const double lim = 10.0 * srt[srt.size() / 2];
+    long k = kc;
+    while (k >= kq && kap[k - kq] > lim) k--;
+    return k == kc ? INFINITY : (double)(k + 1) * dr2;
+}
+
 extern "C" int fmdx_set_eam(fmdx_system_t *s, int nelem, int nrho, double drho, int nr2, double dr2, double rc2,
                             const double *F, const double *FDD, const double *rho, const double *rhoDD,
                             const double *phi, const double *phiDD, const int *elem_of_kind)
@@ -277,6 +300,14 @@ extern "C" int fmdx_set_eam(fmdx_system_t *s, int nelem, int nrho, double drho, 
     EamDev &E = s->D.eam;
     E.ne = nelem; E.nrho = nrho; E.nr2 = nr2;
     E.drho = drho; E.inv_drho = 1.0 / drho; E.dr2 = dr2; E.inv_dr2 = 1.0 / dr2; E.rc2 = rc2;
+    /* FP32 kernels evaluate the pairs from rx2 up in FP64: the rounding band below rc^2 and the
+       tables' steep runs (k_force.hip, exact pairs) */
+    E.rx2 = rc2 * (1.0 - 1e-5);
+    for (int e = 0; e < nelem; e++)
+    {
+        E.rx2 = fmin(E.rx2, steep_start(&hR[e * m2], m2, dr2, rc2));
+        for (int f = 0; f < nelem; f++) E.rx2 = fmin(E.rx2, steep_start(&hP[(e * nelem + f) * m2], m2, dr2, rc2));
+    }
     E.Fc = s->Fc; E.rhoc = s->rhoc; E.phic = s->phic;
     E.Fc32 = s->Fc32; E.rhoc32 = s->rhoc32; E.phic32 = s->phic32;
     E.rp = s->rp; E.rp32 = s->rp32;
@@ -558,6 +589,7 @@ extern "C" int fmdx_read_thermo(fmdx_system_t *s, fmdx_thermo_t *out)
     out->bad_force_cell = ctr[4];
     out->bad_position_cell = ctr[5];
     out->replay_mismatch = ctr[6];
+    out->exact_unmatched = ctr[7];
     out->audit_atoms = ctr[8];
     out->audit_max_cell = ctr[9];
     out->audit_nonfinite = ctr[10];

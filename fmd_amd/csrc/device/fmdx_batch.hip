@@ -381,6 +381,7 @@ extern "C" int fmdx_batch_status(fmdx_batch_t *b, fmdx_thermo_t *out)
         o.bad_force_cell = ct[4];
         o.bad_position_cell = ct[5];
         o.replay_mismatch = ct[6];
+        o.exact_unmatched = ct[7];
         o.audit_atoms = o.audit_max_cell = o.audit_nonfinite = 0;
     }
     return FMDX_OK;

@@ -46,6 +46,7 @@ struct EamDev
 {
     int ne, nrho, nr2;
     double drho, inv_drho, dr2, inv_dr2, rc2;
+    double rx2;            /* FP32 kernels: pairs from this r^2 up are evaluated in FP64 (k_force.hip) */
     const double4 *Fc;     /* [ne][nrho-1] */
     const double4 *rhoc;   /* [ne][nr2-1]  */
     const double4 *phic;   /* [ne*ne][nr2-1] */
@@ -92,7 +93,8 @@ struct KDesc
     uint8_t *mvkind;
     unsigned *counters;   /* [0] movers, [1] deleted, [2] max cell count, [3] error word, [4] cell + 1 of
                              the first non-finite force, [5] cell + 1 of the first unbinnable position,
-                             [6] force-pass rescans after a tile mismatch, [8..10] audit: atoms, fullest
+                             [6] force-pass rescans after a tile mismatch, [7] exact-shell pairs not
+                             found (evaluated in FP32), [8..10] audit: atoms, fullest
                              cell, atoms with a non-finite x, v or F (NCOUNTERS entries) */
     /* potentials */
     int nkinds;

@@ -4,6 +4,8 @@
  * with -D; the tuner builds its candidates that way.
  *
  *   FMDX_TILE64 / FMDX_TILE32   candidates per LDS tile, FP64 / FP32 kernels
+ *   FMDX_TILE32_EAM             the same for the FP32 single-element EAM passes, whose tiles also
+ *                               hold each candidate's list place (for the exact FP64 pairs)
  *   FMDX_EAM_NP64_0 / _1        pairs per flush iteration, FP64 force / density pass
  *   FMDX_EAM_NP32_0 / _1        the same for the FP32 (mixed) passes
  *
@@ -31,6 +33,9 @@
 #endif
 #ifndef FMDX_TILE32
 #define FMDX_TILE32 384
+#endif
+#ifndef FMDX_TILE32_EAM
+#define FMDX_TILE32_EAM 344
 #endif
 #ifndef FMDX_EAM_NP32_0
 #define FMDX_EAM_NP32_0 3

@@ -76,7 +76,7 @@ template <> struct Prec<double>
 };
 template <> struct Prec<float>
 {
-    enum { CT = FMDX_TILE32 };
+    enum { CT = FMDX_TILE32 };   /* the EAM kernels use FMDX_TILE32_EAM */
     typedef float4 Q;
     static __device__ __forceinline__ const float4 *rho(const EamDev &E) { return E.rhoc32; }
     static __device__ __forceinline__ const float4 *phi(const EamDev &E) { return E.phic32; }
@@ -89,13 +89,13 @@ template <> struct Prec<float>
 /* LDS candidate tile (positions in the home frame, F' of the candidate, and
    the kind where the generic kernel dispatches on it: KD); N covers the tile
    plus the padding of the last scan trip */
-template <typename R, bool FP, bool KD> struct Tile;
+template <typename R, bool FP, bool KD, int CT> struct Tile;
 struct Rec3 { double x, y, z; };
 template <bool FP> struct RecOf { typedef double4 T; };
 template <> struct RecOf<false> { typedef Rec3 T; };
-template <bool FP, bool KD> struct Tile<double, FP, KD>
+template <bool FP, bool KD, int CT> struct Tile<double, FP, KD, CT>
 {
-    enum { N = Prec<double>::CT + WG };
+    enum { N = CT + WG };
     typename RecOf<FP>::T p[N];   /* x, y, z (, F') */
     uint8_t k[KD ? N : 1];
     __device__ __forceinline__ void put(int m, double x, double y, double zz, double f, int kk)
@@ -116,10 +116,11 @@ template <bool FP, bool KD> struct Tile<double, FP, KD>
         else f = 0.0;
     }
 };
-template <bool FP, bool KD> struct Tile<float, FP, KD>
+template <bool FP, bool KD, int CT> struct Tile<float, FP, KD, CT>
 {
-    enum { N = Prec<float>::CT + WG };
+    enum { N = CT + WG };
     float4 p[N];
+    unsigned short g[N];   /* the candidate's place in the 27-cell list, for the exact FP64 pairs */
     uint8_t k[KD ? N : 1];
     __device__ __forceinline__ void put(int m, float x, float y, float zz, float f, int kk)
     {
@@ -160,11 +161,13 @@ template <> struct Shifts<double>
 template <> struct Shifts<float>
 {
     float s[3][27];
+    signed char w[3][27];   /* the wrap, for the exact FP64 pairs */
     __device__ __forceinline__ void set(const KDesc &D, int n, const int (&wr)[3], const int (&dj)[3], const int (&hc)[3])
     {
         s[0][n] = (float)((dj[0] * D.h0 + wr[0] * D.L0) - hc[0] * D.h0);
         s[1][n] = (float)((dj[1] * D.h1 + wr[1] * D.L1) - hc[1] * D.h1);
         s[2][n] = (float)((dj[2] * D.h2 + wr[2] * D.L2) - hc[2] * D.h2);
+        w[0][n] = (signed char)wr[0]; w[1][n] = (signed char)wr[1]; w[2][n] = (signed char)wr[2];
     }
     __device__ __forceinline__ float sx(const KDesc &D, int n) const { (void)D; return s[0][n]; }
     __device__ __forceinline__ float sy(const KDesc &D, int n) const { (void)D; return s[1][n]; }
@@ -332,6 +335,85 @@ static __device__ __forceinline__ void pair_hit(const KDesc &D, const PairParam 
     }
 }
 
+/* ---- exact pairs of the FP32 kernels ----------------------------------------
+   The FP32 separation of two atoms is off by up to ~1e-6 of rc^2 in r^2 (the
+   rounding of the cell-relative positions, their image shift and the
+   difference).  That decides nothing for a smooth pair function, but two kinds
+   of pairs need the FP64 separation: a pair within that rounding of the cutoff
+   is in or out of the reference's pair set (r^2 < rc^2, eam.h, lj.h, morse.h),
+   and a tabulated function that is not smooth at the cutoff -- the setfl
+   tables end in the two zero knots the reference appends (eam.c:444-471),
+   which the natural spline turns into large oscillating curvature over the
+   last intervals below rc^2 -- turns the rounding into large errors of the
+   derivative there.  Pairs whose FP32 r^2 reaches rx2 (the lower of
+   rc^2 (1 - 1e-5) and the start of the table's steep run, EamDev.rx2) are
+   evaluated here in FP64 from the FP64 positions: the reference's pair set
+   and the FP64 build's arithmetic, each pair term then rounded once to FP32
+   into the compensated sums.  They are a thin shell, well under 1 % of the
+   pairs. */
+#define RX_BAND 1e-5   /* the rounding band below rc^2, relative */
+
+struct ExactCtx
+{
+    const long *nb_base;   /* first slot of each of the 27 neighbour cells */
+    const int *nb_pre;     /* their candidate-list offsets (28 entries) */
+    const signed char (*w)[27];   /* their wraps */
+    long is;               /* the home atom's slot */
+};
+
+/* FP64 separation x_i - x_j of the home atom and the candidate at place g of
+   the 27-cell list, formed as the FP64 kernels form it (x_j + w L, then the
+   difference); returns the candidate's slot */
+static __device__ __forceinline__ long exact_sep(const KDesc &D, const ExactCtx &X, int g, double &dx, double &dy,
+                                                 double &dz)
+{
+#pragma clang fp contract(off)
+    int lo = 0, hi = 27;
+    while (hi - lo > 1)
+    {
+        const int mid = (lo + hi) >> 1;
+        if (X.nb_pre[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    const long js = X.nb_base[lo] + (g - X.nb_pre[lo]);
+    const double xj = D.x[js] + (double)X.w[0][lo] * D.L0;
+    const double yj = D.y[js] + (double)X.w[1][lo] * D.L1;
+    const double zj = D.z[js] + (double)X.w[2][lo] * D.L2;
+    dx = D.x[X.is] - xj;
+    dy = D.y[X.is] - yj;
+    dz = D.z[X.is] - zj;
+    return js;
+}
+
+/* one exact single-element EAM pair (eam.h:270-310 for the density, eam.h:177-228 for the force) */
+template <int PASS, typename A>
+static __device__ void exact_eam1(const KDesc &D, const ExactCtx &X, int g, double rc2, A *acc)
+{
+    double dx, dy, dz;
+    const long js = exact_sep(D, X, g, dx, dy, dz);
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    if (!(r2 < rc2)) return;
+    const EamDev &E = D.eam;
+    const double t = r2 * E.inv_dr2;
+    int k = (int)t;
+    if (k > E.nr2 - 2) k = E.nr2 - 2;
+    const double b = t - (double)k;
+    const double4 qr = E.rp[2 * (long)k];
+    if (PASS == 1)
+    {
+        acc[3].add((float)sp_val<double, double4>(qr, b));
+        return;
+    }
+    const double4 qp = E.rp[2 * (long)k + 1];
+    const double rho = sp_der<double, double4>(qr, b, E.inv_dr2);
+    const double ph = sp_der<double, double4>(qp, b, E.inv_dr2);
+    const double mag = 2.0 * (D.aux[X.is] * rho + D.aux[js] * rho + ph);
+    acc[0].add((float)(-mag * dx));
+    acc[1].add((float)(-mag * dy));
+    acc[2].add((float)(-mag * dz));
+    acc[3].add((float)(0.5 * sp_val<double, double4>(qp, b)));
+}
+
 /* Coefficient records through a buffer resource over the rp table: a lane
    without a pair passes an offset beyond the table, which the buffer unit
    answers with zeros and no memory access, so the gathers stay branch-free
@@ -373,8 +455,11 @@ static __device__ __forceinline__ float4 rec_at(__amdgpu_buffer_rsrc_t r, unsign
    (FMDX_EAM_NP*, fmdx_tuned.h). */
 template <typename R, int PASS, typename TileT, typename A>
 static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T, unsigned long long word, int s,
-                                                  int g0, int S, R xi, R yi, R zi, R Fpi, R rc2, A *acc)
+                                                  int g0, int S, R xi, R yi, R zi, R Fpi, double rc2,
+                                                  unsigned long long &xw, A *acc)
 {
+    constexpr bool EX = sizeof(R) == sizeof(float);   /* FP32: pairs of the exact shell go to *xw */
+    const R rx2 = EX ? (R)D.eam.rx2 : (R)rc2;
     typedef typename Prec<R>::Q Q;
     constexpr int NP = sizeof(R) == sizeof(double) ? (PASS == 0 ? FMDX_EAM_NP64_0 : FMDX_EAM_NP64_1)
                                                    : (PASS == 0 ? FMDX_EAM_NP32_0 : FMDX_EAM_NP32_1);
@@ -400,7 +485,8 @@ static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T
             dy[j] = yi - y;
             dz[j] = zi - z;
             R r2 = dx[j] * dx[j] + dy[j] * dy[j] + dz[j] * dz[j];
-            u[j] = v && r2 < rc2;
+            u[j] = v && r2 < rx2;
+            if (EX) xw |= (v && r2 >= rx2) ? 1ull << t : 0ull;
             r2 = u[j] ? r2 : (R)0;
             const R tt = r2 * inv;
             const int k = min((int)tt, kmax);
@@ -447,11 +533,45 @@ static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T
 /* Single-kind LJ or Morse: the set bits of a hit word two per iteration
    without branches, as flush_eam1 does; a slot without a pair evaluates the
    formulas at r^2 = rc^2 and adds exact zeros. */
-template <typename R, int CAT, typename TileT, typename A>
-static __device__ __forceinline__ void flush_pair2(const TileT &T, unsigned long long word, int s, int g0, int S,
-                                                   R xi, R yi, R zi, const PairParam &p, A *acc)
+/* one exact single-kind LJ or Morse pair (lj.h:26-58, morse.h:26-58) */
+template <int CAT, typename A>
+static __device__ __forceinline__ void exact_pair1(const KDesc &D, const ExactCtx &X, int g, const PairParam &p, A *acc)
 {
+    double dx, dy, dz;
+    exact_sep(D, X, g, dx, dy, dz);
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    if (!(r2 < p.rc2)) return;
+    double fac, en;
+    if (CAT == CAT_LJ) lj_terms<double>(p, r2, fac, en);
+    else morse_terms<double>(p, r2, fac, en);
+    acc[0].add((float)(fac * dx));
+    acc[1].add((float)(fac * dy));
+    acc[2].add((float)(fac * dz));
+    acc[3].add((float)en);
+}
+
+/* one exact pair of any category (the generic dispatch of pair_hit in FP64) */
+template <int PASS, typename A>
+static __device__ void exact_pair(const KDesc &D, const ExactCtx &X, int g, const PairParam &p, A *acc)
+{
+    double dx, dy, dz;
+    const long js = exact_sep(D, X, g, dx, dy, dz);
+    AccD t[4];
+    for (int q = 0; q < 4; q++) t[q].zero();
+    const double Fpi = (PASS == 0 && D.has_eam) ? D.aux[X.is] : 0.0;
+    const double Fpj = (PASS == 0 && D.has_eam) ? D.aux[js] : 0.0;
+    pair_hit<double, PASS, CAT_GEN>(D, p, Fpi, Fpj, dx, dy, dz, t);
+    for (int q = 0; q < 4; q++) acc[q].add((float)t[q].get());
+}
+
+template <typename R, int CAT, typename TileT, typename A>
+static __device__ __forceinline__ void flush_pair2(const KDesc &D, const TileT &T, unsigned long long word, int s,
+                                                   int g0, int S, R xi, R yi, R zi, const PairParam &p,
+                                                   bool &ex, A *acc)
+{
+    constexpr bool EX = sizeof(R) == sizeof(float);   /* FP32: a pair in the rounding band below rc^2 sets ex */
     const R rc2 = (R)p.rc2;
+    const R rx2 = EX ? (R)(p.rc2 * (1.0 - RX_BAND)) : rc2;
     while (__ballot(word != 0ull))
     {
         const bool v1 = word != 0ull;
@@ -467,7 +587,8 @@ static __device__ __forceinline__ void flush_pair2(const TileT &T, unsigned long
         const R dx2 = xi - x2, dy2 = yi - y2, dz2 = zi - z2;
         R r21 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
         R r22 = dx2 * dx2 + dy2 * dy2 + dz2 * dz2;
-        const bool u1 = v1 && r21 < rc2, u2 = v2 && r22 < rc2;
+        const bool u1 = v1 && r21 < rx2, u2 = v2 && r22 < rx2;
+        if (EX) ex = ex || (v1 && r21 >= rx2) || (v2 && r22 >= rx2);
         r21 = u1 ? r21 : rc2;
         r22 = u2 ? r22 : rc2;
         R fac1, en1, fac2, en2;
@@ -484,12 +605,52 @@ static __device__ __forceinline__ void flush_pair2(const TileT &T, unsigned long
     }
 }
 
+/* the bits of hit word w whose pairs have an FP32 r^2 of at least rx2 */
+template <typename TileT>
+static __device__ unsigned long long band_bits(const TileT &T, unsigned long long w, int s, int g0, int S, float xi,
+                                               float yi, float zi, float rx2)
+{
+    unsigned long long out = 0ull;
+    while (w)
+    {
+        const int t = __builtin_ctzll(w);
+        w &= w - 1ull;
+        float x, y, z, f;
+        T.load(s + (g0 + t) * S, x, y, z, f);
+        const float dx = xi - x, dy = yi - y, dz = zi - z;
+        if (dx * dx + dy * dy + dz * dz >= rx2) out |= 1ull << t;
+    }
+    return out;
+}
+
+/* the pairs of the exact shell collected from one hit word (bit t: trip t of
+   this lane's stripe), evaluated in FP64 after the word's FP32 flush */
+template <int PASS, int CAT, typename TileT, typename A>
+static __device__ __forceinline__ void flush_exact(const KDesc &D, const TileT &T, unsigned long long xw, int s, int g0, int S,
+                                   const ExactCtx &X, const PairParam *ppi, const PairParam &p1, A *acc)
+{
+    while (__ballot(xw != 0ull))
+    {
+        if (xw)
+        {
+            const int t = __builtin_ctzll(xw);
+            xw &= xw - 1ull;
+            const int m = s + (g0 + t) * S;
+            if constexpr (CAT == CAT_EAM1) exact_eam1<PASS>(D, X, T.g[m], p1.rc2, acc);
+            else if constexpr (CAT == CAT_LJ || CAT == CAT_MORSE) exact_pair1<CAT>(D, X, T.g[m], p1, acc);
+            else exact_pair<PASS>(D, X, T.g[m], ppi[T.k[m]], acc);
+        }
+    }
+}
+
 template <typename R, int PASS, int CAT>
 static __device__ void b_cell(const KDesc &D, const int c)
 {
     typedef typename AccOf<R>::T Acc;
     typedef Prec<R> P;
-    enum { CT = P::CT };
+    /* candidates per tile: the FP32 EAM kernels keep a smaller tile, so that the density pass's LDS
+       (tile plus the list places of the exact pairs) still lets 20 workgroups share a CU */
+    enum { CT = sizeof(R) == sizeof(float) && CAT == CAT_EAM1 ? FMDX_TILE32_EAM : (int)P::CT };
     constexpr bool WITH_FP = PASS == 0 && (CAT == CAT_GEN || CAT == CAT_EAM1);
     if ((PASS == 1 && !D.has_eam) || D.cnt[c] == 0)
     {
@@ -503,7 +664,8 @@ static __device__ void b_cell(const KDesc &D, const int c)
     __shared__ long nb_base[27];
     __shared__ int nb_pre[28];
     __shared__ Shifts<R> SH;
-    __shared__ Tile<R, WITH_FP, CAT == CAT_GEN> T;
+    __shared__ Tile<R, WITH_FP, CAT == CAT_GEN, CT> T;
+
     extern __shared__ PairParam pp[];   /* nkinds^2 entries, sized at launch */
     /* the stripe sums reuse the tile once the scan is over */
     double(*red)[WG] = reinterpret_cast<double(*)[WG]>(&T);
@@ -600,6 +762,7 @@ static __device__ void b_cell(const KDesc &D, const int c)
         const int S = WG / nhc;
         const int a = lane % nhc, s = lane / nhc;
         const long is = hbase + h0 + a;
+        const ExactCtx X = {nb_base, nb_pre, SH.w, is};
         /* a lane without an active home atom scans with NaN coordinates: no hits */
         R xi = qnan, yi = qnan, zi = qnan, Fpi = 0;
         int ki = 0;
@@ -664,7 +827,11 @@ static __device__ void b_cell(const KDesc &D, const int c)
                 const unsigned long long bal = __ballot(keep);
                 const int off = nt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
                                                                    __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
-                if (keep) T.put(off, x, y, z, NSCAN ? x * x + y * y + z * z : fpv, kj);
+                if (keep)
+                {
+                    T.put(off, x, y, z, NSCAN ? x * x + y * y + z * z : fpv, kj);
+                    if constexpr (sizeof(R) == sizeof(float)) T.g[off] = (unsigned short)g;
+                }
                 /* the lane that staged this lane's home atom (list entry 13 is the home
                    cell itself, not an image) hands over the tile position it wrote */
                 const int src = gself - gpos;
@@ -753,10 +920,19 @@ static __device__ void b_cell(const KDesc &D, const int c)
                     if (ds >= 0 && ds < ng * S && ds % S == 0) word &= ~(1ull << (ds / S));
                     if (record && wi < HBITS_WORDS) D.hbits[((long)c * HBITS_WORDS + wi) * WG + lane] = word;
                 }
+                unsigned long long xw = 0ull;   /* FP32: trips of this word whose pairs are in the exact shell */
                 if constexpr (CAT == CAT_EAM1)
-                    flush_eam1<R, PASS>(D, T, word, s, g0, S, xi, yi, zi, Fpi, (R)p1.rc2, acc);
+                    flush_eam1<R, PASS>(D, T, word, s, g0, S, xi, yi, zi, Fpi, p1.rc2, xw, acc);
                 else if constexpr ((CAT == CAT_LJ || CAT == CAT_MORSE) && PASS == 0)
-                    flush_pair2<R, CAT>(T, word, s, g0, S, xi, yi, zi, p1, acc);
+                {
+                    /* the rounding band is met so seldom that the word's bits are walked again to
+                       find its pairs rather than tracked pair by pair */
+                    const unsigned long long w0 = word;
+                    bool ex = false;
+                    flush_pair2<R, CAT>(D, T, word, s, g0, S, xi, yi, zi, p1, ex, acc);
+                    if constexpr (sizeof(R) == sizeof(float))
+                        if (__ballot(ex)) xw = band_bits(T, w0, s, g0, S, xi, yi, zi, (R)(p1.rc2 * (1.0 - RX_BAND)));
+                }
                 else
                     while (__ballot(word != 0ull))
                     {
@@ -767,12 +943,27 @@ static __device__ void b_cell(const KDesc &D, const int c)
                             const int m = s + (g0 + t) * S;
                             R x, y, z, fj;
                             T.load(m, x, y, z, fj);
-                            if (CAT == CAT_GEN)
-                                pair_hit<R, PASS, CAT>(D, ppi[T.k[m]], Fpi, fj, xi - x, yi - y, zi - z, acc);
-                            else
-                                pair_hit<R, PASS, CAT>(D, p1, Fpi, fj, xi - x, yi - y, zi - z, acc);
+                            const PairParam &pq = CAT == CAT_GEN ? ppi[T.k[m]] : p1;
+                            if constexpr (sizeof(R) == sizeof(float))
+                            {
+                                /* the exact shell: the rounding band, and for EAM the table's steep run */
+                                const R ex = xi - x, ey = yi - y, ez = zi - z;
+                                const R r2 = ex * ex + ey * ey + ez * ez;
+                                const int pc = CAT == CAT_GEN ? pq.cat : (CAT == CAT_EAM1 ? FMDX_POT_EAM : pq.cat);
+                                const double rx = pc == FMDX_POT_EAM ? D.eam.rx2 : pq.rc2 * (1.0 - RX_BAND);
+                                if (pc != FMDX_POT_NONE && r2 >= (R)rx)
+                                {
+                                    xw |= 1ull << t;
+                                    continue;
+                                }
+                            }
+                            pair_hit<R, PASS, CAT>(D, pq, Fpi, fj, xi - x, yi - y, zi - z, acc);
                         }
                     }
+                if constexpr (sizeof(R) == sizeof(float))
+                {
+                    if (__ballot(xw != 0ull)) flush_exact<PASS, CAT>(D, T, xw, s, g0, S, X, ppi, p1, acc);
+                }
                 wi++;
             }
         }

@@ -173,10 +173,16 @@ fmd_pot_t *fmd_pot_morse_apply(fmd_t *md, unsigned atomkind1, unsigned atomkind2
 
 /* ------------------------------------------------------------------- EAM */
 
-/* EAM_convert_r_to_r2, eam.c:324-341 */
+/* EAM_convert_r_to_r2, eam.c:324-341.  The resampling reaches the last knot, r = (Nr-1) dr,
+   where the spline evaluation (cspline.c:76-89, fmdh_spline_val) takes the interval above it and
+   reads one knot past the table with weight zero.  src and its second derivatives therefore carry
+   one zero entry past the table (load_setfl allocates src with Nr + 1 zeroed entries): the read
+   stays in bounds and adds an exact zero.  Whatever followed an unpadded array could be a NaN or
+   an infinity, and 0 x NaN in the last r^2 knot would spread through the natural spline's
+   solve to every second derivative of the table. */
 static void r_to_r2(const eam_t *e, const double *src, double *dst)
 {
-    double *dd = malloc(e->Nr * sizeof(double));
+    double *dd = calloc((size_t)e->Nr + 1, sizeof(double));
     fmdh_spline_prepare(e->dr, src, e->Nr, dd);
     for (int i = 0; i < e->Nr2; i++) dst[i] = fmdh_spline_val(e->dr, src, dd, sqrt(i * e->dr2));
     free(dd);
@@ -218,7 +224,7 @@ static eam_t *load_setfl(fmd_t *md, const char *path)
     if ((e->Nr - 1) * e->dr <= cutoff) goto corrupt;
     e->cutoff_sqr = cutoff * cutoff;
     e->dr2 = ((e->Nr - 1) * e->dr) * ((e->Nr - 1) * e->dr) / (e->Nr2 - 1);
-    tmp = malloc(e->Nr * sizeof(double));
+    tmp = calloc((size_t)e->Nr + 1, sizeof(double));   /* one zero past the table: see r_to_r2 */
     for (int i = 0; i < e->ElementsNo; i++)
     {
         eam_element_t *el = &e->el[i];

@@ -97,6 +97,8 @@ typedef struct
     unsigned bad_force_cell;
     unsigned bad_position_cell;
     unsigned replay_mismatch;
+    unsigned exact_unmatched;   /* mixed build: exact-shell pairs evaluated in FP32 because their FP64
+                                   positions were not found (always 0 unless the search fails) */
     /* filled by fmdx_audit: atoms counted over the cells, the fullest cell, and the atoms whose
        position, velocity or force is not finite */
     unsigned audit_atoms;

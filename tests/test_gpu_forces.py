@@ -100,17 +100,15 @@ def test_eam_two_elements(rng, tmp_path, precision):
 @pytest.mark.parametrize("pot", ["lj", "eam"])
 @pytest.mark.parametrize("precision", [F.FMDX_PREC_FP64, F.FMDX_PREC_MIXED])
 def test_pairs_at_the_cutoff(pot, precision, synth_setfl):
-    """Dimers just inside and just outside the cutoff.  The candidate scans select with a
-    margin above rc^2 (1e-12 relative in FP64, 2.5e-4 in FP32) and the pair evaluation
-    tests r^2 < rc^2 again; a dimer in that band outside rc must contribute nothing,
-    and LJ and the synthetic table are both nonzero at rc.
-
-    The mixed EAM bar is looser: the synthetic table ends at rc without a taper, so
-    its last r^2 interval (0.006 A^2 wide) falls from rho(rc) = 0.014 to the zero
-    knots the reference appends (eam.c:445), and there the FP32 rounding of the
-    positions (~2e-7 A) moves the interval fraction by ~2e-3; an isolated dimer
-    at rc gets all of its force from that interval."""
-    rel = 2e-13 if precision == F.FMDX_PREC_FP64 else 2e-5
+    """Dimers just inside and just outside the cutoff, 2e-13 of rc away from it.  The
+    candidate scans select with a margin above rc^2 (1e-12 relative in FP64, 2.5e-4 in
+    FP32) and the pair evaluation tests r^2 < rc^2 again; a dimer outside rc must
+    contribute nothing, and LJ and the synthetic table are both nonzero at rc.  The
+    mixed build evaluates pairs within 1e-5 of rc^2 (and, for the synthetic table,
+    over the steep run of spline intervals below rc^2 that its missing taper leaves)
+    in FP64 from the FP64 positions, so it keeps the same pair set and meets its 1e-5
+    bar on these dimers too, which get all of their force from those intervals."""
+    rel = 2e-13
     if pot == "lj":
         rc, pots, names, mass = AR["rc"], [("lj", 0, 0, AR["sig"], AR["eps"], AR["rc"])], ["Ar"], [AR["m"]]
     else:
@@ -138,7 +136,7 @@ def test_pairs_at_the_cutoff(pot, precision, synth_setfl):
     outside = np.isin(ids, [2 * j + b for j in range(8) if j % 2 == 1 for b in (0, 1)])
     assert np.all(ref["f"][outside] == 0.0)
     assert np.all(got["f"][outside] == 0.0)
-    tol = 1e-10 if precision == F.FMDX_PREC_FP64 else (1e-5 if pot == "lj" else 5e-3)
+    tol = 1e-10 if precision == F.FMDX_PREC_FP64 else 1e-5
     assert rel_err(got["f"], ref["f"], scale=fmax) < tol
 
 

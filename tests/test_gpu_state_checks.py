@@ -49,7 +49,7 @@ def test_audit_matches_the_downloaded_state(rng, cu01, precision):
     assert t.audit_max_cell == int(np.max(got["cell_count"]))
     assert t.audit_max_cell <= t.cap
     assert t.audit_nonfinite == 0
-    assert t.replay_mismatch == 0
+    assert t.replay_mismatch == 0 and t.exact_unmatched == 0
     assert t.bad_force_cell == 0 and t.bad_position_cell == 0
     # the atoms did move between cells, so the check saw re-binned cells
     assert not np.array_equal(got["atomid"], o.atoms()["atomid"])
@@ -65,7 +65,7 @@ def test_audit_on_the_synthetic_table_mixed(rng, synth_setfl):
         d.forces()
         d.vv_finish(2e-3)
     t = d.audit()
-    assert t.err == 0 and t.audit_nonfinite == 0 and t.replay_mismatch == 0
+    assert t.err == 0 and t.audit_nonfinite == 0 and t.replay_mismatch == 0 and t.exact_unmatched == 0
     assert t.audit_atoms == t.natoms == 4 * 7 ** 3
 
 
