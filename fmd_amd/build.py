@@ -29,7 +29,7 @@ ARCH = os.environ.get("FMD_OFFLOAD_ARCH", "gfx950")
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-munsafe-fp-atomics",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result", "-Wno-unused-value", "-I" + INC, "-I" + DEV_SRC]
-DEV_FILES = ["fmdx_api.hip", "fmdx_batch.hip", "k_step.hip", "k_force.hip", "k_ttm.hip"]
+DEV_FILES = ["fmdx_api.hip", "fmdx_batch.hip", "fmdx_noalloc.hip", "k_step.hip", "k_force.hip", "k_ttm.hip"]
 HOST_FILES = ["core.c", "potential.c", "matter.c", "state.c", "dync.c", "turi.c", "ext.c"]
 
 

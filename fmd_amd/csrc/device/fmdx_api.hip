@@ -22,6 +22,7 @@ extern "C" const char *fmdx_strerror(int code)
         case FMDX_ENOMEM: return "device allocation failed";
         case FMDX_ESTATE: return "call out of order";
         case FMDX_ENODEV: return "no GPU visible";
+        case FMDX_ESIZE: return "system larger than the fixed scratch of the allocation-free entry point";
     }
     return "unknown";
 }
@@ -478,7 +479,7 @@ extern "C" int fmdx_download(fmdx_system_t *s, unsigned *natoms, unsigned *cell_
     return FMDX_OK;
 }
 
-static TtmDev ttm_dev(fmdx_system_t *s, int on)
+TtmDev ttm_dev(fmdx_system_t *s, int on)
 {
     TtmDev T;
     memset(&T, 0, sizeof T);

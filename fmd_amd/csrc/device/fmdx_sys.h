@@ -72,6 +72,9 @@ static inline void set_step(KDesc &D, const double *mass, int ag, double dt, int
     for (int k = 0; k < MAXK; k++) D.hdtm[k] = dt * 0.5 / mass[k];
 }
 
+/* the TTM friction source of a force call (fmdx_api.hip) */
+TtmDev ttm_dev(fmdx_system_t *s, int on);
+
 /* pair-category specialisation of the force kernels for a system (k_force.hip) */
 int force_category(const KDesc &D);
 

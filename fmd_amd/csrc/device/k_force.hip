@@ -730,8 +730,9 @@ static __device__ void b_cell(const KDesc &D, const int c)
     const bool one_block = nh > 0 && nh <= WG;
     int hw = -1;
     if (PASS == 0 && D.has_eam) hw = __builtin_amdgcn_readfirstlane(D.htrips[c]);
-    if (PASS == 1 && lane == 0 && !one_block) D.htrips[c] = -1;
-    const bool record = PASS == 1 && one_block;
+    /* a density pass without hit-word storage (hbits null) records nothing: the force pass rescans */
+    const bool record = PASS == 1 && one_block && D.hbits != nullptr;
+    if (PASS == 1 && lane == 0 && !record) D.htrips[c] = -1;
     bool replay = PASS == 0 && one_block && hw >= 0;
     /* tile sizes of the density pass's first four rounds (16 bits each): the force pass replays
        a round only if it staged as many candidates, and rescans from the first round that differs */

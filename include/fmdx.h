@@ -47,7 +47,8 @@ enum
     FMDX_EARG = -2,      /* invalid argument */
     FMDX_ENOMEM = -3,    /* device allocation failed */
     FMDX_ESTATE = -4,    /* call out of order (e.g. no atoms uploaded) */
-    FMDX_ENODEV = -5     /* no GPU visible */
+    FMDX_ENODEV = -5,    /* no GPU visible */
+    FMDX_ESIZE = -6      /* beyond the fixed scratch of fmdx_update_forces_noalloc */
 };
 
 /* sticky device error bits */
@@ -163,6 +164,15 @@ int fmdx_ncells(fmdx_system_t *s, unsigned *n);
 
 /* step seams (asynchronous) */
 int fmdx_update_forces(fmdx_system_t *s, int active_group, int ttm_friction);
+/* fmdx_update_forces without any device allocation: no counterpart in the reference.  For callers
+   that cannot allocate (HIP graph capture, serving loops that forbid hipMalloc).  The call's
+   scratch -- per-cell energies, the density pass's tile-round records, energy partials -- lives in
+   one fixed 16 MiB device array per GPU instead of the system's buffers; the density pass keeps no
+   hit words, so the force pass rescans (same results, bit for bit).  Systems of more than
+   FMDX_NOALLOC_MAX_CELLS cells are refused with FMDX_ESIZE.  Every call on one GPU shares that
+   array: calls that overlap on one GPU (two streams, two threads) race. */
+#define FMDX_NOALLOC_MAX_CELLS 590000
+int fmdx_update_forces_noalloc(fmdx_system_t *s, int active_group, int ttm_friction);
 int fmdx_vv_start(fmdx_system_t *s, int active_group, double dt, int thermostat, double tau, double T0);
 int fmdx_vv_finish(fmdx_system_t *s, int active_group, double dt);
 int fmdx_se_step(fmdx_system_t *s, int active_group, double dt);
