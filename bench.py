@@ -146,27 +146,83 @@ def mixed_force_error(f, restore):
     return float(np.max(np.abs(d))) / scale, float(np.sqrt(np.mean(d * d))) / scale, above
 
 
-def eam_force_error(cells, setfl, seed=1):
-    """Mixed against FP64 forces on the EAM Cu benchmark state (the same recipe as the timed run)."""
+def device_atoms(dev):
+    """cell-ordered atoms of a device system (x, f, F'(rho) as aux, atom ids) through the C-ABI"""
+    fx = F.load_fmdx()
+    n = C.c_uint(0)
+    F._ck(fx.fmdx_download(dev, C.byref(n), None, None, None, None, None, None, None, None), "download")
+    n = n.value
+    x, f, aux, aid = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n), np.zeros(n, np.uint32)
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    F._ck(fx.fmdx_download(dev, C.byref(C.c_uint(0)), None, p(x), None, p(f), p(aux), None, p(aid), None), "download")
+    return dict(x=x, f=f, aux=aux, atomid=aid)
+
+
+def eam_parity(cells, setfl, seed=1):
+    """Parity of the full-size EAM Cu state (the recipe of the timed run, before any step): the
+    oracle (FP64 restatement of eam.c:29-146,225-322, one host thread) against the device's FP64
+    and mixed force passes -- per-atom forces over max|F|, F'(rho) (the reference's vaream), and the
+    potential energy -- and mixed against FP64 on the device (max, RMS, atoms over 1e-5)."""
     from fmd_amd import api
+    from oracle.oracle import Oracle
     a = W.CU["a"]
     rng = np.random.default_rng(seed)
     x = W.fcc_positions(cells, cells, cells, a)
     x = np.mod(x + rng.normal(0.0, 0.02, size=x.shape), cells * a)
     v = W.maxwell_velocities(len(x), 63.546, 600.0, rng)
+    L = cells * a
+    o = Oracle([L] * 3, [1, 1, 1], [63.546], ["Cu"])
+    o.load_eam(setfl)
+    o.apply_eam(0, 0)
+    o.make_grid()
+    o.add_atoms(x, v)
+    o.distribute()
+    t0 = time.perf_counter()
+    o.forces()
+    t_oracle = time.perf_counter() - t0
+    ref, eref = o.atoms(), o.energies()
+    o.close()
+    order_ref = np.argsort(ref["atomid"])
+    fmax = float(np.max(np.abs(ref["f"])))
     f = api.Fmd()
+    out = dict(natoms=len(x), oracle_seconds=t_oracle, oracle_threads=1)
     try:
-        f.fmd_box_setSize(cells * a, cells * a, cells * a)
+        f.fmd_box_setSize(L, L, L)
         f.fmd_box_setPBC(True, True, True)
         f.set_kinds(["Cu"], [63.546])
         f.fmd_pot_apply(0, 0, f.fmd_pot_eam_alloy_load(setfl.encode()))
         f.load_atoms(x, v)
         if not f.fmd_ext_distribute():
             raise RuntimeError("distribute failed")
-        emax, erms, above = mixed_force_error(f, F.FMDX_PREC_FP64)
+        fx = F.load_fmdx()
+        dev = C.c_void_p(f.fmd_ext_getDevice())
+        got = {}
+        for name, prec in (("fp64", F.FMDX_PREC_FP64), ("mixed", F.FMDX_PREC_MIXED)):
+            f.fmd_ext_setPrecision(prec)
+            F._ck(fx.fmdx_update_forces(dev, -1, 0), "forces")
+            th = F.Thermo()
+            F._ck(fx.fmdx_read_thermo(dev, C.byref(th)), "read_thermo")
+            d = device_atoms(dev)
+            od = np.argsort(d["atomid"])
+            if not np.array_equal(d["atomid"][od], ref["atomid"][order_ref]):
+                raise RuntimeError("atom sets differ")
+            df = d["f"][od] - ref["f"][order_ref]
+            out[name + "_vs_oracle"] = dict(
+                max_rel_force=float(np.max(np.abs(df))) / fmax,
+                max_rel_Fprime=float(np.max(np.abs(d["aux"][od] - ref["aux"][order_ref])) /
+                                     np.max(np.abs(ref["aux"]))),
+                rel_pe=abs(th.pe - eref["pe"]) / abs(eref["pe"]),
+                same_cell_order=bool(np.array_equal(d["atomid"], ref["atomid"])),
+                device_err_word=int(th.err))
+            got[name] = d["f"][od]
+        dm = got["mixed"] - got["fp64"]
+        scale = max(float(np.max(np.abs(got["fp64"]))), 1e-300)
+        out["mixed_vs_fp64"] = dict(max_rel=float(np.max(np.abs(dm))) / scale,
+                                    rms_rel=float(np.sqrt(np.mean(dm * dm))) / scale,
+                                    frac_atoms_over_1em5=float(np.mean(np.max(np.abs(dm), axis=1) > 1e-5 * scale)))
     finally:
         f.free()
-    return dict(natoms=len(x), max_rel=emax, rms_rel=erms, frac_atoms_over_1em5=above)
+    return out
 
 
 def batched_ensemble(nsys, steps, plain_sample, seed=3):
@@ -456,6 +512,7 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-parity", action="store_true", help="skip the full-size oracle comparison")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-collision measurement")
     ap.add_argument("--batch-systems", type=int, default=512)
@@ -494,10 +551,10 @@ def main():
             log("bench: other precision %s" % other)
         ens = srv = ttm = None
         drift = {}
-        ferr = None
-        if not args.no_mixed:
-            ferr = guarded("mixed force error", eam_force_error, args.cells, setfl)
-            log("bench: mixed force error %s" % ferr)
+        parity = None
+        if not args.no_parity:
+            parity = guarded("full-size parity", eam_parity, args.cells, setfl)
+            log("bench: full-size parity %s" % parity)
         if not args.no_drift:
             cu01 = os.path.join(ROOT, "tests", "golden", "data", "Cu01.eam.alloy")
             for pname, pv in (("fp64", F.FMDX_PREC_FP64), ("mixed", F.FMDX_PREC_MIXED)):
@@ -573,8 +630,10 @@ def main():
                                   "value": other["natoms"] * other["steps"] / (other["gpu_ms"] * 1e-3),
                                   "ms_per_step": other["gpu_ms"] / other["steps"],
                                   "pass0_ms": other["pass0_ms"], "pass1_ms": other["pass1_ms"]}
-    if ferr is not None:
-        out["force_err_mixed_vs_fp64"] = ferr
+    if parity is not None:
+        out["parity"] = parity
+        if "mixed_vs_fp64" in parity:
+            out["force_err_mixed_vs_fp64"] = dict(natoms=parity["natoms"], **parity["mixed_vs_fp64"])
     if drift:
         out["nve_drift"] = drift
     if ttm is not None:
