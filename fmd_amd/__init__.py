@@ -20,6 +20,7 @@ ROOT = os.path.dirname(HERE)
 FMDX_OK = 0
 FMDX_EHIP, FMDX_EARG, FMDX_ENOMEM, FMDX_ESTATE, FMDX_ENODEV, FMDX_ESIZE = -1, -2, -3, -4, -5, -6
 FMDX_NOALLOC_MAX_CELLS = 590000
+FMDX_FORCES_ENERGY = 1
 FMDX_PREC_FP64 = 0
 FMDX_PREC_MIXED = 1
 POT_LJ, POT_EAM, POT_MORSE = 1, 2, 3
@@ -87,6 +88,7 @@ def load_fmdx():
         lib.fmdx_ncells.argtypes = [P, C.POINTER(C.c_uint)]
         lib.fmdx_update_forces.argtypes = [P, C.c_int, C.c_int]
         lib.fmdx_update_forces_noalloc.argtypes = [P, C.c_int, C.c_int]
+        lib.fmdx_update_forces_ex.argtypes = [P, C.c_int, C.c_int, C.c_int]
         lib.fmdx_vv_start.argtypes = [P, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double]
         lib.fmdx_vv_finish.argtypes = [P, C.c_int, C.c_double]
         lib.fmdx_se_step.argtypes = [P, C.c_int, C.c_double]

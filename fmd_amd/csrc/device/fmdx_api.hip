@@ -95,6 +95,7 @@ extern "C" int fmdx_system_create(fmdx_system_t **out, const double l[3], const 
     D.h0 = cellh[0]; D.h1 = cellh[1]; D.h2 = cellh[2];
     D.nkinds = nkinds;
     D.single_cat = -1;
+    D.energy = 1;
     for (int k = 0; k < MAXK; k++)
     {
         D.mass[k] = k < nkinds ? mass[k] : 1.0;
@@ -147,7 +148,7 @@ extern "C" int fmdx_system_destroy(fmdx_system_t *s)
     dfree(s->dpairs);
     dfree(s->Fc); dfree(s->rhoc); dfree(s->phic);
     dfree(s->Fc32); dfree(s->rhoc32); dfree(s->phic32);
-    dfree(s->rp); dfree(s->rp32);
+    dfree(s->rf); dfree(s->rf32);
     for (int i = 0; i < s->nttm_mem; i++) hipFree(s->ttm_mem[i]);
     for (int i = 0; i < s->nev; i++) hipEventDestroy(s->ev[i]);
     for (int i = 0; i < NPROF * 3; i++)
@@ -275,18 +276,24 @@ extern "C" int fmdx_set_eam(fmdx_system_t *s, int nelem, int nrho, double drho, 
             b[i] = make_float4((float)a[i].x, (float)a[i].y, (float)a[i].z, (float)a[i].w);
     };
     cvt(hF, fF); cvt(hR, fR); cvt(hP, fP);
-    /* element 0: rho and phi coefficients of one interval side by side, so that a
-       single-element pair fetches one 64-byte (FP64) or 32-byte (FP32) record */
-    std::vector<double4> hRP(2 * (size_t)m2);
-    for (long k = 0; k < m2; k++) { hRP[2 * k] = hR[k]; hRP[2 * k + 1] = hP[k]; }
-    std::vector<float4> fRP(hRP.size());
-    cvt(hRP, fRP);
+    /* element 0's force record per interval: the derivative coefficients of rho and phi
+       first, the values last (a force pass without energy reads the first 48 / 24 bytes) */
+    std::vector<double4> hRF(2 * (size_t)m2);
+    std::vector<float4> fRF(2 * (size_t)m2);
+    for (long k = 0; k < m2; k++)
+    {
+        const double4 r = hR[k], p = hP[k];
+        hRF[2 * k] = make_double4(r.y, r.z, r.w, p.y);
+        hRF[2 * k + 1] = make_double4(p.z, p.w, r.x, p.x);
+        fRF[2 * k] = make_float4((float)r.y, (float)r.z, (float)r.w, (float)p.y);
+        fRF[2 * k + 1] = make_float4((float)p.z, (float)p.w, (float)r.x, (float)p.x);
+    }
     /* tables of an earlier call may still be read by work queued on the stream */
     CK(hipStreamSynchronize(s->st));
     dfree(s->Fc); dfree(s->rhoc); dfree(s->phic); dfree(s->Fc32); dfree(s->rhoc32); dfree(s->phic32);
-    dfree(s->rp); dfree(s->rp32);
+    dfree(s->rf); dfree(s->rf32);
     int r = 0;
-    r |= dalloc(&s->rp, hRP.size()); r |= dalloc(&s->rp32, fRP.size());
+    r |= dalloc(&s->rf, hRF.size()); r |= dalloc(&s->rf32, fRF.size());
     r |= dalloc(&s->Fc, hF.size()); r |= dalloc(&s->rhoc, hR.size()); r |= dalloc(&s->phic, hP.size());
     r |= dalloc(&s->Fc32, fF.size()); r |= dalloc(&s->rhoc32, fR.size()); r |= dalloc(&s->phic32, fP.size());
     if (r) return FMDX_ENOMEM;
@@ -296,8 +303,8 @@ extern "C" int fmdx_set_eam(fmdx_system_t *s, int nelem, int nrho, double drho, 
     CK(hipMemcpyAsync(s->Fc32, fF.data(), fF.size() * sizeof(float4), hipMemcpyHostToDevice, s->st));
     CK(hipMemcpyAsync(s->rhoc32, fR.data(), fR.size() * sizeof(float4), hipMemcpyHostToDevice, s->st));
     CK(hipMemcpyAsync(s->phic32, fP.data(), fP.size() * sizeof(float4), hipMemcpyHostToDevice, s->st));
-    CK(hipMemcpyAsync(s->rp, hRP.data(), hRP.size() * sizeof(double4), hipMemcpyHostToDevice, s->st));
-    CK(hipMemcpyAsync(s->rp32, fRP.data(), fRP.size() * sizeof(float4), hipMemcpyHostToDevice, s->st));
+    CK(hipMemcpyAsync(s->rf, hRF.data(), hRF.size() * sizeof(double4), hipMemcpyHostToDevice, s->st));
+    CK(hipMemcpyAsync(s->rf32, fRF.data(), fRF.size() * sizeof(float4), hipMemcpyHostToDevice, s->st));
     EamDev &E = s->D.eam;
     E.ne = nelem; E.nrho = nrho; E.nr2 = nr2;
     E.drho = drho; E.inv_drho = 1.0 / drho; E.dr2 = dr2; E.inv_dr2 = 1.0 / dr2; E.rc2 = rc2;
@@ -311,7 +318,7 @@ extern "C" int fmdx_set_eam(fmdx_system_t *s, int nelem, int nrho, double drho, 
     }
     E.Fc = s->Fc; E.rhoc = s->rhoc; E.phic = s->phic;
     E.Fc32 = s->Fc32; E.rhoc32 = s->rhoc32; E.phic32 = s->phic32;
-    E.rp = s->rp; E.rp32 = s->rp32;
+    E.rf = s->rf; E.rf32 = s->rf32;
     /* the kernels run on the system stream: the tables have landed before any of them reads */
     CK(hipStreamSynchronize(s->st));
     /* hit words kept between the two EAM passes (one bit per lane and trip) */
@@ -500,9 +507,16 @@ TtmDev ttm_dev(fmdx_system_t *s, int on)
 
 extern "C" int fmdx_update_forces(fmdx_system_t *s, int active_group, int ttm_friction)
 {
+    return fmdx_update_forces_ex(s, active_group, ttm_friction, FMDX_FORCES_ENERGY);
+}
+
+extern "C" int fmdx_update_forces_ex(fmdx_system_t *s, int active_group, int ttm_friction, int flags)
+{
     if (!s || !s->uploaded) return FMDX_ESTATE;
     if (s->D.has_eam && !s->rhoc) return FMDX_ESTATE;
+    if (flags & ~FMDX_FORCES_ENERGY) return FMDX_EARG;
     s->D.ag = active_group;
+    s->D.energy = (flags & FMDX_FORCES_ENERGY) ? 1 : 0;
     s->D.ttm = ttm_dev(s, ttm_friction);
     if (s->profile && s->prof_n < NPROF)
     {
@@ -512,7 +526,7 @@ extern "C" int fmdx_update_forces(fmdx_system_t *s, int active_group, int ttm_fr
         CK(hipEventRecord(e[1], s->st));
         CK(launch_pass0(s->D, s->st, s->precision));
         CK(hipEventRecord(e[2], s->st));
-        CK(launch_reduce_pe(s->D, s->st));
+        if (s->D.energy) CK(launch_reduce_pe(s->D, s->st));
         s->prof_n++;
     }
     else

@@ -291,6 +291,7 @@ extern "C" int fmdx_batch_add(fmdx_batch_t *b, fmdx_system_t *s, double dt, int 
     if (r) return r;
     CK(hipStreamSynchronize(s->st));
     set_step(s->D, s->mass, -1, dt, thermostat, tau, T0);
+    s->D.energy = 1;   /* every batched step sums the energies (the status record carries them) */
     s->member = 1;
     b->sys.push_back(s);
     b->dirty = 1;

@@ -51,8 +51,10 @@ struct EamDev
     const double4 *rhoc;   /* [ne][nr2-1]  */
     const double4 *phic;   /* [ne*ne][nr2-1] */
     const float4 *Fc32, *rhoc32, *phic32;
-    const double4 *rp;     /* [2*(nr2-1)] element 0: rho, phi coefficients per interval */
-    const float4 *rp32;
+    /* element 0's force record per interval (k_force.hip, FRec): FP64 2 x double4,
+       (rho1 rho2 rho3 phi1)(phi2 phi3 rho0 phi0); FP32 2 x float4, same order */
+    const double4 *rf;
+    const float4 *rf32;
 };
 
 struct TtmDev
@@ -118,6 +120,7 @@ struct KDesc
     /* step parameters, set by the host before a launch */
     int ag;                /* active group, -1 = every atom */
     int thermostat;        /* Berendsen rescale in the start step */
+    int energy;            /* the force call also sums the potential energy */
     double dt, tau, T0;
     double hdtm[MAXK];     /* dt * 0.5 / mass per kind (integrators.c:67) */
     TtmDev ttm;            /* TTM friction source (forces.c:292-330) */

@@ -79,6 +79,7 @@ extern "C" int fmdx_update_forces_noalloc(fmdx_system_t *s, int active_group, in
         D.htrips = (int *)base;
         D.hbits = nullptr;
         D.ag = active_group;
+        D.energy = 1;
         D.ttm = ttm_dev(s, ttm_friction);
         s->D.ag = active_group;
         if (launch_forces(D, s->st, s->precision) != hipSuccess) r = FMDX_EHIP;

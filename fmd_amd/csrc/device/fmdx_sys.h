@@ -22,8 +22,8 @@ struct fmdx_system
     PairParam hp[MAXK * MAXK];
     PairParam *dpairs;
     /* EAM */
-    double4 *Fc, *rhoc, *phic, *rp;
-    float4 *Fc32, *rhoc32, *phic32, *rp32;
+    double4 *Fc, *rhoc, *phic, *rf;
+    float4 *Fc32, *rhoc32, *phic32, *rf32;
     /* TTM */
     int ttm_on;
     TtmState T;

@@ -69,7 +69,7 @@ template <> struct Prec<double>
     typedef double4 Q;
     static __device__ __forceinline__ const double4 *rho(const EamDev &E) { return E.rhoc; }
     static __device__ __forceinline__ const double4 *phi(const EamDev &E) { return E.phic; }
-    static __device__ __forceinline__ const double4 *rp(const EamDev &E) { return E.rp; }
+    static __device__ __forceinline__ const void *rf(const EamDev &E) { return E.rf; }
     static __device__ __forceinline__ const double *px(const KDesc &D) { return D.x; }
     static __device__ __forceinline__ const double *py(const KDesc &D) { return D.y; }
     static __device__ __forceinline__ const double *pz(const KDesc &D) { return D.z; }
@@ -80,7 +80,7 @@ template <> struct Prec<float>
     typedef float4 Q;
     static __device__ __forceinline__ const float4 *rho(const EamDev &E) { return E.rhoc32; }
     static __device__ __forceinline__ const float4 *phi(const EamDev &E) { return E.phic32; }
-    static __device__ __forceinline__ const float4 *rp(const EamDev &E) { return E.rp32; }
+    static __device__ __forceinline__ const void *rf(const EamDev &E) { return E.rf32; }
     static __device__ __forceinline__ const float *px(const KDesc &D) { return D.xr; }
     static __device__ __forceinline__ const float *py(const KDesc &D) { return D.yr; }
     static __device__ __forceinline__ const float *pz(const KDesc &D) { return D.zr; }
@@ -269,7 +269,7 @@ static __device__ __forceinline__ void morse_terms(const PairParam &p, R r2, R &
 }
 
 /* one candidate pair inside the search radius; acc = {fx, fy, fz, energy or density} */
-template <typename R, int PASS, int CAT, typename A>
+template <typename R, int PASS, int CAT, typename A, bool EN = true>
 static __device__ __forceinline__ void pair_hit(const KDesc &D, const PairParam &p, R Fpi, R Fpj, R dx, R dy,
                                                 R dz, A *acc)
 {
@@ -287,8 +287,7 @@ static __device__ __forceinline__ void pair_hit(const KDesc &D, const PairParam 
         int k = (int)t;
         if (k > E.nr2 - 2) k = E.nr2 - 2;
         const R b = t - (R)k;
-        const Q q = CAT == CAT_EAM1 ? Prec<R>::rp(E)[2 * (long)k]
-                                    : Prec<R>::rho(E)[(long)p.jloc * (E.nr2 - 1) + k];
+        const Q q = Prec<R>::rho(E)[(long)p.jloc * (E.nr2 - 1) + k];
         acc[3].add(sp_val<R, Q>(q, b));
         return;
     }
@@ -299,29 +298,17 @@ static __device__ __forceinline__ void pair_hit(const KDesc &D, const PairParam 
         int k = (int)t;
         if (k > E.nr2 - 2) k = E.nr2 - 2;
         const R b = t - (R)k;
-        R rho_ip, rho_jp;
-        Q q;
-        if (CAT == CAT_EAM1)
-        {
-            const Q *rec = Prec<R>::rp(E) + 2 * (long)k;
-            rho_ip = sp_der<R, Q>(rec[0], b, inv);
-            rho_jp = rho_ip;
-            q = rec[1];
-        }
-        else
-        {
-            const long m = E.nr2 - 1;
-            const Q *rt = Prec<R>::rho(E);
-            rho_ip = sp_der<R, Q>(rt[(long)p.iloc * m + k], b, inv);
-            rho_jp = (p.jloc == p.iloc) ? rho_ip : sp_der<R, Q>(rt[(long)p.jloc * m + k], b, inv);
-            q = Prec<R>::phi(E)[(long)(p.iloc * E.ne + p.jloc) * m + k];
-        }
+        const long m = E.nr2 - 1;
+        const Q *rt = Prec<R>::rho(E);
+        const R rho_ip = sp_der<R, Q>(rt[(long)p.iloc * m + k], b, inv);
+        const R rho_jp = (p.jloc == p.iloc) ? rho_ip : sp_der<R, Q>(rt[(long)p.jloc * m + k], b, inv);
+        const Q q = Prec<R>::phi(E)[(long)(p.iloc * E.ne + p.jloc) * m + k];
         const R phid = sp_der<R, Q>(q, b, inv);
         const R mag = (R)2 * (Fpi * rho_jp + Fpj * rho_ip + phid);
         acc[0].add(-mag * dx);
         acc[1].add(-mag * dy);
         acc[2].add(-mag * dz);
-        acc[3].add((R)0.5 * sp_val<R, Q>(q, b));
+        if (EN) acc[3].add((R)0.5 * sp_val<R, Q>(q, b));
     }
     else if (cat == FMDX_POT_LJ || cat == FMDX_POT_MORSE)
     {
@@ -331,7 +318,7 @@ static __device__ __forceinline__ void pair_hit(const KDesc &D, const PairParam 
         acc[0].add(fac * dx);
         acc[1].add(fac * dy);
         acc[2].add(fac * dz);
-        acc[3].add(en);
+        if (EN) acc[3].add(en);
     }
 }
 
@@ -386,8 +373,8 @@ static __device__ __forceinline__ long exact_sep(const KDesc &D, const ExactCtx 
 }
 
 /* one exact single-element EAM pair (eam.h:270-310 for the density, eam.h:177-228 for the force) */
-template <int PASS, typename A>
-static __device__ void exact_eam1(const KDesc &D, const ExactCtx &X, int g, double rc2, A *acc)
+template <int PASS, bool EN, typename A>
+static __device__ __forceinline__ void exact_eam1(const KDesc &D, const ExactCtx &X, int g, double rc2, A *acc)
 {
     double dx, dy, dz;
     const long js = exact_sep(D, X, g, dx, dy, dz);
@@ -398,48 +385,95 @@ static __device__ void exact_eam1(const KDesc &D, const ExactCtx &X, int g, doub
     int k = (int)t;
     if (k > E.nr2 - 2) k = E.nr2 - 2;
     const double b = t - (double)k;
-    const double4 qr = E.rp[2 * (long)k];
+    const double4 qr = E.rhoc[k];
     if (PASS == 1)
     {
         acc[3].add((float)sp_val<double, double4>(qr, b));
         return;
     }
-    const double4 qp = E.rp[2 * (long)k + 1];
+    const double4 qp = E.phic[k];
     const double rho = sp_der<double, double4>(qr, b, E.inv_dr2);
     const double ph = sp_der<double, double4>(qp, b, E.inv_dr2);
     const double mag = 2.0 * (D.aux[X.is] * rho + D.aux[js] * rho + ph);
     acc[0].add((float)(-mag * dx));
     acc[1].add((float)(-mag * dy));
     acc[2].add((float)(-mag * dz));
-    acc[3].add((float)(0.5 * sp_val<double, double4>(qp, b)));
+    if (EN) acc[3].add((float)(0.5 * sp_val<double, double4>(qp, b)));
 }
 
-/* Coefficient records through a buffer resource over the rp table: a lane
-   without a pair passes an offset beyond the table, which the buffer unit
-   answers with zeros and no memory access, so the gathers stay branch-free
-   while only the lanes with a pair fetch. */
-/* The 128-bit result is reinterpreted as a whole vector: converting its
-   elements one by one (bit_cast of each lane of the returned vector) has been
-   compiled into a single 32-bit load whose value fills every component. */
+/* Coefficient records through a buffer resource: a lane without a pair passes
+   an offset beyond the table, which the buffer unit answers with zeros and no
+   memory access, so the gathers stay branch-free while only the lanes with a
+   pair fetch.  The 128-bit results are reinterpreted as whole vectors:
+   converting their elements one by one (bit_cast of each lane of the returned
+   vector) has been compiled into a single 32-bit load whose value fills every
+   component. */
 typedef unsigned U4 __attribute__((ext_vector_type(4)));
+typedef unsigned U2 __attribute__((ext_vector_type(2)));
 typedef float F4 __attribute__((ext_vector_type(4)));
+typedef float F2 __attribute__((ext_vector_type(2)));
 typedef double D2 __attribute__((ext_vector_type(2)));
 static __device__ __forceinline__ __amdgpu_buffer_rsrc_t rp_rsrc(const void *base, unsigned bytes)
 {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, (int)bytes, 0x00020000);
 }
-static __device__ __forceinline__ double4 rec_at(__amdgpu_buffer_rsrc_t r, unsigned off, double4 *)
+static __device__ __forceinline__ D2 ld2d(__amdgpu_buffer_rsrc_t r, unsigned off)
 {
-    const D2 a = __builtin_bit_cast(D2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
-    const D2 b = __builtin_bit_cast(D2, __builtin_amdgcn_raw_buffer_load_b128(r, off + 16u, 0, 0));
+    return __builtin_bit_cast(D2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+static __device__ __forceinline__ F4 ld4f(__amdgpu_buffer_rsrc_t r, unsigned off)
+{
+    return __builtin_bit_cast(F4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+static __device__ __forceinline__ F2 ld2f(__amdgpu_buffer_rsrc_t r, unsigned off)
+{
+    return __builtin_bit_cast(F2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+/* one interval of the density table (c0..c3) */
+static __device__ __forceinline__ double4 rho_at(__amdgpu_buffer_rsrc_t r, unsigned off, double4 *)
+{
+    const D2 a = ld2d(r, off), b = ld2d(r, off + 16u);
     return make_double4(a.x, a.y, b.x, b.y);
 }
-static __device__ __forceinline__ float4 rec_at(__amdgpu_buffer_rsrc_t r, unsigned off, float4 *)
+static __device__ __forceinline__ float4 rho_at(__amdgpu_buffer_rsrc_t r, unsigned off, float4 *)
 {
-    const F4 a = __builtin_bit_cast(F4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    const F4 a = ld4f(r, off);
     return make_float4(a.x, a.y, a.z, a.w);
 }
 #define RP_NONE 0x80000000u   /* an offset beyond any table */
+
+/* Force record of one interval of a single-element table: the derivative
+   coefficients of rho and phi first, the values last, so that a force pass
+   without energy reads 48 of the 64 bytes (FP64: three 16-byte loads instead
+   of four) and the FP32 pass 24 of 32 (a 16- and an 8-byte load):
+     FP64  (rho1 rho2)(rho3 phi1)(phi2 phi3)(rho0 phi0)
+     FP32  (rho1 rho2 rho3 phi1)(phi2 phi3 rho0 phi0)            */
+template <typename R> struct FRec { R r1, r2, r3, p1, p2, p3, p0; };
+template <bool EN>
+static __device__ __forceinline__ FRec<double> frec_at(__amdgpu_buffer_rsrc_t r, unsigned off, double *)
+{
+    const D2 a = ld2d(r, off), b = ld2d(r, off + 16u), c = ld2d(r, off + 32u);
+    FRec<double> q = {a.x, a.y, b.x, b.y, c.x, c.y, 0.0};
+    if (EN) q.p0 = ld2d(r, off + 48u).y;
+    return q;
+}
+template <bool EN>
+static __device__ __forceinline__ FRec<float> frec_at(__amdgpu_buffer_rsrc_t r, unsigned off, float *)
+{
+    const F4 a = ld4f(r, off);
+    FRec<float> q = {a.x, a.y, a.z, a.w, 0.0f, 0.0f, 0.0f};
+    if (EN)
+    {
+        const F4 b = ld4f(r, off + 16u);
+        q.p2 = b.x; q.p3 = b.y; q.p0 = b.w;
+    }
+    else
+    {
+        const F2 b = ld2f(r, off + 16u);
+        q.p2 = b.x; q.p3 = b.y;
+    }
+    return q;
+}
 
 /* Single-element EAM: the set bits of a hit word, NP per iteration and
    without branches, so that the coefficient gathers of NP pairs are in
@@ -452,13 +486,15 @@ static __device__ __forceinline__ float4 rec_at(__amdgpu_buffer_rsrc_t r, unsign
    is 8 % slower; 3 or 4 held at 4 waves, by re-reading the tile after the
    gathers, gain nothing: the FP64 passes are bound by the gathers'
    throughput), FP32 takes 3 in the force pass and 4 in the density pass
-   (FMDX_EAM_NP*, fmdx_tuned.h). */
-template <typename R, int PASS, typename TileT, typename A>
+   (FMDX_EAM_NP*, fmdx_tuned.h).  The density pass reads element 0's density
+   table (32 or 16 bytes per interval), the force pass its force records; EN
+   selects whether the force pass also takes the pair energy. */
+template <typename R, int PASS, bool EN, typename TileT, typename A>
 static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T, unsigned long long word, int s,
                                                   int g0, int S, R xi, R yi, R zi, R Fpi, double rc2,
                                                   unsigned long long &xw, A *acc)
 {
-    constexpr bool EX = sizeof(R) == sizeof(float);   /* FP32: pairs of the exact shell go to *xw */
+    constexpr bool EX = sizeof(R) == sizeof(float);   /* FP32: pairs of the exact shell go to xw */
     const R rx2 = EX ? (R)D.eam.rx2 : (R)rc2;
     typedef typename Prec<R>::Q Q;
     constexpr int NP = sizeof(R) == sizeof(double) ? (PASS == 0 ? FMDX_EAM_NP64_0 : FMDX_EAM_NP64_1)
@@ -466,8 +502,10 @@ static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T
     const EamDev &E = D.eam;
     const R inv = (R)E.inv_dr2;
     const int kmax = E.nr2 - 2;
-    const unsigned rec = (unsigned)sizeof(Q);   /* one of the two halves of an interval's record */
-    const __amdgpu_buffer_rsrc_t rb = rp_rsrc(Prec<R>::rp(E), (unsigned)(E.nr2 - 1) * 2u * rec);
+    /* bytes per interval: density record (Q), force record (2 Q) */
+    const unsigned rec = PASS == 1 ? (unsigned)sizeof(Q) : 2u * (unsigned)sizeof(Q);
+    const __amdgpu_buffer_rsrc_t rb = rp_rsrc(PASS == 1 ? (const void *)Prec<R>::rho(E) : Prec<R>::rf(E),
+                                              (unsigned)(E.nr2 - 1) * rec);
     while (__ballot(word != 0ull))
     {
         R dx[NP], dy[NP], dz[NP], fj[NP], b[NP];
@@ -491,13 +529,13 @@ static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T
             const R tt = r2 * inv;
             const int k = min((int)tt, kmax);
             b[j] = tt - (R)k;
-            o[j] = u[j] ? (unsigned)k * 2u * rec : RP_NONE;
+            o[j] = u[j] ? (unsigned)k * rec : RP_NONE;
         }
         if (PASS == 1)
         {
             Q q[NP];
 #pragma unroll
-            for (int j = 0; j < NP; j++) q[j] = rec_at(rb, o[j], (Q *)nullptr);
+            for (int j = 0; j < NP; j++) q[j] = rho_at(rb, o[j], (Q *)nullptr);
 #pragma unroll
             for (int j = 0; j < NP; j++)
             {
@@ -507,24 +545,25 @@ static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T
         }
         else
         {
-            Q qr[NP], qp[NP];
+            FRec<R> q[NP];
+#pragma unroll
+            for (int j = 0; j < NP; j++) q[j] = frec_at<EN>(rb, o[j], (R *)nullptr);
 #pragma unroll
             for (int j = 0; j < NP; j++)
             {
-                qr[j] = rec_at(rb, o[j], (Q *)nullptr);
-                qp[j] = rec_at(rb, o[j] + rec, (Q *)nullptr);
-            }
-#pragma unroll
-            for (int j = 0; j < NP; j++)
-            {
-                const R rho = sp_der<R, Q>(qr[j], b[j], inv);
-                const R ph = sp_der<R, Q>(qp[j], b[j], inv);
+                const FRec<R> &c = q[j];
+                const R bb = b[j];
+                const R rho = (c.r1 + bb * ((R)2 * c.r2 + (R)3 * bb * c.r3)) * inv;
+                const R ph = (c.p1 + bb * ((R)2 * c.p2 + (R)3 * bb * c.p3)) * inv;
                 const R mag = (R)2 * (Fpi * rho + fj[j] * rho + ph);
-                const R e = (R)0.5 * sp_val<R, Q>(qp[j], b[j]);
                 acc[0].add(u[j] ? -mag * dx[j] : (R)0);
                 acc[1].add(u[j] ? -mag * dy[j] : (R)0);
                 acc[2].add(u[j] ? -mag * dz[j] : (R)0);
-                acc[3].add(u[j] ? e : (R)0);
+                if (EN)
+                {
+                    const R e = (R)0.5 * (c.p0 + bb * (c.p1 + bb * (c.p2 + bb * c.p3)));
+                    acc[3].add(u[j] ? e : (R)0);
+                }
             }
         }
     }
@@ -534,7 +573,7 @@ static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T
    without branches, as flush_eam1 does; a slot without a pair evaluates the
    formulas at r^2 = rc^2 and adds exact zeros. */
 /* one exact single-kind LJ or Morse pair (lj.h:26-58, morse.h:26-58) */
-template <int CAT, typename A>
+template <int CAT, bool EN, typename A>
 static __device__ __forceinline__ void exact_pair1(const KDesc &D, const ExactCtx &X, int g, const PairParam &p, A *acc)
 {
     double dx, dy, dz;
@@ -547,12 +586,12 @@ static __device__ __forceinline__ void exact_pair1(const KDesc &D, const ExactCt
     acc[0].add((float)(fac * dx));
     acc[1].add((float)(fac * dy));
     acc[2].add((float)(fac * dz));
-    acc[3].add((float)en);
+    if (EN) acc[3].add((float)en);
 }
 
 /* one exact pair of any category (the generic dispatch of pair_hit in FP64) */
-template <int PASS, typename A>
-static __device__ void exact_pair(const KDesc &D, const ExactCtx &X, int g, const PairParam &p, A *acc)
+template <int PASS, bool EN, typename A>
+static __device__ __forceinline__ void exact_pair(const KDesc &D, const ExactCtx &X, int g, const PairParam &p, A *acc)
 {
     double dx, dy, dz;
     const long js = exact_sep(D, X, g, dx, dy, dz);
@@ -560,11 +599,11 @@ static __device__ void exact_pair(const KDesc &D, const ExactCtx &X, int g, cons
     for (int q = 0; q < 4; q++) t[q].zero();
     const double Fpi = (PASS == 0 && D.has_eam) ? D.aux[X.is] : 0.0;
     const double Fpj = (PASS == 0 && D.has_eam) ? D.aux[js] : 0.0;
-    pair_hit<double, PASS, CAT_GEN>(D, p, Fpi, Fpj, dx, dy, dz, t);
+    pair_hit<double, PASS, CAT_GEN, AccD, EN>(D, p, Fpi, Fpj, dx, dy, dz, t);
     for (int q = 0; q < 4; q++) acc[q].add((float)t[q].get());
 }
 
-template <typename R, int CAT, typename TileT, typename A>
+template <typename R, int CAT, bool EN, typename TileT, typename A>
 static __device__ __forceinline__ void flush_pair2(const KDesc &D, const TileT &T, unsigned long long word, int s,
                                                    int g0, int S, R xi, R yi, R zi, const PairParam &p,
                                                    bool &ex, A *acc)
@@ -597,11 +636,11 @@ static __device__ __forceinline__ void flush_pair2(const KDesc &D, const TileT &
         acc[0].add(u1 ? fac1 * dx1 : (R)0);
         acc[1].add(u1 ? fac1 * dy1 : (R)0);
         acc[2].add(u1 ? fac1 * dz1 : (R)0);
-        acc[3].add(u1 ? en1 : (R)0);
+        if (EN) acc[3].add(u1 ? en1 : (R)0);
         acc[0].add(u2 ? fac2 * dx2 : (R)0);
         acc[1].add(u2 ? fac2 * dy2 : (R)0);
         acc[2].add(u2 ? fac2 * dz2 : (R)0);
-        acc[3].add(u2 ? en2 : (R)0);
+        if (EN) acc[3].add(u2 ? en2 : (R)0);
     }
 }
 
@@ -625,7 +664,7 @@ static __device__ unsigned long long band_bits(const TileT &T, unsigned long lon
 
 /* the pairs of the exact shell collected from one hit word (bit t: trip t of
    this lane's stripe), evaluated in FP64 after the word's FP32 flush */
-template <int PASS, int CAT, typename TileT, typename A>
+template <int PASS, int CAT, bool EN, typename TileT, typename A>
 static __device__ __forceinline__ void flush_exact(const KDesc &D, const TileT &T, unsigned long long xw, int s, int g0, int S,
                                    const ExactCtx &X, const PairParam *ppi, const PairParam &p1, A *acc)
 {
@@ -636,14 +675,16 @@ static __device__ __forceinline__ void flush_exact(const KDesc &D, const TileT &
             const int t = __builtin_ctzll(xw);
             xw &= xw - 1ull;
             const int m = s + (g0 + t) * S;
-            if constexpr (CAT == CAT_EAM1) exact_eam1<PASS>(D, X, T.g[m], p1.rc2, acc);
-            else if constexpr (CAT == CAT_LJ || CAT == CAT_MORSE) exact_pair1<CAT>(D, X, T.g[m], p1, acc);
-            else exact_pair<PASS>(D, X, T.g[m], ppi[T.k[m]], acc);
+            if constexpr (CAT == CAT_EAM1) exact_eam1<PASS, EN>(D, X, T.g[m], p1.rc2, acc);
+            else if constexpr (CAT == CAT_LJ || CAT == CAT_MORSE) exact_pair1<CAT, EN>(D, X, T.g[m], p1, acc);
+            else exact_pair<PASS, EN>(D, X, T.g[m], ppi[T.k[m]], acc);
         }
     }
 }
 
-template <typename R, int PASS, int CAT>
+/* EN: the force pass also sums the pair energy into cell_pe (the density pass always keeps the
+   embedding energy of each cell, a by-product of F'(rho)) */
+template <typename R, int PASS, int CAT, bool EN>
 static __device__ void b_cell(const KDesc &D, const int c)
 {
     typedef typename AccOf<R>::T Acc;
@@ -657,7 +698,7 @@ static __device__ void b_cell(const KDesc &D, const int c)
         if (threadIdx.x == 0)
         {
             if (PASS == 1) { D.cell_emb[c] = 0.0; D.htrips[c] = -1; }
-            else D.cell_pe[c] = 0.0;
+            else if (EN) D.cell_pe[c] = 0.0;
         }
         return;
     }
@@ -923,14 +964,14 @@ static __device__ void b_cell(const KDesc &D, const int c)
                 }
                 unsigned long long xw = 0ull;   /* FP32: trips of this word whose pairs are in the exact shell */
                 if constexpr (CAT == CAT_EAM1)
-                    flush_eam1<R, PASS>(D, T, word, s, g0, S, xi, yi, zi, Fpi, p1.rc2, xw, acc);
+                    flush_eam1<R, PASS, EN>(D, T, word, s, g0, S, xi, yi, zi, Fpi, p1.rc2, xw, acc);
                 else if constexpr ((CAT == CAT_LJ || CAT == CAT_MORSE) && PASS == 0)
                 {
                     /* the rounding band is met so seldom that the word's bits are walked again to
                        find its pairs rather than tracked pair by pair */
                     const unsigned long long w0 = word;
                     bool ex = false;
-                    flush_pair2<R, CAT>(D, T, word, s, g0, S, xi, yi, zi, p1, ex, acc);
+                    flush_pair2<R, CAT, EN>(D, T, word, s, g0, S, xi, yi, zi, p1, ex, acc);
                     if constexpr (sizeof(R) == sizeof(float))
                         if (__ballot(ex)) xw = band_bits(T, w0, s, g0, S, xi, yi, zi, (R)(p1.rc2 * (1.0 - RX_BAND)));
                 }
@@ -958,12 +999,12 @@ static __device__ void b_cell(const KDesc &D, const int c)
                                     continue;
                                 }
                             }
-                            pair_hit<R, PASS, CAT>(D, pq, Fpi, fj, xi - x, yi - y, zi - z, acc);
+                            pair_hit<R, PASS, CAT, Acc, EN>(D, pq, Fpi, fj, xi - x, yi - y, zi - z, acc);
                         }
                     }
                 if constexpr (sizeof(R) == sizeof(float))
                 {
-                    if (__ballot(xw != 0ull)) flush_exact<PASS, CAT>(D, T, xw, s, g0, S, X, ppi, p1, acc);
+                    if (__ballot(xw != 0ull)) flush_exact<PASS, CAT, EN>(D, T, xw, s, g0, S, X, ppi, p1, acc);
                 }
                 wi++;
             }
@@ -1054,7 +1095,7 @@ static __device__ void b_cell(const KDesc &D, const int c)
     if (lane == 0)
     {
         if (PASS == 1) D.cell_emb[c] = cell_acc;
-        else D.cell_pe[c] = cell_acc;
+        else if (EN) D.cell_pe[c] = cell_acc;
     }
 }
 
@@ -1094,15 +1135,15 @@ static __device__ __forceinline__ int xcd_cell(const KDesc &D, int b, int n)
     return (px * D.nc1 + py) * D.nc2 + pz;
 }
 
-template <typename R, int PASS, int CAT>
-__global__ __launch_bounds__(WG) void k_cell(KDesc D) { b_cell<R, PASS, CAT>(D, xcd_cell(D, blockIdx.x, gridDim.x)); }
+template <typename R, int PASS, int CAT, bool EN>
+__global__ __launch_bounds__(WG) void k_cell(KDesc D) { b_cell<R, PASS, CAT, EN>(D, xcd_cell(D, blockIdx.x, gridDim.x)); }
 
 template <typename R, int PASS, int CAT>
 __global__ __launch_bounds__(WG) void kb_cell(BatchMap M)
 {
     const int2 it = M.items[blockIdx.x];
     const KDesc &D = M.descs[it.x];
-    b_cell<R, PASS, CAT>(D, it.y);
+    b_cell<R, PASS, CAT, true>(D, it.y);
 }
 /* FP32 positions relative to the origin of the home cell, from the FP64 positions */
 static __device__ void b_relpos(const KDesc &D, int blk)
@@ -1126,24 +1167,32 @@ static inline size_t pp_bytes(const KDesc &D) { return sizeof(PairParam) * D.nki
 
 int force_category(const KDesc &D)
 {
-    if (D.single_cat == FMDX_POT_EAM && D.eam.ne == 1 && D.eam.rp) return CAT_EAM1;
+    if (D.single_cat == FMDX_POT_EAM && D.eam.ne == 1 && D.eam.rf) return CAT_EAM1;
     if (D.nkinds == 1 && D.single_cat == FMDX_POT_LJ) return CAT_LJ;
     if (D.nkinds == 1 && D.single_cat == FMDX_POT_MORSE) return CAT_MORSE;
     return CAT_GEN;
 }
 
-template <typename R, int PASS>
-static void launch_cell(const KDesc &D, hipStream_t st)
+template <typename R, int PASS, bool EN>
+static void launch_cell_en(const KDesc &D, hipStream_t st)
 {
     const dim3 g(D.ncell), b(WG);
     const size_t lds = pp_bytes(D);
     switch (force_category(D))
     {
-        case CAT_EAM1: hipLaunchKernelGGL((k_cell<R, PASS, CAT_EAM1>), g, b, lds, st, D); break;
-        case CAT_LJ: hipLaunchKernelGGL((k_cell<R, PASS, CAT_LJ>), g, b, lds, st, D); break;
-        case CAT_MORSE: hipLaunchKernelGGL((k_cell<R, PASS, CAT_MORSE>), g, b, lds, st, D); break;
-        default: hipLaunchKernelGGL((k_cell<R, PASS, CAT_GEN>), g, b, lds, st, D); break;
+        case CAT_EAM1: hipLaunchKernelGGL((k_cell<R, PASS, CAT_EAM1, EN>), g, b, lds, st, D); break;
+        case CAT_LJ: hipLaunchKernelGGL((k_cell<R, PASS, CAT_LJ, EN>), g, b, lds, st, D); break;
+        case CAT_MORSE: hipLaunchKernelGGL((k_cell<R, PASS, CAT_MORSE, EN>), g, b, lds, st, D); break;
+        default: hipLaunchKernelGGL((k_cell<R, PASS, CAT_GEN, EN>), g, b, lds, st, D); break;
     }
+}
+
+/* the density pass has one form; the force pass takes the pair energy when D.energy is set */
+template <typename R, int PASS>
+static void launch_cell(const KDesc &D, hipStream_t st)
+{
+    if (PASS == 1 || D.energy) launch_cell_en<R, PASS, true>(D, st);
+    else launch_cell_en<R, PASS, false>(D, st);
 }
 
 /* pass 1 (EAM density + F'(rho)) and pass 0 (forces + energy), launched separately so that
@@ -1177,7 +1226,7 @@ hipError_t launch_forces(const KDesc &D, hipStream_t st, int precision)
     if (e != hipSuccess) return e;
     e = launch_pass0(D, st, precision);
     if (e != hipSuccess) return e;
-    return launch_reduce_pe(D, st);
+    return D.energy ? launch_reduce_pe(D, st) : hipSuccess;
 }
 
 template <typename R, int PASS>

@@ -250,6 +250,20 @@ fmd_handle_t fmd_timer_makeSimple(fmd_t *md, fmd_real_t start, fmd_real_t interv
     return md->ntimers++;
 }
 
+/* whether fmdh_send_timer_ticks would send a tick at time t (the same predicates) */
+bool fmdh_tick_due(fmd_t *md, double t)
+{
+    for (unsigned i = 0; i < md->ntimers; i++)
+    {
+        timer_t_ *tm = &md->timers[i];
+        if (tm->enabled && t >= tm->start - md->timestep / 2.0 &&
+            (t < tm->stop + md->timestep / 2.0 || tm->stop < tm->start) &&
+            fmdh_timer_is_its_time(t, md->timestep / 2.0, tm->start, tm->interval))
+            return true;
+    }
+    return false;
+}
+
 /* _fmd_timer_sendTimerTickEvents, timer.c:64-84 */
 void fmdh_send_timer_ticks(fmd_t *md)
 {

@@ -194,10 +194,22 @@ bool fmdh_distribute(fmd_t *md)
 
 /* --------------------------------------------------------------- loops */
 
-static void update_forces(fmd_t *md)
+/* The reference sums the potential energy in every force call (forces.c:332-382); here the pair
+   passes sum it only when it can be read before the next force call: when the step ends the
+   loop (the caller may read it after), or when a handler may run after it -- a timer tick due at
+   the step's end time, or any turi (field events) -- since a handler may call the getters. */
+static bool energy_needed(fmd_t *md, double t_next, double t_end)
+{
+    if (!(t_next < t_end)) return true;
+    if (md->EventHandler == NULL) return false;
+    if (md->nturies > 0) return true;
+    return fmdh_tick_due(md, t_next);
+}
+
+static void update_forces(fmd_t *md, bool energy)
 {
     int fric = fmdh_ttm_friction_active(md) ? 1 : 0;
-    if (fmdx_update_forces(md->dev, md->ActiveGroup, fric) != FMDX_OK)
+    if (fmdx_update_forces_ex(md->dev, md->ActiveGroup, fric, energy ? FMDX_FORCES_ENERGY : 0) != FMDX_OK)
         fmdh_err_function_failed(md, true, HERE, "fmdx_update_forces");
     md->thermo_host_valid = false;
 }
@@ -210,7 +222,7 @@ static void ticks(fmd_t *md)
 static void velocity_verlet(fmd_t *md, double duration, bool thermostat)
 {
     double start = md->time;
-    update_forces(md);
+    update_forces(md, true);
     if (md->nturies > 0) fmdh_turies_update(md, true, true, true);
     ticks(md);
     unsigned steps = 0;
@@ -224,7 +236,7 @@ static void velocity_verlet(fmd_t *md, double duration, bool thermostat)
         if (fmdx_vv_start(md->dev, md->ActiveGroup, md->timestep, thermostat, md->BerendsenThermostatParam,
                           md->DesiredTemperature) != FMDX_OK)
             fmdh_err_function_failed(md, true, HERE, "fmdx_vv_start");
-        update_forces(md);
+        update_forces(md, energy_needed(md, md->time + md->timestep, start + duration));
         if (fmdx_vv_finish(md->dev, md->ActiveGroup, md->timestep) != FMDX_OK)
             fmdh_err_function_failed(md, true, HERE, "fmdx_vv_finish");
         md->KineticEnergyUpdated = true;
@@ -243,7 +255,7 @@ static void symplectic_euler(fmd_t *md, double duration)
     double start = md->time;
     if (md->active_ttm_turi >= 0) fmdh_ttm_get_ready(md);
     if (md->nturies > 0) fmdh_turies_update(md, true, true, false);
-    update_forces(md);
+    update_forces(md, true);
     if (md->nturies > 0) fmdh_turies_update(md, false, false, true);
     ticks(md);
     unsigned steps = 0;
@@ -256,7 +268,7 @@ static void symplectic_euler(fmd_t *md, double duration)
         md->time += md->timestep;
         md->time_iteration++;
         if (md->nturies > 0) fmdh_turies_update(md, true, true, false);
-        update_forces(md);
+        update_forces(md, energy_needed(md, md->time, start + duration));
         if (md->nturies > 0) fmdh_turies_update(md, false, false, true);
         ticks(md);
         if (++steps % CHECK_EVERY == 0) fmdh_check_device(md);

@@ -247,6 +247,7 @@ bool fmdh_download_all(fmd_t *md, unsigned *n, double **x, double **v, int **grp
 
 /* dynamics */
 void fmdh_send_timer_ticks(fmd_t *md);
+bool fmdh_tick_due(fmd_t *md, double t);
 bool fmdh_timer_is_its_time(double t, double dt_half, double start, double interval);
 
 /* turies */

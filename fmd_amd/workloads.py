@@ -170,10 +170,13 @@ def check_device(fx, dev):
 
 
 def vv_berendsen_steps(fx, dev, nsteps, dt, tau, T0):
-    """nsteps velocity-Verlet steps with Berendsen rescaling through the C-ABI seams."""
-    for _ in range(nsteps):
+    """nsteps velocity-Verlet steps with Berendsen rescaling through the C-ABI seams, as
+    fmd_dync_equilibrate runs them without a timer: the potential energy is summed on the last
+    step only (the thermostat needs the temperature, integrators.c:44-50)."""
+    for k in range(nsteps):
         F._ck(fx.fmdx_vv_start(dev, -1, dt, 1, tau, T0), "vv_start")
-        F._ck(fx.fmdx_update_forces(dev, -1, 0), "update_forces")
+        flags = F.FMDX_FORCES_ENERGY if k == nsteps - 1 else 0
+        F._ck(fx.fmdx_update_forces_ex(dev, -1, 0, flags), "update_forces")
         F._ck(fx.fmdx_vv_finish(dev, -1, dt), "vv_finish")
 
 

@@ -164,6 +164,12 @@ int fmdx_ncells(fmdx_system_t *s, unsigned *n);
 
 /* step seams (asynchronous) */
 int fmdx_update_forces(fmdx_system_t *s, int active_group, int ttm_friction);
+/* the same with flags: FMDX_FORCES_ENERGY also sums the potential energy into the thermo record
+   (fmdx_update_forces always does); without it the pair passes skip the pair-energy terms and the
+   energy reductions, and the record keeps the energy of the last call that summed it -- for time
+   loops whose step is not observed (integrators.c:44-50 needs only the temperature) */
+#define FMDX_FORCES_ENERGY 1
+int fmdx_update_forces_ex(fmdx_system_t *s, int active_group, int ttm_friction, int flags);
 /* fmdx_update_forces without any device allocation: no counterpart in the reference.  For callers
    that cannot allocate (HIP graph capture, serving loops that forbid hipMalloc).  The call's
    scratch -- per-cell energies, the density pass's tile-round records, energy partials -- lives in

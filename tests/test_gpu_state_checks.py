@@ -96,3 +96,30 @@ def test_batch_member_guards(rng):
     b.close()          # the batch lets its members go
     d.set_capacity(64)
     d.close()
+
+
+@pytest.mark.parametrize("precision", [F.FMDX_PREC_FP64, F.FMDX_PREC_MIXED])
+@pytest.mark.parametrize("table", ["cu01", "lj"])
+def test_force_call_without_energy(rng, cu01, precision, table):
+    """fmdx_update_forces_ex without FMDX_FORCES_ENERGY: the same force and F'(rho) bits as with
+    it, the recorded potential energy left as the last energy call set it."""
+    print("forces without energy:", table, precision, flush=True)
+    lib = F.load_fmdx()
+    if table == "cu01":
+        o, d = _cu(rng, cu01, precision, n=(7, 7, 7))
+    else:
+        x = lattice_fcc((7, 7, 7), 5.26, rng, jitter=0.1)
+        o, d = build([7 * 5.26] * 3, [1, 1, 1], [39.948], ["Ar"], [("lj", 0, 0, 3.405, 0.0104, 2.5 * 3.405)], x,
+                     precision=precision)
+    d.forces()
+    pe0 = d.read().pe
+    ref = d.download()
+    d.vv_start(1e-3)
+    F._ck(lib.fmdx_update_forces_ex(d.h, -1, 0, 0), "fmdx_update_forces_ex")
+    t = d.read()
+    assert t.pe == pe0 and t.err == 0
+    a = d.download()
+    F._ck(lib.fmdx_update_forces_ex(d.h, -1, 0, F.FMDX_FORCES_ENERGY), "fmdx_update_forces_ex")
+    b = d.download()
+    assert np.array_equal(a["f"], b["f"]) and np.array_equal(a["aux"], b["aux"])
+    assert d.read().pe != pe0 and not np.array_equal(a["x"], ref["x"])
