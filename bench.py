@@ -273,29 +273,37 @@ def batched_ensemble(nsys, steps, plain_sample, seed=3):
                 members_with_errors=errs)
 
 
-def served_collisions(njobs, rate, seed=4):
-    """Served two-cluster Morse Cu collision jobs (Poisson arrivals) through the
-    continuous-batching loop in mixed precision, against the plain FP64
-    one-job-at-a-time loop on the same arrival stream."""
+def served_collisions(njobs, rates, seed=4):
+    """Served two-cluster Morse Cu collision jobs through the continuous-batching loop in mixed
+    precision, at each Poisson arrival rate, against the plain FP64 one-job-at-a-time loop (first
+    come first served) on the same arrival stream.  The same njobs jobs at every rate; the plain
+    run time of each job is measured once and its FCFS schedule computed for each stream."""
     from fmd_amd import ensemble as E
     from fmd_amd import serve as S
-    specs = E.collision_specs(njobs, seed=seed, rate=rate)
-    served = S.Server(E.collision_job, precision=F.FMDX_PREC_MIXED).run(specs)
-    plain = S.run_plain(specs, E.collision_job)
-    a, p = S.summarize(served), S.summarize(plain)
+    base = E.collision_specs(njobs, seed=seed, rate=1.0)
+    t0 = time.perf_counter()
+    service = S.plain_service(base, E.collision_job)
+    plain_s = time.perf_counter() - t0
+    per_rate = []
+    for rate in rates:
+        specs = E.collision_specs(njobs, seed=seed, rate=rate)
+        served = S.Server(E.collision_job, precision=F.FMDX_PREC_MIXED).run(specs)
+        plain = S.run_plain(specs, E.collision_job, service=service)
+        a, p = S.summarize(served), S.summarize(plain)
+        per_rate.append(dict(rate=rate, served_mixed=a, plain_fp64=p,
+                             jobs_per_s_ratio=a["jobs_per_s"] / p["jobs_per_s"],
+                             p99_ratio_plain_over_served=p["p99_s"] / a["p99_s"]))
+        log("bench: served at %g jobs/s: %s" % (rate, per_rate[-1]))
     ferr = []
-    for sp in specs[:4]:   # mixed against FP64 forces on the first jobs' initial states
+    for sp in base[:4]:   # mixed against FP64 forces on the first jobs' initial states
         j = E.collision_job(sp, F.FMDX_PREC_FP64)
         try:
             ferr.append(mixed_force_error(j.f, F.FMDX_PREC_FP64))
         finally:
             j.free()
-    return dict(workload="%d jobs, Poisson arrivals at %g jobs/s; each two Morse-Cu FCC spheres R=15-50 A, "
-                         "20 A gap, 10-50 A/ps, impact parameter 0-(R1+R2), vacuum box, 1000-4000 steps of 1 fs"
-                         % (njobs, rate),
-                served_mixed=a, plain_fp64=p,
-                jobs_per_s_ratio=a["jobs_per_s"] / p["jobs_per_s"],
-                p99_ratio_plain_over_served=p["p99_s"] / a["p99_s"],
+    return dict(workload="%d jobs per rate, Poisson arrivals; each two Morse-Cu FCC spheres R=15-50 A, 20 A gap, "
+                         "10-50 A/ps, impact parameter 0-(R1+R2), vacuum box, 1000-4000 steps of 1 fs" % njobs,
+                rates=per_rate, plain_run_seconds=plain_s,
                 max_rel_force_err_mixed_vs_fp64=max(e[0] for e in ferr),
                 rms_rel_force_err_mixed_vs_fp64=max(e[1] for e in ferr),
                 force_err_sample="initial states of the first %d jobs" % len(ferr))
@@ -518,8 +526,8 @@ def main():
     ap.add_argument("--batch-systems", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=40)
     ap.add_argument("--plain-sample", type=int, default=16)
-    ap.add_argument("--serve-jobs", type=int, default=24)
-    ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--serve-jobs", type=int, default=64)
+    ap.add_argument("--serve-rates", default="5,50,200", help="Poisson arrival rates (jobs/s), comma-separated")
     ap.add_argument("--no-ttm", action="store_true", help="skip the TTM slab measurement")
     ap.add_argument("--ttm-steps", type=int, default=200)
     ap.add_argument("--no-drift", action="store_true", help="skip the NVE drift measurement")
@@ -583,7 +591,8 @@ def main():
                           args.plain_sample)
             log("bench: batched ensemble %s" % ens)
         if not args.no_serve:
-            srv = guarded("served collisions", served_collisions, args.serve_jobs, args.serve_rate)
+            srv = guarded("served collisions", served_collisions, args.serve_jobs,
+                          [float(r) for r in args.serve_rates.split(",")])
             log("bench: served collisions %s" % srv)
         cpu = None
         if not args.no_cpu_baseline:

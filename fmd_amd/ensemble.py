@@ -84,15 +84,17 @@ def _sphere(R, a, rng):
 
 
 def collision_specs(n, seed=4, rate=50.0, steps=(1000, 4000)):
-    """n jobs with Poisson arrival times (s) at `rate` jobs/s."""
+    """n jobs with Poisson arrival times (s) at `rate` jobs/s.  The jobs and the arrivals come
+    from separate streams: the same seed gives the same n jobs at every rate, arriving at the
+    times of one unit-rate Poisson stream divided by the rate."""
     rng = np.random.default_rng(seed)
-    t = 0.0
+    unit = np.cumsum(np.random.default_rng([seed, 1]).exponential(1.0, size=n))
     out = []
     for i in range(n):
-        t += rng.exponential(1.0 / rate)
-        out.append(dict(index=i, arrival=t, R1=float(rng.uniform(15.0, 50.0)), R2=float(rng.uniform(15.0, 50.0)),
-                        speed=float(rng.uniform(10.0, 50.0)), b=float(rng.uniform(0.0, 1.0)),
-                        nsteps=int(rng.integers(steps[0], steps[1] + 1)), seed=int(rng.integers(1 << 31))))
+        out.append(dict(index=i, arrival=float(unit[i] / rate), R1=float(rng.uniform(15.0, 50.0)),
+                        R2=float(rng.uniform(15.0, 50.0)), speed=float(rng.uniform(10.0, 50.0)),
+                        b=float(rng.uniform(0.0, 1.0)), nsteps=int(rng.integers(steps[0], steps[1] + 1)),
+                        seed=int(rng.integers(1 << 31))))
     return out
 
 

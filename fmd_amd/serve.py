@@ -123,13 +123,10 @@ class Server:
         return n * EST_BYTES_PER_ATOM + (1 << 20)
 
 
-def run_plain(specs, build, dt=1e-3, keep_positions=False):
-    """The same jobs one at a time, FP64, through the single-system time loop,
-    first come first served; a job's latency is its queueing delay plus its
-    own run time, both taken from the measured run times."""
-    out = []
-    t_free = 0.0
-    for sp in sorted(specs, key=lambda s: s["arrival"]):
+def plain_service(specs, build, dt=1e-3, keep_positions=False):
+    """Each job alone, FP64, through the single-system time loop: {index: (run time s, Result)}."""
+    out = {}
+    for sp in sorted(specs, key=lambda s: s["index"]):
         r = Result(sp["index"], sp["arrival"])
         job = build(sp, FMDX_PREC_FP64)
         f = job.f
@@ -137,10 +134,6 @@ def run_plain(specs, build, dt=1e-3, keep_positions=False):
         # step count exactly nsteps: the reference loop runs while time < duration (integrators.c:191-203)
         f.fmd_dync_integrate(-1, (job.nsteps - 0.5) * dt, dt)
         svc = time.perf_counter() - t0
-        start = max(sp["arrival"], t_free)
-        t_free = start + svc
-        r.admit, r.finish = start, t_free
-        r.latency = t_free - sp["arrival"]
         r.natoms, r.nsteps = job.natoms, job.nsteps
         r.temperature = f.fmd_matt_getTemperature()
         r.pe = f.fmd_ext_getPotentialEnergy()
@@ -149,6 +142,30 @@ def run_plain(specs, build, dt=1e-3, keep_positions=False):
             r.x = f.atoms()
         r.natoms_end = len(f.atoms()["x"])
         job.free()
+        out[sp["index"]] = (svc, r)
+    return out
+
+
+def run_plain(specs, build, dt=1e-3, keep_positions=False, service=None):
+    """The same jobs one at a time, FP64, through the single-system time loop,
+    first come first served; a job's latency is its queueing delay plus its
+    own run time, both taken from the measured run times.  `service` (from
+    plain_service) reuses run times measured for the same jobs, e.g. under
+    another arrival stream: a one-at-a-time server's schedule depends only on
+    the arrivals and the run times."""
+    if service is None:
+        service = plain_service(specs, build, dt, keep_positions)
+    out = []
+    t_free = 0.0
+    for sp in sorted(specs, key=lambda s: s["arrival"]):
+        svc, r0 = service[sp["index"]]
+        r = Result(sp["index"], sp["arrival"])
+        for k in ("natoms", "nsteps", "temperature", "pe", "err", "x", "natoms_end"):
+            setattr(r, k, getattr(r0, k))
+        start = max(sp["arrival"], t_free)
+        t_free = start + svc
+        r.admit, r.finish = start, t_free
+        r.latency = t_free - sp["arrival"]
         out.append(r)
     return sorted(out, key=lambda r: r.index)
 
