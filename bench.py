@@ -36,7 +36,9 @@ METRIC = "atom-steps/s EAM Cu 4M atoms (%HBM roofline); batched jobs/s + p99 vs 
 HBM_PEAK_GBS = 8000.0
 # algorithmic HBM bytes per atom of the EAM force kernel (pass 0): read x (24) + F' (8), write F (24)
 PASS0_BYTES = {F.FMDX_PREC_FP64: 56.0, F.FMDX_PREC_MIXED: 12.0 + 8.0 + 24.0}
-PASS0_KERNEL = {F.FMDX_PREC_FP64: "k_cell<double,0,EAM1>", F.FMDX_PREC_MIXED: "k_cell<float,0,EAM1>"}
+# the force pass as the timed steps run it: every step but the last skips the pair energy
+# (k_cell<R, pass 0, single-element EAM, energy>)
+PASS0_KERNEL = {F.FMDX_PREC_FP64: "k_cell<double,0,1,false>", F.FMDX_PREC_MIXED: "k_cell<float,0,1,false>"}
 # HBM bytes per launch of the dominant kernels, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "eam_cu_4m_traffic.json")
 # algorithmic HBM bytes per atom-step of the whole EAM step (integrate 120 + bin 8 + rho 32 + force 56)
@@ -625,7 +627,8 @@ def main():
                    "natoms": n, "precision": args.precision},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(PASS0_KERNEL[prec], n),
-                     "kernel": PASS0_KERNEL[prec] + " (EAM force pass)",
+                     "kernel": PASS0_KERNEL[prec] + " (EAM force pass without the pair energy; kernel_ms averages "
+                                                    "the timed steps' force passes, the last of which sums it)",
                      "kernel_ms": r["pass0_ms"], "algorithmic_bytes_per_atom": PASS0_BYTES[prec],
                      "unit_busy": {k: v for k, v in (measured_counters(PASS0_KERNEL[prec], n) or {}).items()
                                    if k.endswith("_busy")} or None},
