@@ -1,21 +1,27 @@
 """Build-time tuner of the pair kernels' compile-time shape (fmdx_tuned.h).
 
-The LDS candidate tile (FMDX_TILE64 / FMDX_TILE32 / FMDX_TILE32_EAM) and the pairs per iteration
-of the single-element EAM passes (FMDX_EAM_NP{64,32}_{0,1}) are constants of
-k_force.hip.  They are chosen at build time, not per run: the tile decides how
-a cell's candidates split into staging rounds and stripes, and with it the
-order in which each atom's pair terms are summed, so one build gives the same
-bits for a system alone, in any batch and on every run.  The cell size (the
-binning grid) and one wave per home cell are fixed by design, not tuned.
+The waves per home cell (FMDX_WAVES_PER_CELL), the LDS candidate tile per
+precision and pair category (FMDX_TILE64 / FMDX_TILE32 / FMDX_TILE32_EAM /
+FMDX_TILE32_LJ / FMDX_TILE32_MORSE) and the pairs per iteration of the
+single-element EAM passes (FMDX_EAM_NP{64,32}_{0,1}) are constants of
+k_force.hip.  They are chosen at build time, not per run: the waves and the tile
+decide how a cell's candidates split into staging rounds and stripes, and with
+it the order in which each atom's pair terms are summed, so one build gives the
+same bits for a system alone, in any batch and on every run.  The cell size
+(the binning grid) is fixed by design, not tuned.
 
 Procedure: coordinate descent over PARAMS in order.  For each candidate value,
 with the other parameters at their current choice, k_force.hip is compiled
 with -D overrides and linked into a complete libfmdx.so / libfmd.so pair in a
 scratch directory (the candidates of one parameter compile in parallel).  A
-child process loads that pair and times the 4M-atom EAM Cu step (HIP events
-around `steps` velocity-Verlet steps after `warmup`, best of `repeats` runs)
-in the precision the parameter belongs to.  The fastest candidate wins; one
-within `tol` of the incumbent does not replace it.  With --write the choice is
+child process loads that pair and times the workload the parameter belongs
+to, best of `repeats` runs: the 4M-atom EAM Cu step in FP64 or mixed precision
+(HIP events around `steps` velocity-Verlet steps after `warmup`), or, for the
+LJ and Morse tiles, the 512-system mixed-precision ensemble step of the batch
+engine (wall time of `steps` captured steps).  A candidate whose potential
+energy differs from the incumbent's by more than `pe_tol` (relative) is
+rejected as wrong, not slow.  The fastest candidate wins; one within `tol` of
+the incumbent does not replace it.  With --write the choice is
 written to fmdx_tuned.h with the measured table as a comment, and
 `python -m fmd_amd.build` compiles it in.
 
@@ -35,13 +41,14 @@ from concurrent.futures import ThreadPoolExecutor
 
 from . import build as B
 
-FP64, MIXED = 0, 1
-PRECISION_NAME = {FP64: "fp64", MIXED: "mixed"}
+FP64, MIXED, ENSEMBLE = 0, 1, 2
+WORKLOAD_NAME = {FP64: "fp64 EAM step", MIXED: "mixed EAM step", ENSEMBLE: "mixed ensemble step"}
 TUNED_H = os.path.join(B.DEV_SRC, "fmdx_tuned.h")
 TUNED_FILE = "k_force.hip"   # the only translation unit the parameters reach
 
-# (macro, candidates, precision whose step time decides)
+# (macro, candidates, workload whose step time decides)
 PARAMS = [
+    ("FMDX_WAVES_PER_CELL", (1, 2), FP64),
     ("FMDX_TILE64", (192, 224, 256), FP64),
     ("FMDX_EAM_NP64_0", (2, 3), FP64),
     ("FMDX_EAM_NP64_1", (2, 3), FP64),
@@ -49,6 +56,8 @@ PARAMS = [
     ("FMDX_TILE32_EAM", (280, 312, 344), MIXED),
     ("FMDX_EAM_NP32_0", (2, 3, 4), MIXED),
     ("FMDX_EAM_NP32_1", (2, 4, 6), MIXED),
+    ("FMDX_TILE32_LJ", (320, 384, 448), ENSEMBLE),
+    ("FMDX_TILE32_MORSE", (320, 384, 448), ENSEMBLE),
 ]
 
 
@@ -123,9 +132,9 @@ class Builder:
         return d
 
 
-def time_libs(libdir, precision, setfl, cells, steps, warmup, repeats, timeout):
-    """Best-of-`repeats` ms per step of the EAM Cu step with the libraries in `libdir`."""
-    cmd = [sys.executable, "-m", "fmd_amd.autotune", "--child", libdir, "--precision", str(precision),
+def time_libs(libdir, workload, setfl, cells, steps, warmup, repeats, timeout):
+    """Best-of-`repeats` ms per step of `workload` with the libraries in `libdir`."""
+    cmd = [sys.executable, "-m", "fmd_amd.autotune", "--child", libdir, "--workload", str(workload),
            "--setfl", setfl, "--cells", str(cells), "--steps", str(steps), "--warmup", str(warmup),
            "--repeats", str(repeats)]
     r = subprocess.run(cmd, cwd=B.ROOT, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=timeout)
@@ -134,18 +143,51 @@ def time_libs(libdir, precision, setfl, cells, steps, warmup, repeats, timeout):
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
+def child_ensemble(args):
+    """The 512-system LJ/Morse ensemble, mixed precision, batched and graph-captured."""
+    import time
+    import fmd_amd as F
+    from . import ensemble as E
+    jobs = [E.ensemble_job(sp, F.FMDX_PREC_MIXED) for sp in E.ensemble_specs(512, seed=3)]
+    best = None
+    for _ in range(args.repeats):
+        b = F.Batch(F.FMDX_PREC_MIXED)
+        for j in jobs:
+            b.add(j.dev, j.dt)
+        b.step(args.warmup)
+        t0 = time.perf_counter()
+        b.step(args.steps)
+        ms = (time.perf_counter() - t0) / args.steps * 1e3
+        st = b.status()
+        b.close()
+        if any(x.err for x in st):
+            raise RuntimeError("ensemble member reported a device error")
+        if best is None or ms < best["ms"]:
+            best = dict(ms=ms, pass1_ms=float("nan"), pass0_ms=float("nan"), pe=sum(x.pe for x in st))
+    for j in jobs:
+        j.free()
+    print(json.dumps(best), flush=True)
+
+
 def child(args):
     """Runs in its own process so that each candidate's libraries are the ones loaded."""
     import fmd_amd as F
     F.LIB_DIR = args.child
+    if args.workload == ENSEMBLE:
+        return child_ensemble(args)
     from . import workloads as W
     best = None
     for _ in range(args.repeats):
-        r = W.time_eam_cu_steps(args.cells, args.steps, args.warmup, args.precision, args.setfl)
+        r = W.time_eam_cu_steps(args.cells, args.steps, args.warmup, args.workload, args.setfl)
         ms = r["gpu_ms"] / r["steps"]
         if best is None or ms < best["ms"]:
             best = dict(ms=ms, pass1_ms=r["pass1_ms"], pass0_ms=r["pass0_ms"], pe=r["pe"])
     print(json.dumps(best), flush=True)
+
+
+def agrees(pe, ref, tol):
+    """Same potential energy to `tol` relative: tiles and waves only reorder the pair sums."""
+    return abs(pe - ref) <= tol * max(abs(ref), 1e-300)
 
 
 def tune(args):
@@ -160,19 +202,23 @@ def tune(args):
     try:
         log("autotune: building the fixed objects in %s" % scratch)
         bld = Builder(scratch)
-        for macro, cands, prec in PARAMS:
+        for macro, cands, work in PARAMS:
             if macro not in wanted:
                 continue
             trial = [dict(chosen, **{macro: c}) for c in cands]
-            log("autotune: %s, candidates %s, %s step" % (macro, list(cands), PRECISION_NAME[prec]))
+            log("autotune: %s, candidates %s, %s" % (macro, list(cands), WORKLOAD_NAME[work]))
             with ThreadPoolExecutor(max_workers=args.jobs) as ex:
                 dirs = list(ex.map(bld.libs, trial))
-            times = []
+            runs = {}
             for c, d in zip(cands, dirs):
-                r = time_libs(d, prec, setfl, args.cells, args.steps, args.warmup, args.repeats, args.timeout)
-                times.append((c, r["ms"]))
-                log("autotune:   %s=%d  %.3f ms/step (density pass %.3f, force pass %.3f)" %
-                    (macro, c, r["ms"], r["pass1_ms"], r["pass0_ms"]))
+                r = time_libs(d, work, setfl, args.cells, args.steps, args.warmup, args.repeats, args.timeout)
+                runs[c] = r
+                log("autotune:   %s=%d  %.3f ms/step (density pass %.3f, force pass %.3f, pe %.9g)" %
+                    (macro, c, r["ms"], r["pass1_ms"], r["pass0_ms"], r["pe"]))
+            ref = runs.get(chosen[macro], runs[cands[0]])["pe"]
+            times = [(c, r["ms"]) for c, r in runs.items() if agrees(r["pe"], ref, args.pe_tol)]
+            for c in sorted(set(runs) - set(c for c, _ in times)):
+                log("autotune:   %s=%d rejected: potential energy %.9g against %.9g" % (macro, c, runs[c]["pe"], ref))
             inc = dict(times).get(chosen[macro])
             best_c, best_t = min(times, key=lambda ct: ct[1])
             if inc is None or best_t < inc * (1.0 - args.tol):
@@ -183,8 +229,8 @@ def tune(args):
         shutil.rmtree(scratch, ignore_errors=True)
     result = dict(chosen={m: chosen[m] for m, _, _ in PARAMS},
                   table=[dict(macro=m, choice=v, ms=dict(ts)) for m, v, ts in table],
-                  workload="EAM Cu FCC %d^3, synthetic setfl, %d steps after %d, best of %d" %
-                  (args.cells, args.steps, args.warmup, args.repeats))
+                  workload="EAM Cu FCC %d^3, synthetic setfl, and the 512-system LJ/Morse ensemble; "
+                           "%d steps after %d, best of %d" % (args.cells, args.steps, args.warmup, args.repeats))
     if args.write:
         write_tuned(chosen, table)
         log("autotune: wrote %s; rebuild with python -m fmd_amd.build" % TUNED_H)
@@ -198,13 +244,17 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--repeats", type=int, default=2)
-    ap.add_argument("--params", default="", help="comma list of TILE64, NP64_0, NP64_1, TILE32, NP32_0, NP32_1")
+    ap.add_argument("--params", default="",
+                    help="comma list of WAVES_PER_CELL, TILE64, NP64_0, NP64_1, TILE32, TILE32_EAM, "
+                         "NP32_0, NP32_1, TILE32_LJ, TILE32_MORSE")
     ap.add_argument("--tol", type=float, default=0.01, help="relative gain needed to replace the incumbent")
+    ap.add_argument("--pe-tol", type=float, default=1e-6,
+                    help="relative potential-energy difference beyond which a candidate is rejected")
     ap.add_argument("--jobs", type=int, default=4, help="parallel compilations")
     ap.add_argument("--timeout", type=float, default=300.0, help="seconds per timing child")
     ap.add_argument("--write", action="store_true", help="write the choice to fmdx_tuned.h")
     ap.add_argument("--child", default=None, help=argparse.SUPPRESS)
-    ap.add_argument("--precision", type=int, default=FP64, help=argparse.SUPPRESS)
+    ap.add_argument("--workload", type=int, default=FP64, help=argparse.SUPPRESS)
     ap.add_argument("--setfl", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args(argv)
     if args.child:

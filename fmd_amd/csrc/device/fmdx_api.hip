@@ -324,7 +324,7 @@ extern "C" int fmdx_set_eam(fmdx_system_t *s, int nelem, int nrho, double drho, 
     /* hit words kept between the two EAM passes (one bit per lane and trip) */
     if (!s->D.hbits)
     {
-        const size_t nw = (size_t)s->D.ncell * HBITS_WORDS * 64;
+        const size_t nw = (size_t)s->D.ncell * HBITS_WORDS * force_threads_per_cell();
         if (dalloc(&s->D.hbits, nw)) return FMDX_ENOMEM;
         /* never read before the density pass writes it; cleared all the same */
         CK(hipMemsetAsync(s->D.hbits, 0, nw * sizeof(unsigned long long), s->st));
@@ -638,7 +638,7 @@ extern "C" int fmdx_memory(fmdx_system_t *s, size_t *bytes)
     const size_t nmov = s->natoms_uploaded + 64;
     size_t b = s->uploaded ? ns * per_slot + nmov * (6 * sizeof(double) + 9) : 0;
     b += (size_t)D.ncell * (6 * sizeof(int) + 2 * sizeof(double) + sizeof(unsigned long long));
-    if (D.hbits) b += (size_t)D.ncell * HBITS_WORDS * 64 * sizeof(unsigned long long);
+    if (D.hbits) b += (size_t)D.ncell * HBITS_WORDS * force_threads_per_cell() * sizeof(unsigned long long);
     if (s->rhoc) b += (size_t)(D.eam.nrho + 6 * D.eam.nr2) * (sizeof(double4) + sizeof(float4));
     *bytes = b;
     return FMDX_OK;

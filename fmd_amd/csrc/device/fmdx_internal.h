@@ -108,7 +108,7 @@ struct KDesc
     const PairParam *pairs;   /* [MAXK*MAXK] */
     EamDev eam;
     /* pair-candidate hits found by the density pass, replayed by the force pass */
-    unsigned long long *hbits;   /* [ncell][HBITS_WORDS][64] one bit per (lane, scan trip) */
+    unsigned long long *hbits;   /* [ncell][HBITS_WORDS][threads per cell] one bit per (thread, scan trip) */
     int *htrips;                 /* [ncell] hit words recorded, -1 = none (the force pass rescans) */
     unsigned long long *htile;   /* [ncell] candidates staged in each of the first 4 tile rounds, 16 bits each */
     /* reductions */
@@ -239,5 +239,7 @@ hipError_t launch_ttm_presolve(const TtmState &S, unsigned *err, hipStream_t st)
 hipError_t launch_ttm_solve(const TtmState &S, double time, double dt, unsigned *err, hipStream_t st);
 hipError_t launch_fill(double *p, long n, double v, hipStream_t st);
 hipError_t launch_audit(const KDesc &D, hipStream_t st);
+/* threads per home cell of the pair kernels (the hit words are kept per thread); k_force.hip */
+int force_threads_per_cell();
 
 #endif /* FMDX_INTERNAL_H */

@@ -6,8 +6,11 @@
  *   FMDX_TILE64 / FMDX_TILE32   candidates per LDS tile, FP64 / FP32 kernels
  *   FMDX_TILE32_EAM             the same for the FP32 single-element EAM passes, whose tiles also
  *                               hold each candidate's list place (for the exact FP64 pairs)
+ *   FMDX_TILE32_LJ / _MORSE     the same for the FP32 single-kind LJ and Morse passes
+ *                               (FMDX_TILE32: the FP32 generic kernel)
  *   FMDX_EAM_NP64_0 / _1        pairs per flush iteration, FP64 force / density pass
  *   FMDX_EAM_NP32_0 / _1        the same for the FP32 (mixed) passes
+ *   FMDX_WAVES_PER_CELL         wavefronts per home cell (1 or 2): one workgroup per cell either way
  *
  * Fixed by design, not tuned: the cell size (the binning grid, cellh >= rc,
  * whose exact atom order the kernels share) and one wave per home cell (its
@@ -34,8 +37,17 @@
 #ifndef FMDX_TILE32
 #define FMDX_TILE32 384
 #endif
+#ifndef FMDX_TILE32_LJ
+#define FMDX_TILE32_LJ 384
+#endif
+#ifndef FMDX_TILE32_MORSE
+#define FMDX_TILE32_MORSE 384
+#endif
 #ifndef FMDX_TILE32_EAM
 #define FMDX_TILE32_EAM 344
+#endif
+#ifndef FMDX_WAVES_PER_CELL
+#define FMDX_WAVES_PER_CELL 1
 #endif
 #ifndef FMDX_EAM_NP32_0
 #define FMDX_EAM_NP32_0 3

@@ -40,7 +40,8 @@
  */
 #include "fmdx_internal.h"
 
-#define WG 64
+#define WG 64                             /* lanes per wavefront */
+#define NT (WG * FMDX_WAVES_PER_CELL)     /* threads per home cell (one workgroup) */
 #define NOKIND 0xFF    /* padding or a candidate outside the active group */
 
 /* pair-category specialisations chosen on the host */
@@ -76,7 +77,7 @@ template <> struct Prec<double>
 };
 template <> struct Prec<float>
 {
-    enum { CT = FMDX_TILE32 };   /* the EAM kernels use FMDX_TILE32_EAM */
+    enum { CT = FMDX_TILE32 };   /* the generic kernel; the others use FMDX_TILE32_EAM / _LJ / _MORSE */
     typedef float4 Q;
     static __device__ __forceinline__ const float4 *rho(const EamDev &E) { return E.rhoc32; }
     static __device__ __forceinline__ const float4 *phi(const EamDev &E) { return E.phic32; }
@@ -95,7 +96,7 @@ template <bool FP> struct RecOf { typedef double4 T; };
 template <> struct RecOf<false> { typedef Rec3 T; };
 template <bool FP, bool KD, int CT> struct Tile<double, FP, KD, CT>
 {
-    enum { N = CT + WG };
+    enum { N = CT + NT };
     typename RecOf<FP>::T p[N];   /* x, y, z (, F') */
     uint8_t k[KD ? N : 1];
     __device__ __forceinline__ void put(int m, double x, double y, double zz, double f, int kk)
@@ -118,7 +119,7 @@ template <bool FP, bool KD, int CT> struct Tile<double, FP, KD, CT>
 };
 template <bool FP, bool KD, int CT> struct Tile<float, FP, KD, CT>
 {
-    enum { N = CT + WG };
+    enum { N = CT + NT };
     float4 p[N];
     unsigned short g[N];   /* the candidate's place in the 27-cell list, for the exact FP64 pairs */
     uint8_t k[KD ? N : 1];
@@ -689,9 +690,15 @@ static __device__ void b_cell(const KDesc &D, const int c)
 {
     typedef typename AccOf<R>::T Acc;
     typedef Prec<R> P;
-    /* candidates per tile: the FP32 EAM kernels keep a smaller tile, so that the density pass's LDS
-       (tile plus the list places of the exact pairs) still lets 20 workgroups share a CU */
-    enum { CT = sizeof(R) == sizeof(float) && CAT == CAT_EAM1 ? FMDX_TILE32_EAM : (int)P::CT };
+    /* candidates per tile, per precision and pair category (fmdx_tuned.h): the FP32 EAM kernels
+       keep a smaller tile, so that the density pass's LDS (tile plus the list places of the exact
+       pairs) still lets 20 workgroups share a CU */
+    enum
+    {
+        CT = sizeof(R) == sizeof(double) ? (int)P::CT
+           : CAT == CAT_EAM1 ? FMDX_TILE32_EAM : CAT == CAT_LJ ? FMDX_TILE32_LJ
+           : CAT == CAT_MORSE ? FMDX_TILE32_MORSE : (int)P::CT
+    };
     constexpr bool WITH_FP = PASS == 0 && (CAT == CAT_GEN || CAT == CAT_EAM1);
     if ((PASS == 1 && !D.has_eam) || D.cnt[c] == 0)
     {
@@ -709,12 +716,17 @@ static __device__ void b_cell(const KDesc &D, const int c)
 
     extern __shared__ PairParam pp[];   /* nkinds^2 entries, sized at launch */
     /* the stripe sums reuse the tile once the scan is over */
-    double(*red)[WG] = reinterpret_cast<double(*)[WG]>(&T);
-    static_assert(sizeof(T) >= 4 * WG * sizeof(double), "tile too small for the stripe sums");
+    double(*red)[NT] = reinterpret_cast<double(*)[NT]>(&T);
+    static_assert(sizeof(T) >= 4 * NT * sizeof(double), "tile too small for the stripe sums");
+    /* with several waves per home cell: each wave's survivors of a staging batch (double-buffered
+       by batch parity), and the tile position of each home atom of the block */
+    __shared__ int wcnt[2][FMDX_WAVES_PER_CELL];
+    __shared__ short selfpos[FMDX_WAVES_PER_CELL > 1 ? NT : 1];
 
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x;   /* the thread within the home cell's workgroup */
+    const int wv = lane / WG;       /* its wave */
     const int nkk = D.nkinds * D.nkinds;
-    for (int i = lane; i < nkk; i += WG) pp[i] = D.pairs[i];
+    for (int i = lane; i < nkk; i += NT) pp[i] = D.pairs[i];
     const int cz = c % D.nc2, ct = c / D.nc2, cy = ct % D.nc1, cx = ct / D.nc1;
     int my_cnt = 0;
     if (lane < 27)
@@ -743,8 +755,8 @@ static __device__ void b_cell(const KDesc &D, const int c)
     }
     if (lane < 27) nb_pre[lane + 1] = incl;
     if (lane == 0) nb_pre[0] = 0;
-    const int ncand = __builtin_amdgcn_readfirstlane(__shfl(incl, 26));
     __syncthreads();
+    const int ncand = __builtin_amdgcn_readfirstlane(nb_pre[27]);
 
     const int nh = __builtin_amdgcn_readfirstlane(D.cnt[c]);
     const int ag = D.ag;
@@ -768,7 +780,7 @@ static __device__ void b_cell(const KDesc &D, const int c)
     const R pinf = (R)__builtin_inf();
     const R qnan = (R)__builtin_nan("");
     const long hbase = (long)c * D.cap;
-    const bool one_block = nh > 0 && nh <= WG;
+    const bool one_block = nh > 0 && nh <= NT;
     int hw = -1;
     if (PASS == 0 && D.has_eam) hw = __builtin_amdgcn_readfirstlane(D.htrips[c]);
     /* a density pass without hit-word storage (hbits null) records nothing: the force pass rescans */
@@ -798,10 +810,10 @@ static __device__ void b_cell(const KDesc &D, const int c)
     const R cull2 = rcut2 * (sizeof(R) == sizeof(double) ? (R)(1.0 + 1e-12) : (R)(1.0 + 1e-5));
     double cell_acc = 0.0;
 
-    for (int h0 = 0; h0 < nh; h0 += WG)
+    for (int h0 = 0; h0 < nh; h0 += NT)
     {
-        const int nhc = min(WG, nh - h0);
-        const int S = WG / nhc;
+        const int nhc = min(NT, nh - h0);
+        const int S = NT / nhc;
         const int a = lane % nhc, s = lane / nhc;
         const long is = hbase + h0 + a;
         const ExactCtx X = {nb_base, nb_pre, SH.w, is};
@@ -832,7 +844,12 @@ static __device__ void b_cell(const KDesc &D, const int c)
             /* tile position of this lane's home atom in this round, -1 while it is not in the tile */
             int selft = -1;
             const int gself = nb_pre[13] + h0 + a;   /* its place in the 27-cell candidate list */
-            while (gpos < ncand && nt <= (int)CT - WG)
+            if constexpr (FMDX_WAVES_PER_CELL > 1)
+            {
+                if (lane < nhc) selfpos[lane] = -1;
+            }
+            int batch = 0;
+            while (gpos < ncand && nt <= (int)CT - NT)
             {
                 const int g = gpos + lane;
                 bool keep = false;
@@ -867,8 +884,22 @@ static __device__ void b_cell(const KDesc &D, const int c)
                     }
                 }
                 const unsigned long long bal = __ballot(keep);
-                const int off = nt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
-                                                                   __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+                int off = nt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+                int kept = __popcll(bal);
+                if constexpr (FMDX_WAVES_PER_CELL > 1)
+                {
+                    /* survivors are compacted in list order across the waves */
+                    const int par = batch & 1;
+                    if ((lane & (WG - 1)) == 0) wcnt[par][wv] = kept;
+                    __syncthreads();
+                    kept = 0;
+                    for (int w = 0; w < FMDX_WAVES_PER_CELL; w++)
+                    {
+                        if (w < wv) off += wcnt[par][w];
+                        kept += wcnt[par][w];
+                    }
+                }
                 if (keep)
                 {
                     T.put(off, x, y, z, NSCAN ? x * x + y * y + z * z : fpv, kj);
@@ -876,13 +907,21 @@ static __device__ void b_cell(const KDesc &D, const int c)
                 }
                 /* the lane that staged this lane's home atom (list entry 13 is the home
                    cell itself, not an image) hands over the tile position it wrote */
-                const int src = gself - gpos;
-                const int got = __shfl((keep && lo == 13) ? off : -1, src & (WG - 1));
-                if (src >= 0 && src < WG) selft = got;
-                nt += __popcll(bal);
-                gpos += WG;
+                if constexpr (FMDX_WAVES_PER_CELL == 1)
+                {
+                    const int src = gself - gpos;
+                    const int got = __shfl((keep && lo == 13) ? off : -1, src & (WG - 1));
+                    if (src >= 0 && src < WG) selft = got;
+                }
+                else
+                {
+                    const int hk = g - nb_pre[13] - h0;
+                    if (keep && lo == 13 && hk >= 0 && hk < nhc) selfpos[hk] = (short)off;
+                }
+                nt += kept;
+                gpos += NT;
+                batch++;
             }
-            if (s >= S) selft = -1;
             {
                 const unsigned sh16 = 16u * (unsigned)min(round, 3);
                 if (record && round < 4) tiles |= (unsigned long long)nt << sh16;
@@ -896,18 +935,20 @@ static __device__ void b_cell(const KDesc &D, const int c)
             }
             const int trips = (nt + S - 1) / S;
             const int npad = trips * S;
-            for (int m = nt + lane; m < npad; m += WG)
+            for (int m = nt + lane; m < npad; m += NT)
             {
                 if constexpr (NSCAN) T.put(m, (R)0, (R)0, (R)0, pinf, NOKIND);
                 else T.put(m, -pinf, -pinf, -pinf, (R)0, NOKIND);
             }
             __syncthreads();
+            if constexpr (FMDX_WAVES_PER_CELL > 1) selft = selfpos[a];
+            if (s >= S) selft = -1;
             for (int g0 = 0; g0 < trips; g0 += 64)
             {
                 const int ng = min(64, trips - g0);
                 unsigned long long word;
                 if (replay)
-                    word = wi < hw ? D.hbits[((long)c * HBITS_WORDS + wi) * WG + lane] : 0ull;
+                    word = wi < hw ? D.hbits[((long)c * HBITS_WORDS + wi) * NT + lane] : 0ull;
                 else
                 {
                     /* bits are shifted in from the last trip down, so trip t lands on bit t */
@@ -960,7 +1001,7 @@ static __device__ void b_cell(const KDesc &D, const int c)
                     /* the home atom meets itself on one trip of one stripe: clear that bit */
                     const int ds = selft - s - g0 * S;
                     if (ds >= 0 && ds < ng * S && ds % S == 0) word &= ~(1ull << (ds / S));
-                    if (record && wi < HBITS_WORDS) D.hbits[((long)c * HBITS_WORDS + wi) * WG + lane] = word;
+                    if (record && wi < HBITS_WORDS) D.hbits[((long)c * HBITS_WORDS + wi) * NT + lane] = word;
                 }
                 unsigned long long xw = 0ull;   /* FP32: trips of this word whose pairs are in the exact shell */
                 if constexpr (CAT == CAT_EAM1)
@@ -1136,10 +1177,10 @@ static __device__ __forceinline__ int xcd_cell(const KDesc &D, int b, int n)
 }
 
 template <typename R, int PASS, int CAT, bool EN>
-__global__ __launch_bounds__(WG) void k_cell(KDesc D) { b_cell<R, PASS, CAT, EN>(D, xcd_cell(D, blockIdx.x, gridDim.x)); }
+__global__ __launch_bounds__(NT) void k_cell(KDesc D) { b_cell<R, PASS, CAT, EN>(D, xcd_cell(D, blockIdx.x, gridDim.x)); }
 
 template <typename R, int PASS, int CAT>
-__global__ __launch_bounds__(WG) void kb_cell(BatchMap M)
+__global__ __launch_bounds__(NT) void kb_cell(BatchMap M)
 {
     const int2 it = M.items[blockIdx.x];
     const KDesc &D = M.descs[it.x];
@@ -1165,6 +1206,8 @@ __global__ __launch_bounds__(256) void kb_relpos(BatchMap M)
 
 static inline size_t pp_bytes(const KDesc &D) { return sizeof(PairParam) * D.nkinds * D.nkinds; }
 
+int force_threads_per_cell() { return NT; }
+
 int force_category(const KDesc &D)
 {
     if (D.single_cat == FMDX_POT_EAM && D.eam.ne == 1 && D.eam.rf) return CAT_EAM1;
@@ -1176,7 +1219,7 @@ int force_category(const KDesc &D)
 template <typename R, int PASS, bool EN>
 static void launch_cell_en(const KDesc &D, hipStream_t st)
 {
-    const dim3 g(D.ncell), b(WG);
+    const dim3 g(D.ncell), b(NT);
     const size_t lds = pp_bytes(D);
     switch (force_category(D))
     {
@@ -1237,7 +1280,7 @@ static void launchb_cells(const BatchPlan &P, hipStream_t st, const int2 *const 
     {
         if (n[cat] == 0) continue;
         BatchMap mc = {P.descs, lists[cat]};
-        const dim3 g(n[cat]), b(WG);
+        const dim3 g(n[cat]), b(NT);
         switch (cat)
         {
             case CAT_EAM1: hipLaunchKernelGGL((kb_cell<R, PASS, CAT_EAM1>), g, b, ppb, st, mc); break;

@@ -42,3 +42,16 @@ def test_builder_key_names_every_parameter():
     key = A.Builder.key(A.read_tuned())
     assert key.count("_") >= len(A.PARAMS) - 1
     assert "tile64" in key and "np32_1" in key
+
+
+def test_every_workload_is_named_and_candidates_hold_the_incumbent():
+    vals = A.read_tuned()
+    for m, cands, work in A.PARAMS:
+        assert work in A.WORKLOAD_NAME
+        assert vals[m] in cands, m
+
+
+def test_potential_energy_guard():
+    assert A.agrees(-1.0000001, -1.0, 1e-6)
+    assert not A.agrees(-1.00001, -1.0, 1e-6)
+    assert not A.agrees(float("nan"), -1.0, 1e-6)
