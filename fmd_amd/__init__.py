@@ -14,7 +14,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_DIR = os.path.join(HERE, "lib")
+# the built libraries: fmd_amd/lib, or another build of them (FMD_AMD_LIB_DIR)
+LIB_DIR = os.environ.get("FMD_AMD_LIB_DIR") or os.path.join(HERE, "lib")
 ROOT = os.path.dirname(HERE)
 
 FMDX_OK = 0

@@ -87,7 +87,8 @@ def write_tuned(vals, table, path=TUNED_H):
                    for m, v, ts in table)
     head = head.replace("\n */", "\n * Last tuning run: macro, choice, then step time per candidate:" + rows + "\n */", 1)
     body = "#ifndef FMDX_TUNED_H\n#define FMDX_TUNED_H\n"
-    for m, _, _ in PARAMS:
+    # the tuned parameters, then the header's other switches as they were
+    for m in [p[0] for p in PARAMS] + [m for m in vals if m not in {p[0] for p in PARAMS}]:
         body += "#ifndef %s\n#define %s %d\n#endif\n" % (m, m, vals[m])
     body += "#endif\n"
     with open(path, "w") as fp:

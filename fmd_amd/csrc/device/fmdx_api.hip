@@ -148,7 +148,7 @@ extern "C" int fmdx_system_destroy(fmdx_system_t *s)
     dfree(s->dpairs);
     dfree(s->Fc); dfree(s->rhoc); dfree(s->phic);
     dfree(s->Fc32); dfree(s->rhoc32); dfree(s->phic32);
-    dfree(s->rf); dfree(s->rf32);
+    dfree(s->rf); dfree(s->rf32); dfree(s->kn32);
     for (int i = 0; i < s->nttm_mem; i++) hipFree(s->ttm_mem[i]);
     for (int i = 0; i < s->nev; i++) hipEventDestroy(s->ev[i]);
     for (int i = 0; i < NPROF * 3; i++)
@@ -288,12 +288,28 @@ extern "C" int fmdx_set_eam(fmdx_system_t *s, int nelem, int nrho, double drho, 
         fRF[2 * k] = make_float4((float)r.y, (float)r.z, (float)r.w, (float)p.y);
         fRF[2 * k + 1] = make_float4((float)p.z, (float)p.w, (float)r.x, (float)p.x);
     }
+    /* element 0's derivative knots of rho and phi on the reference's r^2 grid: D_k = h y'(x_k)
+       (the slope in b = x/h - k, from interval k, or the end of the last one) and Y_k = y''_k h^2/6,
+       so that within interval k  dy/db = D_k + b (6 Y_k + 3 b (Y_{k+1} - Y_k)).  The reference's
+       own (y, y'') form would take the slope as y_{k+1} - y_k, which in FP32 cancels ~12 bits */
+    std::vector<float4> fK((size_t)nr2);
+    {
+        const double h26 = dr2 * dr2 / 6.0;
+        for (long k = 0; k < nr2; k++)
+        {
+            const long i = k < m2 ? k : m2 - 1;
+            const double4 r = hR[i], p = hP[i];
+            const double dr = k < m2 ? r.y : r.y + 2.0 * r.z + 3.0 * r.w;
+            const double dp = k < m2 ? p.y : p.y + 2.0 * p.z + 3.0 * p.w;
+            fK[k] = make_float4((float)dr, (float)dp, (float)(rhoDD[k] * h26), (float)(phiDD[k] * h26));
+        }
+    }
     /* tables of an earlier call may still be read by work queued on the stream */
     CK(hipStreamSynchronize(s->st));
     dfree(s->Fc); dfree(s->rhoc); dfree(s->phic); dfree(s->Fc32); dfree(s->rhoc32); dfree(s->phic32);
-    dfree(s->rf); dfree(s->rf32);
+    dfree(s->rf); dfree(s->rf32); dfree(s->kn32);
     int r = 0;
-    r |= dalloc(&s->rf, hRF.size()); r |= dalloc(&s->rf32, fRF.size());
+    r |= dalloc(&s->rf, hRF.size()); r |= dalloc(&s->rf32, fRF.size()); r |= dalloc(&s->kn32, fK.size());
     r |= dalloc(&s->Fc, hF.size()); r |= dalloc(&s->rhoc, hR.size()); r |= dalloc(&s->phic, hP.size());
     r |= dalloc(&s->Fc32, fF.size()); r |= dalloc(&s->rhoc32, fR.size()); r |= dalloc(&s->phic32, fP.size());
     if (r) return FMDX_ENOMEM;
@@ -305,6 +321,7 @@ extern "C" int fmdx_set_eam(fmdx_system_t *s, int nelem, int nrho, double drho, 
     CK(hipMemcpyAsync(s->phic32, fP.data(), fP.size() * sizeof(float4), hipMemcpyHostToDevice, s->st));
     CK(hipMemcpyAsync(s->rf, hRF.data(), hRF.size() * sizeof(double4), hipMemcpyHostToDevice, s->st));
     CK(hipMemcpyAsync(s->rf32, fRF.data(), fRF.size() * sizeof(float4), hipMemcpyHostToDevice, s->st));
+    CK(hipMemcpyAsync(s->kn32, fK.data(), fK.size() * sizeof(float4), hipMemcpyHostToDevice, s->st));
     EamDev &E = s->D.eam;
     E.ne = nelem; E.nrho = nrho; E.nr2 = nr2;
     E.drho = drho; E.inv_drho = 1.0 / drho; E.dr2 = dr2; E.inv_dr2 = 1.0 / dr2; E.rc2 = rc2;
@@ -318,7 +335,7 @@ extern "C" int fmdx_set_eam(fmdx_system_t *s, int nelem, int nrho, double drho, 
     }
     E.Fc = s->Fc; E.rhoc = s->rhoc; E.phic = s->phic;
     E.Fc32 = s->Fc32; E.rhoc32 = s->rhoc32; E.phic32 = s->phic32;
-    E.rf = s->rf; E.rf32 = s->rf32;
+    E.rf = s->rf; E.rf32 = s->rf32; E.kn32 = s->kn32;
     /* the kernels run on the system stream: the tables have landed before any of them reads */
     CK(hipStreamSynchronize(s->st));
     /* hit words kept between the two EAM passes (one bit per lane and trip) */

@@ -55,6 +55,10 @@ struct EamDev
        (rho1 rho2 rho3 phi1)(phi2 phi3 rho0 phi0); FP32 2 x float4, same order */
     const double4 *rf;
     const float4 *rf32;
+    /* element 0's derivative knots, FP32, k < nr2: (rho D_k, phi D_k, rho Y_k, phi Y_k) with
+       D_k = h y'(x_k) and Y_k = y''_k h^2/6: the table the LDS-resident force pass holds
+       (k_force.hip, FMDX_EAM_LDS) */
+    const float4 *kn32;
 };
 
 struct TtmDev

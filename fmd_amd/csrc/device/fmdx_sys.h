@@ -23,7 +23,7 @@ struct fmdx_system
     PairParam *dpairs;
     /* EAM */
     double4 *Fc, *rhoc, *phic, *rf;
-    float4 *Fc32, *rhoc32, *phic32, *rf32;
+    float4 *Fc32, *rhoc32, *phic32, *rf32, *kn32;
     /* TTM */
     int ttm_on;
     TtmState T;

@@ -11,10 +11,12 @@
  *   FMDX_EAM_NP64_0 / _1        pairs per flush iteration, FP64 force / density pass
  *   FMDX_EAM_NP32_0 / _1        the same for the FP32 (mixed) passes
  *   FMDX_WAVES_PER_CELL         wavefronts per home cell (1 or 2): one workgroup per cell either way
+ *   FMDX_EAM_LDS                0, or the waves per workgroup of the FP32 single-element EAM passes
+ *                               with the spline table resident in LDS (one workgroup per CU)
+ *   FMDX_EAM_LDS_TILE           candidates per LDS tile in those passes (one tile per wave)
  *
  * Fixed by design, not tuned: the cell size (the binning grid, cellh >= rc,
- * whose exact atom order the kernels share) and one wave per home cell (its
- * lanes split over (atom, stripe) pairs for any cell population).
+ * whose exact atom order the kernels share).
  * Last tuning run: macro, choice, then step time per candidate:
  *   FMDX_TILE64        224  192: 3.986 ms  224: 3.899 ms  256: 4.053 ms
  *   FMDX_EAM_NP64_0      2  2: 3.893 ms  3: 4.083 ms
@@ -48,6 +50,12 @@
 #endif
 #ifndef FMDX_WAVES_PER_CELL
 #define FMDX_WAVES_PER_CELL 1
+#endif
+#ifndef FMDX_EAM_LDS
+#define FMDX_EAM_LDS 0
+#endif
+#ifndef FMDX_EAM_LDS_TILE
+#define FMDX_EAM_LDS_TILE 184
 #endif
 #ifndef FMDX_EAM_NP32_0
 #define FMDX_EAM_NP32_0 3

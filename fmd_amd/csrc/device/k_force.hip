@@ -490,10 +490,10 @@ static __device__ __forceinline__ FRec<float> frec_at(__amdgpu_buffer_rsrc_t r, 
    (FMDX_EAM_NP*, fmdx_tuned.h).  The density pass reads element 0's density
    table (32 or 16 bytes per interval), the force pass its force records; EN
    selects whether the force pass also takes the pair energy. */
-template <typename R, int PASS, bool EN, typename TileT, typename A>
+template <typename R, int PASS, bool EN, bool LT, typename TileT, typename A>
 static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T, unsigned long long word, int s,
                                                   int g0, int S, R xi, R yi, R zi, R Fpi, double rc2,
-                                                  unsigned long long &xw, A *acc)
+                                                  unsigned long long &xw, A *acc, const float4 *lt)
 {
     constexpr bool EX = sizeof(R) == sizeof(float);   /* FP32: pairs of the exact shell go to xw */
     const R rx2 = EX ? (R)D.eam.rx2 : (R)rc2;
@@ -512,6 +512,7 @@ static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T
         R dx[NP], dy[NP], dz[NP], fj[NP], b[NP];
         bool u[NP];
         unsigned o[NP];
+        int kk[NP];
 #pragma unroll
         for (int j = 0; j < NP; j++)
         {
@@ -530,9 +531,54 @@ static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T
             const R tt = r2 * inv;
             const int k = min((int)tt, kmax);
             b[j] = tt - (R)k;
+            kk[j] = k;
             o[j] = u[j] ? (unsigned)k * rec : RP_NONE;
         }
-        if (PASS == 1)
+        if constexpr (LT)
+        {
+            /* the table in LDS: density coefficients (c0..c3) per interval, or the derivative
+               knots (rho D, phi D, rho Y, phi Y) with dy/db = D_k + b (6 Y_k + 3 b (Y_k+1 - Y_k));
+               a lane without a pair reads interval 0 and adds an exact zero */
+            if (PASS == 1)
+            {
+#pragma unroll
+                for (int j = 0; j < NP; j++)
+                {
+                    const R c = sp_val<R, float4>(lt[kk[j]], b[j]);
+                    acc[3].add(u[j] ? c : (R)0);
+                }
+            }
+            else
+            {
+                float4 k0[NP];
+                float2 y1[NP];
+#pragma unroll
+                for (int j = 0; j < NP; j++)
+                {
+                    k0[j] = lt[kk[j]];
+                    y1[j] = *reinterpret_cast<const float2 *>(&lt[kk[j] + 1].z);
+                }
+#pragma unroll
+                for (int j = 0; j < NP; j++)
+                {
+                    const R bb = b[j];
+                    const R rho = (k0[j].x + bb * ((R)6 * k0[j].z + (R)3 * bb * (y1[j].x - k0[j].z))) * inv;
+                    const R ph = (k0[j].y + bb * ((R)6 * k0[j].w + (R)3 * bb * (y1[j].y - k0[j].w))) * inv;
+                    const R mag = (R)2 * (Fpi * rho + fj[j] * rho + ph);
+                    acc[0].add(u[j] ? -mag * dx[j] : (R)0);
+                    acc[1].add(u[j] ? -mag * dy[j] : (R)0);
+                    acc[2].add(u[j] ? -mag * dz[j] : (R)0);
+                    if (EN)
+                    {
+                        /* the pair energy from the force record's value coefficients in memory */
+                        const FRec<R> c = frec_at<true>(rb, o[j], (R *)nullptr);
+                        const R e = (R)0.5 * (c.p0 + bb * (c.p1 + bb * (c.p2 + bb * c.p3)));
+                        acc[3].add(u[j] ? e : (R)0);
+                    }
+                }
+            }
+        }
+        else if (PASS == 1)
         {
             Q q[NP];
 #pragma unroll
@@ -685,36 +731,80 @@ static __device__ __forceinline__ void flush_exact(const KDesc &D, const TileT &
 
 /* EN: the force pass also sums the pair energy into cell_pe (the density pass always keeps the
    embedding energy of each cell, a by-product of F'(rho)) */
-template <typename R, int PASS, int CAT, bool EN>
+/* The lanes of one home cell meet.  In the LDS-table kernels each wave of the
+   workgroup owns its own home cell, so the meeting is the wave's: one wave's
+   LDS accesses complete in the order it issues them, and the fences keep the
+   compiler from moving memory accesses across. */
+template <bool LT>
+static __device__ __forceinline__ void csync()
+{
+    if constexpr (LT)
+    {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    else
+        __syncthreads();
+}
+
+/* offset of the LDS table after the pair parameters in the dynamic LDS of the LDS-table kernels */
+static __host__ __device__ __forceinline__ unsigned lt_offset(int nkinds)
+{
+    return (unsigned)((sizeof(PairParam) * (size_t)nkinds * (size_t)nkinds + 15u) & ~(size_t)15u);
+}
+
+#if FMDX_EAM_LDS
+#define LW FMDX_EAM_LDS   /* waves, each with its own home cell, per workgroup of the LDS-table kernels */
+#else
+#define LW 1
+#endif
+
+/* One home cell's pair sums.  LT: the FP32 single-element EAM passes with the
+   spline table resident in LDS (k_cell_lds): the calling wave is one of LW
+   in its workgroup, each with its own tile, and the table follows the pair
+   parameters in the dynamic LDS. */
+template <typename R, int PASS, int CAT, bool EN, bool LT = false>
 static __device__ void b_cell(const KDesc &D, const int c)
 {
     typedef typename AccOf<R>::T Acc;
     typedef Prec<R> P;
+    static_assert(!LT || (sizeof(R) == sizeof(float) && CAT == CAT_EAM1 && FMDX_WAVES_PER_CELL == 1),
+                  "LDS tables: FP32 single-element EAM, one wave per cell");
     /* candidates per tile, per precision and pair category (fmdx_tuned.h): the FP32 EAM kernels
        keep a smaller tile, so that the density pass's LDS (tile plus the list places of the exact
        pairs) still lets 20 workgroups share a CU */
     enum
     {
-        CT = sizeof(R) == sizeof(double) ? (int)P::CT
+        CT = LT ? FMDX_EAM_LDS_TILE
+           : sizeof(R) == sizeof(double) ? (int)P::CT
            : CAT == CAT_EAM1 ? FMDX_TILE32_EAM : CAT == CAT_LJ ? FMDX_TILE32_LJ
            : CAT == CAT_MORSE ? FMDX_TILE32_MORSE : (int)P::CT
     };
     constexpr bool WITH_FP = PASS == 0 && (CAT == CAT_GEN || CAT == CAT_EAM1);
+    constexpr int NWS = LT ? LW : 1;                 /* home cells sharing the workgroup's LDS */
+    const int lane = LT ? (int)(threadIdx.x & (WG - 1)) : (int)threadIdx.x;   /* thread within the home cell */
+    const int ws = LT ? (int)(threadIdx.x / WG) : 0;  /* its home cell's LDS slot */
     if ((PASS == 1 && !D.has_eam) || D.cnt[c] == 0)
     {
-        if (threadIdx.x == 0)
+        if (lane == 0)
         {
             if (PASS == 1) { D.cell_emb[c] = 0.0; D.htrips[c] = -1; }
             else if (EN) D.cell_pe[c] = 0.0;
         }
         return;
     }
-    __shared__ long nb_base[27];
-    __shared__ int nb_pre[28];
-    __shared__ Shifts<R> SH;
-    __shared__ Tile<R, WITH_FP, CAT == CAT_GEN, CT> T;
+    __shared__ long nb_base_s[NWS][27];
+    __shared__ int nb_pre_s[NWS][28];
+    __shared__ Shifts<R> SH_s[NWS];
+    __shared__ Tile<R, WITH_FP, CAT == CAT_GEN, CT> T_s[NWS];
+    long *const nb_base = nb_base_s[ws];
+    int *const nb_pre = nb_pre_s[ws];
+    Shifts<R> &SH = SH_s[ws];
+    auto &T = T_s[ws];
 
     extern __shared__ PairParam pp[];   /* nkinds^2 entries, sized at launch */
+    const float4 *const lt = reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(pp) + lt_offset(D.nkinds));
     /* the stripe sums reuse the tile once the scan is over */
     double(*red)[NT] = reinterpret_cast<double(*)[NT]>(&T);
     static_assert(sizeof(T) >= 4 * NT * sizeof(double), "tile too small for the stripe sums");
@@ -723,10 +813,10 @@ static __device__ void b_cell(const KDesc &D, const int c)
     __shared__ int wcnt[2][FMDX_WAVES_PER_CELL];
     __shared__ short selfpos[FMDX_WAVES_PER_CELL > 1 ? NT : 1];
 
-    const int lane = threadIdx.x;   /* the thread within the home cell's workgroup */
-    const int wv = lane / WG;       /* its wave */
+    const int wv = lane / WG;       /* the wave within the home cell */
     const int nkk = D.nkinds * D.nkinds;
-    for (int i = lane; i < nkk; i += NT) pp[i] = D.pairs[i];
+    if (!LT)   /* the LDS-table kernels load them once */
+        for (int i = lane; i < nkk; i += NT) pp[i] = D.pairs[i];
     const int cz = c % D.nc2, ct = c / D.nc2, cy = ct % D.nc1, cx = ct / D.nc1;
     int my_cnt = 0;
     if (lane < 27)
@@ -755,7 +845,7 @@ static __device__ void b_cell(const KDesc &D, const int c)
     }
     if (lane < 27) nb_pre[lane + 1] = incl;
     if (lane == 0) nb_pre[0] = 0;
-    __syncthreads();
+    csync<LT>();
     const int ncand = __builtin_amdgcn_readfirstlane(nb_pre[27]);
 
     const int nh = __builtin_amdgcn_readfirstlane(D.cnt[c]);
@@ -836,7 +926,7 @@ static __device__ void b_cell(const KDesc &D, const int c)
         int round = 0;
         while (gpos < ncand)
         {
-            __syncthreads();   /* the previous tile is no longer read */
+            csync<LT>();   /* the previous tile is no longer read */
             /* fill the tile with the candidates that can reach the home cell: a
                candidate farther than the cutoff from the cell's box cannot pair
                with any home atom; survivors are compacted in list order */
@@ -892,7 +982,7 @@ static __device__ void b_cell(const KDesc &D, const int c)
                     /* survivors are compacted in list order across the waves */
                     const int par = batch & 1;
                     if ((lane & (WG - 1)) == 0) wcnt[par][wv] = kept;
-                    __syncthreads();
+                    csync<LT>();
                     kept = 0;
                     for (int w = 0; w < FMDX_WAVES_PER_CELL; w++)
                     {
@@ -940,7 +1030,7 @@ static __device__ void b_cell(const KDesc &D, const int c)
                 if constexpr (NSCAN) T.put(m, (R)0, (R)0, (R)0, pinf, NOKIND);
                 else T.put(m, -pinf, -pinf, -pinf, (R)0, NOKIND);
             }
-            __syncthreads();
+            csync<LT>();
             if constexpr (FMDX_WAVES_PER_CELL > 1) selft = selfpos[a];
             if (s >= S) selft = -1;
             for (int g0 = 0; g0 < trips; g0 += 64)
@@ -1005,7 +1095,7 @@ static __device__ void b_cell(const KDesc &D, const int c)
                 }
                 unsigned long long xw = 0ull;   /* FP32: trips of this word whose pairs are in the exact shell */
                 if constexpr (CAT == CAT_EAM1)
-                    flush_eam1<R, PASS, EN>(D, T, word, s, g0, S, xi, yi, zi, Fpi, p1.rc2, xw, acc);
+                    flush_eam1<R, PASS, EN, LT>(D, T, word, s, g0, S, xi, yi, zi, Fpi, p1.rc2, xw, acc, lt);
                 else if constexpr ((CAT == CAT_LJ || CAT == CAT_MORSE) && PASS == 0)
                 {
                     /* the rounding band is met so seldom that the word's bits are walked again to
@@ -1055,12 +1145,12 @@ static __device__ void b_cell(const KDesc &D, const int c)
             D.htrips[c] = wi <= HBITS_WORDS ? wi : -1;
             D.htile[c] = tiles;
         }
-        __syncthreads();   /* the tile is reused for the stripe sums */
+        csync<LT>();   /* the tile is reused for the stripe sums */
         red[0][lane] = acc[0].get();
         red[1][lane] = acc[1].get();
         red[2][lane] = acc[2].get();
         red[3][lane] = acc[3].get();
-        __syncthreads();
+        csync<LT>();
         double ecell = 0.0;
         if (lane < nhc)
         {
@@ -1127,9 +1217,9 @@ static __device__ void b_cell(const KDesc &D, const int c)
                 ecell = se;
             }
         }
-        __syncthreads();
+        csync<LT>();
         red[0][lane] = ecell;
-        __syncthreads();
+        csync<LT>();
         if (lane == 0)
             for (int l = 0; l < nhc; l++) cell_acc += red[0][l];
     }
@@ -1179,6 +1269,36 @@ static __device__ __forceinline__ int xcd_cell(const KDesc &D, int b, int n)
 template <typename R, int PASS, int CAT, bool EN>
 __global__ __launch_bounds__(NT) void k_cell(KDesc D) { b_cell<R, PASS, CAT, EN>(D, xcd_cell(D, blockIdx.x, gridDim.x)); }
 
+#if FMDX_EAM_LDS
+/* FP32 single-element EAM pass with its spline table resident in LDS: one
+   workgroup of LW waves per CU (the table, ~80 KB for the reference's 5000-
+   point grids, leaves room for one), loading the table once; each wave then
+   takes the cells of its XCD's run of the visiting order in turn, with a
+   fixed stride (cells of one system cost about the same).  The density pass
+   holds the density coefficients (c0..c3) per interval, the force pass the
+   derivative knots of rho and phi. */
+template <int PASS, bool EN>
+__global__ __launch_bounds__(WG * LW) void k_cell_lds(KDesc D)
+{
+    extern __shared__ PairParam pp[];
+    float4 *const tab = reinterpret_cast<float4 *>(reinterpret_cast<char *>(pp) + lt_offset(D.nkinds));
+    const int nkk = D.nkinds * D.nkinds;
+    for (int i = threadIdx.x; i < nkk; i += WG * LW) pp[i] = D.pairs[i];
+    const float4 *const src = PASS == 1 ? D.eam.rhoc32 : D.eam.kn32;
+    const int ntab = PASS == 1 ? D.eam.nr2 - 1 : D.eam.nr2;
+    for (int i = threadIdx.x; i < ntab; i += WG * LW) tab[i] = src[i];
+    __syncthreads();
+    const int x = blockIdx.x & 7;                 /* the XCD this workgroup runs on */
+    const int q = (int)(gridDim.x >> 3) * LW;     /* waves per XCD */
+    for (int i = (int)(blockIdx.x >> 3) * LW + (int)(threadIdx.x / WG);; i += q)
+    {
+        const int v = x + 8 * i;                  /* a block index of the one-cell-per-workgroup launch */
+        if (v >= D.ncell) break;
+        b_cell<float, PASS, CAT_EAM1, EN, true>(D, xcd_cell(D, v, D.ncell));
+    }
+}
+#endif
+
 template <typename R, int PASS, int CAT>
 __global__ __launch_bounds__(NT) void kb_cell(BatchMap M)
 {
@@ -1216,6 +1336,61 @@ int force_category(const KDesc &D)
     return CAT_GEN;
 }
 
+#if FMDX_EAM_LDS
+/* LDS of the LDS-table kernels for system D: static (LW tiles) plus dynamic (pair
+   parameters and the table); 0 when the kernels cannot hold D's table.  Both passes
+   take the same decision, so that the force pass replays the density pass's tiles. */
+static size_t lds_table_bytes(const KDesc &D)
+{
+    if (!D.eam.kn32 || !D.eam.rhoc32) return 0;
+    static int limit = -1, ncu = 0;
+    static size_t stat = 0;
+    if (limit < 0)
+    {
+        int dev = 0;
+        hipFuncAttributes a1, a0, a0e;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipFuncGetAttributes(&a1, reinterpret_cast<const void *>(&k_cell_lds<1, true>)) != hipSuccess ||
+            hipFuncGetAttributes(&a0, reinterpret_cast<const void *>(&k_cell_lds<0, false>)) != hipSuccess ||
+            hipFuncGetAttributes(&a0e, reinterpret_cast<const void *>(&k_cell_lds<0, true>)) != hipSuccess)
+        {
+            limit = 0;
+            return 0;
+        }
+        stat = a1.sharedSizeBytes > a0.sharedSizeBytes ? a1.sharedSizeBytes : a0.sharedSizeBytes;
+        if (a0e.sharedSizeBytes > stat) stat = a0e.sharedSizeBytes;
+    }
+    const size_t dyn = lt_offset(D.nkinds) + (size_t)D.eam.nr2 * sizeof(float4);
+    return ncu >= 8 && stat + dyn <= (size_t)limit ? dyn : 0;
+}
+
+static int lds_grid()
+{
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    return ncu & ~7;   /* one workgroup per CU, a whole number per XCD */
+}
+
+/* dynamic LDS beyond the default limit has to be granted per kernel first */
+template <int PASS, bool EN>
+static void launch_lds(const KDesc &D, size_t dyn, hipStream_t st)
+{
+    static size_t granted = 0;
+    if (dyn > granted)
+    {
+        /* on failure the error stays for the caller's hipGetLastError */
+        if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_cell_lds<PASS, EN>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) != hipSuccess)
+            return;
+        granted = dyn;
+    }
+    hipLaunchKernelGGL((k_cell_lds<PASS, EN>), dim3(lds_grid()), dim3(WG * LW), dyn, st, D);
+}
+#endif
+
 template <typename R, int PASS, bool EN>
 static void launch_cell_en(const KDesc &D, hipStream_t st)
 {
@@ -1223,7 +1398,17 @@ static void launch_cell_en(const KDesc &D, hipStream_t st)
     const size_t lds = pp_bytes(D);
     switch (force_category(D))
     {
-        case CAT_EAM1: hipLaunchKernelGGL((k_cell<R, PASS, CAT_EAM1, EN>), g, b, lds, st, D); break;
+        case CAT_EAM1:
+#if FMDX_EAM_LDS
+            if constexpr (sizeof(R) == sizeof(float))
+                if (const size_t dyn = lds_table_bytes(D))
+                {
+                    launch_lds<PASS, EN>(D, dyn, st);
+                    break;
+                }
+#endif
+            hipLaunchKernelGGL((k_cell<R, PASS, CAT_EAM1, EN>), g, b, lds, st, D);
+            break;
         case CAT_LJ: hipLaunchKernelGGL((k_cell<R, PASS, CAT_LJ, EN>), g, b, lds, st, D); break;
         case CAT_MORSE: hipLaunchKernelGGL((k_cell<R, PASS, CAT_MORSE, EN>), g, b, lds, st, D); break;
         default: hipLaunchKernelGGL((k_cell<R, PASS, CAT_GEN, EN>), g, b, lds, st, D); break;

@@ -9,7 +9,8 @@ from fmd_amd import build as B
 
 def test_tuned_header_defines_every_parameter():
     vals = A.read_tuned()
-    assert set(vals) == {m for m, _, _ in A.PARAMS}
+    assert {m for m, _, _ in A.PARAMS} <= set(vals)
+    assert vals["FMDX_EAM_LDS"] == 0   # the LDS-table passes are measured slower: off
     for m, cands, _ in A.PARAMS:
         assert vals[m] > 0
         assert len(cands) >= 2
@@ -55,3 +56,29 @@ def test_potential_energy_guard():
     assert A.agrees(-1.0000001, -1.0, 1e-6)
     assert not A.agrees(-1.00001, -1.0, 1e-6)
     assert not A.agrees(float("nan"), -1.0, 1e-6)
+
+
+def test_lds_table_variant_fits_one_workgroup_per_cu(tmp_path):
+    """The FP32 EAM passes with the spline table in LDS (FMDX_EAM_LDS, off by default) still
+    compile, and 16 waves with 184-candidate tiles leave room for a 5002-knot table (80 KB)
+    beside the pair parameters, within the 160 KB of one CU, at no more than 128 VGPRs."""
+    import re
+    import subprocess
+    out = str(tmp_path / "k_force_lds.s")
+    subprocess.run([B.HIPCC] + B.HIP_FLAGS + ["-DFMDX_EAM_LDS=16", "-DFMDX_EAM_LDS_TILE=184", "--cuda-device-only",
+                    "-S", os.path.join(B.DEV_SRC, A.TUNED_FILE), "-o", out], check=True, capture_output=True)
+    text = open(out).read()
+    meta = text[text.index("amdhsa.kernels:"):]
+    found = 0
+    for blk in re.split(r"\n  - ", meta)[1:]:
+        name = re.search(r"\.name:\s+(\S+)", blk)
+        if not name or "k_cell_lds" not in name.group(1):
+            continue
+        name = name.group(1)
+        get = lambda k: int(re.search(r"\." + k + r":\s+(\d+)", blk).group(1))
+        found += 1
+        assert get("max_flat_workgroup_size") == 16 * 64, name
+        assert get("group_segment_fixed_size") + 48 + 5002 * 16 <= 160 * 1024, name
+        assert get("vgpr_count") <= 128, name
+        assert get("private_segment_fixed_size") == 0, name
+    assert found >= 3   # density pass, force pass with and without the pair energy
