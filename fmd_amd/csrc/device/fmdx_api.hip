@@ -255,8 +255,8 @@ static double steep_start(const double4 *c, long m, double dr2, double rc2)
 }
 
 extern "C" int fmdx_set_eam(fmdx_system_t *s, int nelem, int nrho, double drho, int nr2, double dr2, double rc2,
-                            const double *F, const double *FDD, const double *rho, const double *rhoDD,
-                            const double *phi, const double *phiDD, const int *elem_of_kind)
+                            const double *F, const double *F_dd, const double *rho, const double *rho_dd,
+                            const double *phi, const double *phi_dd, const int *elem_of_kind)
 {
     if (!s || nelem < 1 || nelem > MAXE || nrho < 3 || nr2 < 3 || !(drho > 0) || !(dr2 > 0)) return FMDX_EARG;
     if (s->member) return FMDX_ESTATE;
@@ -264,10 +264,10 @@ extern "C" int fmdx_set_eam(fmdx_system_t *s, int nelem, int nrho, double drho, 
     std::vector<double4> hF((size_t)nelem * mr), hR((size_t)nelem * m2), hP((size_t)nelem * nelem * m2);
     for (int e = 0; e < nelem; e++)
     {
-        spline_coeffs(F + (size_t)e * nrho, FDD + (size_t)e * nrho, nrho, drho, &hF[e * mr]);
-        spline_coeffs(rho + (size_t)e * nr2, rhoDD + (size_t)e * nr2, nr2, dr2, &hR[e * m2]);
+        spline_coeffs(F + (size_t)e * nrho, F_dd + (size_t)e * nrho, nrho, drho, &hF[e * mr]);
+        spline_coeffs(rho + (size_t)e * nr2, rho_dd + (size_t)e * nr2, nr2, dr2, &hR[e * m2]);
         for (int f = 0; f < nelem; f++)
-            spline_coeffs(phi + ((size_t)e * nelem + f) * nr2, phiDD + ((size_t)e * nelem + f) * nr2, nr2, dr2,
+            spline_coeffs(phi + ((size_t)e * nelem + f) * nr2, phi_dd + ((size_t)e * nelem + f) * nr2, nr2, dr2,
                           &hP[(e * nelem + f) * m2]);
     }
     std::vector<float4> fF(hF.size()), fR(hR.size()), fP(hP.size());
@@ -301,7 +301,7 @@ extern "C" int fmdx_set_eam(fmdx_system_t *s, int nelem, int nrho, double drho, 
             const double4 r = hR[i], p = hP[i];
             const double dr = k < m2 ? r.y : r.y + 2.0 * r.z + 3.0 * r.w;
             const double dp = k < m2 ? p.y : p.y + 2.0 * p.z + 3.0 * p.w;
-            fK[k] = make_float4((float)dr, (float)dp, (float)(rhoDD[k] * h26), (float)(phiDD[k] * h26));
+            fK[k] = make_float4((float)dr, (float)dp, (float)(rho_dd[k] * h26), (float)(phi_dd[k] * h26));
         }
     }
     /* tables of an earlier call may still be read by work queued on the stream */

@@ -54,17 +54,17 @@ typedef struct
 typedef struct
 {
     char name[64];
-    double mass, latticeParameter;
-    double *F, *F_DD;           /* [Nrho] */
-    double *rho, *rhoDD;        /* [Nr2] on the r^2 grid */
+    double mass, lattice_a;
+    double *F, *Fdd;           /* [Nrho] */
+    double *rho, *rhodd;        /* [Nr2] on the r^2 grid */
 } eam_element_t;
 
 typedef struct
 {
-    int ElementsNo, Nrho, Nr, Nr2;
+    int nelem, Nrho, Nr, Nr2;
     double drho, dr, dr2, cutoff_sqr;
     eam_element_t el[HMAXE];
-    double *phi[HMAXE][HMAXE], *phiDD[HMAXE][HMAXE];
+    double *phi[HMAXE][HMAXE], *phidd[HMAXE][HMAXE];
 } eam_t;
 
 typedef struct

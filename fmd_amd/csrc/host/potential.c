@@ -100,7 +100,7 @@ static void potlist_add(fmd_t *md, struct _fmd_pot *p)
 
 static int eam_find(const eam_t *e, const char *name)
 {
-    for (int i = 0; i < e->ElementsNo; i++)
+    for (int i = 0; i < e->nelem; i++)
         if (strcmp(name, e->el[i].name) == 0) return i;
     return -1;
 }
@@ -191,10 +191,10 @@ static void r_to_r2(const eam_t *e, const double *src, double *dst)
 static void eam_free(eam_t *e)
 {
     if (!e) return;
-    for (int i = 0; i < e->ElementsNo; i++)
+    for (int i = 0; i < e->nelem; i++)
     {
-        free(e->el[i].F); free(e->el[i].F_DD); free(e->el[i].rho); free(e->el[i].rhoDD);
-        for (int j = 0; j <= i; j++) { free(e->phi[i][j]); free(e->phiDD[i][j]); }
+        free(e->el[i].F); free(e->el[i].Fdd); free(e->el[i].rho); free(e->el[i].rhodd);
+        for (int j = 0; j <= i; j++) { free(e->phi[i][j]); free(e->phidd[i][j]); }
     }
     free(e);
 }
@@ -214,8 +214,8 @@ static eam_t *load_setfl(fmd_t *md, const char *path)
     double *tmp = NULL;
     for (int i = 0; i < 3; i++)
         if (!fgets(str, sizeof str, fp)) goto corrupt;
-    if (fscanf(fp, "%d", &e->ElementsNo) != 1 || e->ElementsNo < 1 || e->ElementsNo > HMAXE) goto corrupt;
-    for (int i = 0; i < e->ElementsNo; i++)
+    if (fscanf(fp, "%d", &e->nelem) != 1 || e->nelem < 1 || e->nelem > HMAXE) goto corrupt;
+    for (int i = 0; i < e->nelem; i++)
         if (fscanf(fp, "%63s", e->el[i].name) != 1) goto corrupt;
     double cutoff;
     if (fscanf(fp, "%d%lf%d%lf%lf", &e->Nrho, &e->drho, &e->Nr, &e->dr, &cutoff) != 5) goto corrupt;
@@ -225,10 +225,10 @@ static eam_t *load_setfl(fmd_t *md, const char *path)
     e->cutoff_sqr = cutoff * cutoff;
     e->dr2 = ((e->Nr - 1) * e->dr) * ((e->Nr - 1) * e->dr) / (e->Nr2 - 1);
     tmp = calloc((size_t)e->Nr + 1, sizeof(double));   /* one zero past the table: see r_to_r2 */
-    for (int i = 0; i < e->ElementsNo; i++)
+    for (int i = 0; i < e->nelem; i++)
     {
         eam_element_t *el = &e->el[i];
-        if (fscanf(fp, "%1023s%lf%lf%1023s", str, &el->mass, &el->latticeParameter, str) != 4) goto corrupt;
+        if (fscanf(fp, "%1023s%lf%lf%1023s", str, &el->mass, &el->lattice_a, str) != 4) goto corrupt;
         el->mass /= MD_MASS_UNIT;
         el->F = malloc(e->Nrho * sizeof(double));
         for (int j = 0; j < e->Nrho; j++)
@@ -238,12 +238,12 @@ static eam_t *load_setfl(fmd_t *md, const char *path)
             if (fscanf(fp, "%lf", &tmp[j]) != 1) goto corrupt;
         tmp[e->Nr - 1] = tmp[e->Nr - 2] = 0.;
         r_to_r2(e, tmp, el->rho);
-        el->F_DD = malloc(e->Nrho * sizeof(double));
-        el->rhoDD = malloc(e->Nr2 * sizeof(double));
-        fmdh_spline_prepare(e->drho, el->F, e->Nrho, el->F_DD);
-        fmdh_spline_prepare(e->dr2, el->rho, e->Nr2, el->rhoDD);
+        el->Fdd = malloc(e->Nrho * sizeof(double));
+        el->rhodd = malloc(e->Nr2 * sizeof(double));
+        fmdh_spline_prepare(e->drho, el->F, e->Nrho, el->Fdd);
+        fmdh_spline_prepare(e->dr2, el->rho, e->Nr2, el->rhodd);
     }
-    for (int i = 0; i < e->ElementsNo; i++)
+    for (int i = 0; i < e->nelem; i++)
         for (int j = 0; j <= i; j++)
         {
             for (int k = 0; k < e->Nr - 2; k++)
@@ -256,9 +256,9 @@ static eam_t *load_setfl(fmd_t *md, const char *path)
             e->phi[i][j] = malloc(e->Nr2 * sizeof(double));
             r_to_r2(e, tmp, e->phi[i][j]);
             e->phi[j][i] = e->phi[i][j];
-            e->phiDD[i][j] = malloc(e->Nr2 * sizeof(double));
-            fmdh_spline_prepare(e->dr2, e->phi[i][j], e->Nr2, e->phiDD[i][j]);
-            e->phiDD[j][i] = e->phiDD[i][j];
+            e->phidd[i][j] = malloc(e->Nr2 * sizeof(double));
+            fmdh_spline_prepare(e->dr2, e->phi[i][j], e->Nr2, e->phidd[i][j]);
+            e->phidd[j][i] = e->phidd[i][j];
         }
     free(tmp);
     fclose(fp);
@@ -306,7 +306,7 @@ fmd_real_t fmd_pot_eam_getLatticeParameter(fmd_t *md, fmd_pot_t *pot, fmd_string
         fmdh_err_unacceptable_int(md, false, HERE, "EAM element", -1);
         return 0.;
     }
-    return e->el[i].latticeParameter;
+    return e->el[i].lattice_a;
 }
 
 /* ------------------------------------------------------ categories, cutoff */
@@ -431,27 +431,27 @@ bool fmdh_push_potentials(fmd_t *md)
     if (md->eam)
     {
         eam_t *e = md->eam;
-        int ne = e->ElementsNo;
+        int ne = e->nelem;
         double *F = malloc(sizeof(double) * ne * e->Nrho), *FDD = malloc(sizeof(double) * ne * e->Nrho);
-        double *rho = malloc(sizeof(double) * ne * e->Nr2), *rhoDD = malloc(sizeof(double) * ne * e->Nr2);
-        double *phi = malloc(sizeof(double) * ne * ne * e->Nr2), *phiDD = malloc(sizeof(double) * ne * ne * e->Nr2);
+        double *rho = malloc(sizeof(double) * ne * e->Nr2), *rhodd = malloc(sizeof(double) * ne * e->Nr2);
+        double *phi = malloc(sizeof(double) * ne * ne * e->Nr2), *phidd = malloc(sizeof(double) * ne * ne * e->Nr2);
         for (int a = 0; a < ne; a++)
         {
             memcpy(F + (size_t)a * e->Nrho, e->el[a].F, sizeof(double) * e->Nrho);
-            memcpy(FDD + (size_t)a * e->Nrho, e->el[a].F_DD, sizeof(double) * e->Nrho);
+            memcpy(FDD + (size_t)a * e->Nrho, e->el[a].Fdd, sizeof(double) * e->Nrho);
             memcpy(rho + (size_t)a * e->Nr2, e->el[a].rho, sizeof(double) * e->Nr2);
-            memcpy(rhoDD + (size_t)a * e->Nr2, e->el[a].rhoDD, sizeof(double) * e->Nr2);
+            memcpy(rhodd + (size_t)a * e->Nr2, e->el[a].rhodd, sizeof(double) * e->Nr2);
             for (int b = 0; b < ne; b++)
             {
                 memcpy(phi + ((size_t)a * ne + b) * e->Nr2, e->phi[a][b], sizeof(double) * e->Nr2);
-                memcpy(phiDD + ((size_t)a * ne + b) * e->Nr2, e->phiDD[a][b], sizeof(double) * e->Nr2);
+                memcpy(phidd + ((size_t)a * ne + b) * e->Nr2, e->phidd[a][b], sizeof(double) * e->Nr2);
             }
         }
         int eok[HMAXK];
         for (unsigned k = 0; k < HMAXK; k++) eok[k] = k < n ? md->kinds[k].eam_element : -1;
-        int r = fmdx_set_eam(md->dev, ne, e->Nrho, e->drho, e->Nr2, e->dr2, e->cutoff_sqr, F, FDD, rho, rhoDD, phi,
-                             phiDD, eok);
-        free(F); free(FDD); free(rho); free(rhoDD); free(phi); free(phiDD);
+        int r = fmdx_set_eam(md->dev, ne, e->Nrho, e->drho, e->Nr2, e->dr2, e->cutoff_sqr, F, FDD, rho, rhodd, phi,
+                             phidd, eok);
+        free(F); free(FDD); free(rho); free(rhodd); free(phi); free(phidd);
         if (r != FMDX_OK)
         {
             fmdh_err_function_failed(md, true, HERE, "fmdx_set_eam");

@@ -17,8 +17,10 @@
  * The sums over atoms, the presolve and the FD sub-steps run on the device
  * (fmdx_ttm_*); the host walks the field list to raise the events in the
  * reference's order, downloading arrays only when a handler asks for them.
- * Only TTM turies of 1 x 1 x n cells are computed, as in the reference,
- * whose 3-D updaters are absent (ttm.c:449-453, 489-493).
+ * TTM turies are computed: 1 x 1 x n cells with the reference's 1-D
+ * update, and type 1 grids with the 3-D update of the electron temperature
+ * (the reference's own 3-D updaters are absent, ttm.c:449-453, 489-493).
+ * Other (custom) turi fields report FMD_ERR_NOT_SUPPORTED_YET.
  */
 #include <math.h>
 #include <stdlib.h>

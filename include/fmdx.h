@@ -146,11 +146,11 @@ int fmdx_set_precision(fmdx_system_t *s, int precision);
 
 /* pair table: LJ p0 = 4 eps, p1 = sigma^2; Morse p0 = D0, p1 = alpha, p2 = r0; EAM p0 = iloc, p1 = jloc */
 int fmdx_set_pair(fmdx_system_t *s, int k1, int k2, int cat, double rc2, double p0, double p1, double p2);
-/* EAM tables on the reference's uniform grids: F/FDD [nelem][nrho] on rho = i*drho,
-   rho/rhoDD [nelem][nr2] and phi/phiDD [nelem][nelem][nr2] on r^2 = i*dr2 */
+/* EAM tables on the reference's uniform grids: F/F_dd [nelem][nrho] on rho = i*drho,
+   rho/rho_dd [nelem][nr2] and phi/phi_dd [nelem][nelem][nr2] on r^2 = i*dr2 */
 int fmdx_set_eam(fmdx_system_t *s, int nelem, int nrho, double drho, int nr2, double dr2, double rc2,
-                 const double *F, const double *FDD, const double *rho, const double *rhoDD,
-                 const double *phi, const double *phiDD, const int *elem_of_kind);
+                 const double *F, const double *F_dd, const double *rho, const double *rho_dd,
+                 const double *phi, const double *phi_dd, const int *elem_of_kind);
 
 /* atoms in cell order: cell_count[c] atoms of interior cell c (LOOP3D order, z fastest),
    x and v interleaved [3*natoms] */
