@@ -818,6 +818,14 @@ static __device__ void b_cell(const KDesc &D, const int c)
     if (!LT)   /* the LDS-table kernels load them once */
         for (int i = lane; i < nkk; i += NT) pp[i] = D.pairs[i];
     const int cz = c % D.nc2, ct = c / D.nc2, cy = ct % D.nc1, cx = ct / D.nc1;
+    /* the density pass's record of this cell, loaded here so that it arrives with the neighbour counts */
+    int hw_ld = -1;
+    unsigned long long tiles_ld = 0ull;
+    if (PASS == 0 && D.has_eam)
+    {
+        hw_ld = D.htrips[c];
+        tiles_ld = D.htile[c];
+    }
     int my_cnt = 0;
     if (lane < 27)
     {
@@ -871,16 +879,14 @@ static __device__ void b_cell(const KDesc &D, const int c)
     const R qnan = (R)__builtin_nan("");
     const long hbase = (long)c * D.cap;
     const bool one_block = nh > 0 && nh <= NT;
-    int hw = -1;
-    if (PASS == 0 && D.has_eam) hw = __builtin_amdgcn_readfirstlane(D.htrips[c]);
+    const int hw = __builtin_amdgcn_readfirstlane(hw_ld);
     /* a density pass without hit-word storage (hbits null) records nothing: the force pass rescans */
     const bool record = PASS == 1 && one_block && D.hbits != nullptr;
     if (PASS == 1 && lane == 0 && !record) D.htrips[c] = -1;
     bool replay = PASS == 0 && one_block && hw >= 0;
     /* tile sizes of the density pass's first four rounds (16 bits each): the force pass replays
        a round only if it staged as many candidates, and rescans from the first round that differs */
-    unsigned long long tiles = 0ull;
-    if (replay) tiles = D.htile[c];
+    unsigned long long tiles = replay ? tiles_ld : 0ull;
     PairParam p1;
     if (CAT != CAT_GEN) p1 = pp[0];
     /* the home cell's box in the frame of the staged positions, and the culling
@@ -927,6 +933,9 @@ static __device__ void b_cell(const KDesc &D, const int c)
         while (gpos < ncand)
         {
             csync<LT>();   /* the previous tile is no longer read */
+            /* the round's first recorded hit word loads while the tile is staged */
+            const unsigned long long wfirst =
+                (replay && wi < hw) ? D.hbits[((long)c * HBITS_WORDS + wi) * NT + lane] : 0ull;
             /* fill the tile with the candidates that can reach the home cell: a
                candidate farther than the cutoff from the cell's box cannot pair
                with any home atom; survivors are compacted in list order */
@@ -965,13 +974,13 @@ static __device__ void b_cell(const KDesc &D, const int c)
                     const R ex = fmax(fmax(bx0 - x, x - bx1), (R)0);
                     const R ey = fmax(fmax(by0 - y, y - by1), (R)0);
                     const R ez = fmax(fmax(bz0 - z, z - bz1), (R)0);
+                    /* the candidate's F' (and kind, which only the generic kernel dispatches on) load
+                       with its position, whether or not it survives the culling: a load issued after
+                       the test would add one more round trip to every batch (force pass 1.88 -> 1.80 ms
+                       FP64, 1.60 -> 1.52 ms mixed, on the 4M-atom copper step) */
+                    if (CAT == CAT_GEN) kj = D.kind[js];
+                    if (WITH_FP) fpv = (R)D.aux[js];
                     keep = ex * ex + ey * ey + ez * ez <= cull2 && (ag < 0 || D.grp[js] == ag);
-                    if (keep)
-                    {
-                        /* only the generic kernel dispatches on the candidate's kind */
-                        if (CAT == CAT_GEN) kj = D.kind[js];
-                        if (WITH_FP) fpv = (R)D.aux[js];
-                    }
                 }
                 const unsigned long long bal = __ballot(keep);
                 int off = nt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
@@ -1038,7 +1047,7 @@ static __device__ void b_cell(const KDesc &D, const int c)
                 const int ng = min(64, trips - g0);
                 unsigned long long word;
                 if (replay)
-                    word = wi < hw ? D.hbits[((long)c * HBITS_WORDS + wi) * NT + lane] : 0ull;
+                    word = g0 == 0 ? wfirst : (wi < hw ? D.hbits[((long)c * HBITS_WORDS + wi) * NT + lane] : 0ull);
                 else
                 {
                     /* bits are shifted in from the last trip down, so trip t lands on bit t */
