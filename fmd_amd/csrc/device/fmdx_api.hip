@@ -40,7 +40,7 @@ static void free_slots(KDesc &D)
 {
     dfree(D.x); dfree(D.y); dfree(D.z); dfree(D.vx); dfree(D.vy); dfree(D.vz);
     dfree(D.fx); dfree(D.fy); dfree(D.fz); dfree(D.aux);
-    dfree(D.xr); dfree(D.yr); dfree(D.zr);
+    dfree(D.xq);
     dfree(D.grp); dfree(D.aid); dfree(D.kind);
     dfree(D.mv); dfree(D.code); dfree(D.bkey); dfree(D.bid);
     dfree(D.mvx); dfree(D.mvy); dfree(D.mvz); dfree(D.mvvx); dfree(D.mvvy); dfree(D.mvvz);
@@ -57,7 +57,7 @@ static int alloc_slots(KDesc &D, int cap, unsigned nmov)
     r |= dalloc(&D.vx, ns); r |= dalloc(&D.vy, ns); r |= dalloc(&D.vz, ns);
     r |= dalloc(&D.fx, ns); r |= dalloc(&D.fy, ns); r |= dalloc(&D.fz, ns);
     r |= dalloc(&D.aux, ns);
-    r |= dalloc(&D.xr, ns); r |= dalloc(&D.yr, ns); r |= dalloc(&D.zr, ns);
+    r |= dalloc(&D.xq, ns);
     r |= dalloc(&D.grp, ns); r |= dalloc(&D.aid, ns); r |= dalloc(&D.kind, ns);
     r |= dalloc(&D.mv, ns); r |= dalloc(&D.code, ns);
     r |= dalloc(&D.bkey, ns); r |= dalloc(&D.bid, ns);
@@ -678,7 +678,7 @@ extern "C" int fmdx_set_capacity(fmdx_system_t *s, unsigned cap)
     if (cap < ctr[2]) return FMDX_EARG;
     KDesc N = s->D;
     N.x = N.y = N.z = N.vx = N.vy = N.vz = N.fx = N.fy = N.fz = N.aux = nullptr;
-    N.xr = N.yr = N.zr = nullptr;
+    N.xq = nullptr;
     N.grp = nullptr; N.aid = nullptr; N.kind = nullptr; N.mv = nullptr; N.code = nullptr;
     N.bkey = nullptr; N.bid = nullptr;
     N.mvx = N.mvy = N.mvz = N.mvvx = N.mvvy = N.mvvz = nullptr;

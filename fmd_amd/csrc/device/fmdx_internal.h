@@ -8,8 +8,8 @@
  *             kernels by shifting positions by the box length.
  *   slots   : cell c owns slots [c*cap, c*cap + cnt[c]); the within-cell order
  *             of those slots is the reference's within-cell atom order.
- *   per slot: x y z vx vy vz fx fy fz aux (f64), xr yr zr (f32, cell-relative,
- *             mixed build), grp (i32), aid (u32), kind (u8), mv (i32 binning
+ *   per slot: x y z vx vy vz fx fy fz aux (f64), xq (f32 x y z cell-relative
+ *             and F', mixed build), grp (i32), aid (u32), kind (u8), mv (i32 binning
  *             destination during a re-binning, the atom's turi index during
  *             the TTM field sums), code (u8 migration code)
  */
@@ -82,7 +82,7 @@ struct KDesc
     double h0, h1, h2;
     /* slots */
     double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz, *aux;
-    float *xr, *yr, *zr;
+    float4 *xq;   /* mixed build: x y z relative to the home cell origin (FP32), and F'(rho) in w */
     int *grp;
     unsigned *aid;
     uint8_t *kind;

@@ -71,9 +71,13 @@ template <> struct Prec<double>
     static __device__ __forceinline__ const double4 *rho(const EamDev &E) { return E.rhoc; }
     static __device__ __forceinline__ const double4 *phi(const EamDev &E) { return E.phic; }
     static __device__ __forceinline__ const void *rf(const EamDev &E) { return E.rf; }
-    static __device__ __forceinline__ const double *px(const KDesc &D) { return D.x; }
-    static __device__ __forceinline__ const double *py(const KDesc &D) { return D.y; }
-    static __device__ __forceinline__ const double *pz(const KDesc &D) { return D.z; }
+    /* the slot's position and (FP) F' */
+    template <bool FP>
+    static __device__ __forceinline__ void load(const KDesc &D, long s, double &x, double &y, double &z, double &f)
+    {
+        x = D.x[s]; y = D.y[s]; z = D.z[s];
+        if (FP) f = D.aux[s];
+    }
 };
 template <> struct Prec<float>
 {
@@ -82,9 +86,14 @@ template <> struct Prec<float>
     static __device__ __forceinline__ const float4 *rho(const EamDev &E) { return E.rhoc32; }
     static __device__ __forceinline__ const float4 *phi(const EamDev &E) { return E.phic32; }
     static __device__ __forceinline__ const void *rf(const EamDev &E) { return E.rf32; }
-    static __device__ __forceinline__ const float *px(const KDesc &D) { return D.xr; }
-    static __device__ __forceinline__ const float *py(const KDesc &D) { return D.yr; }
-    static __device__ __forceinline__ const float *pz(const KDesc &D) { return D.zr; }
+    /* the slot's packed record: one 16-byte load for the position and F' */
+    template <bool FP>
+    static __device__ __forceinline__ void load(const KDesc &D, long s, float &x, float &y, float &z, float &f)
+    {
+        const float4 q = D.xq[s];
+        x = q.x; y = q.y; z = q.z;
+        if (FP) f = q.w;
+    }
 };
 
 /* LDS candidate tile (positions in the home frame, F' of the candidate, and
@@ -921,7 +930,11 @@ static __device__ void b_cell(const KDesc &D, const int c)
         {
             ki = D.kind[is];
             act = (ag < 0 || D.grp[is] == ag);
-            if (act) { xi = P::px(D)[is]; yi = P::py(D)[is]; zi = P::pz(D)[is]; }
+            if (act)
+            {
+                R unused;
+                P::template load<false>(D, is, xi, yi, zi, unused);
+            }
             if (PASS == 0 && D.has_eam) Fpi = (R)D.aux[is];
         }
         const PairParam *ppi = pp + ki * D.nkinds;
@@ -968,9 +981,10 @@ static __device__ void b_cell(const KDesc &D, const int c)
                         else hi = mid;
                     }
                     const long js = nb_base[lo] + (g - nb_pre[lo]);
-                    x = P::px(D)[js] + SH.sx(D, lo);
-                    y = P::py(D)[js] + SH.sy(D, lo);
-                    z = P::pz(D)[js] + SH.sz(D, lo);
+                    P::template load<WITH_FP>(D, js, x, y, z, fpv);
+                    x = x + SH.sx(D, lo);
+                    y = y + SH.sy(D, lo);
+                    z = z + SH.sz(D, lo);
                     const R ex = fmax(fmax(bx0 - x, x - bx1), (R)0);
                     const R ey = fmax(fmax(by0 - y, y - by1), (R)0);
                     const R ez = fmax(fmax(bz0 - z, z - bz1), (R)0);
@@ -979,7 +993,6 @@ static __device__ void b_cell(const KDesc &D, const int c)
                        the test would add one more round trip to every batch (force pass 1.88 -> 1.80 ms
                        FP64, 1.60 -> 1.52 ms mixed, on the 4M-atom copper step) */
                     if (CAT == CAT_GEN) kj = D.kind[js];
-                    if (WITH_FP) fpv = (R)D.aux[js];
                     keep = ex * ex + ey * ey + ez * ez <= cull2 && (ag < 0 || D.grp[js] == ag);
                 }
                 const unsigned long long bal = __ballot(keep);
@@ -1189,6 +1202,7 @@ static __device__ void b_cell(const KDesc &D, const int c)
                     ecell = sp_val<double, double4>(q, b);
                 }
                 D.aux[is] = Fp;
+                if constexpr (sizeof(R) == sizeof(float)) D.xq[is].w = (float)Fp;   /* for the force pass's staging */
             }
             else
             {
@@ -1320,9 +1334,8 @@ static __device__ void b_relpos(const KDesc &D, int blk)
 {
     each_slot(D, blk, [&](long slot, int c) {
         const int cz = c % D.nc2, t = c / D.nc2, cy = t % D.nc1, cx = t / D.nc1;
-        D.xr[slot] = (float)(D.x[slot] - cx * D.h0);
-        D.yr[slot] = (float)(D.y[slot] - cy * D.h1);
-        D.zr[slot] = (float)(D.z[slot] - cz * D.h2);
+        D.xq[slot] = make_float4((float)(D.x[slot] - cx * D.h0), (float)(D.y[slot] - cy * D.h1),
+                                 (float)(D.z[slot] - cz * D.h2), 0.0f);
     });
 }
 __global__ __launch_bounds__(256) void k_relpos(KDesc D) { b_relpos(D, blockIdx.x); }
