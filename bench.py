@@ -34,15 +34,17 @@ from fmd_amd import workloads as W  # noqa: E402
 
 METRIC = "atom-steps/s EAM Cu 4M atoms (%HBM roofline); batched jobs/s + p99 vs plain"
 HBM_PEAK_GBS = 8000.0
-# algorithmic HBM bytes per atom of the EAM force kernel (pass 0): read x (24) + F' (8), write F (24)
-PASS0_BYTES = {F.FMDX_PREC_FP64: 56.0, F.FMDX_PREC_MIXED: 12.0 + 8.0 + 24.0}
+# algorithmic HBM bytes per atom of the EAM force kernel (pass 0): read x (24) + F' (8), write F (24);
+# mixed: read the packed FP32 record x y z F' (16), write F (24)
+PASS0_BYTES = {F.FMDX_PREC_FP64: 56.0, F.FMDX_PREC_MIXED: 16.0 + 24.0}
 # the force pass as the timed steps run it: every step but the last skips the pair energy
 # (k_cell<R, pass 0, single-element EAM, energy>)
 PASS0_KERNEL = {F.FMDX_PREC_FP64: "k_cell<double,0,1,false>", F.FMDX_PREC_MIXED: "k_cell<float,0,1,false>"}
 # HBM bytes per launch of the dominant kernels, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "eam_cu_4m_traffic.json")
-# algorithmic HBM bytes per atom-step of the whole EAM step (integrate 120 + bin 8 + rho 32 + force 56)
-STEP_BYTES = {F.FMDX_PREC_FP64: 216.0, F.FMDX_PREC_MIXED: 120.0 + 8.0 + 28.0 + 12.0 + 44.0}
+# algorithmic HBM bytes per atom-step of the whole EAM step (integrate 120 + bin 8 + rho 32 + force 56);
+# mixed: integrate 120 + bin 8 + FP32 record (read x 24, write 16) + rho (read 16, write F' 8 + 4) + force 40
+STEP_BYTES = {F.FMDX_PREC_FP64: 216.0, F.FMDX_PREC_MIXED: 120.0 + 8.0 + 40.0 + 28.0 + 40.0}
 
 
 def log(*a):
