@@ -15,7 +15,8 @@
  * batch.  The lists are ordered so that the round-robin workgroup dispatch
  * gives each XCD a contiguous run of items (neighbour cells share its L2).
  * A whole velocity-Verlet step is captured once in a HIP graph and replayed,
- * which leaves one host launch per step.
+ * which leaves one host launch per step: two graphs, one that also sums the
+ * pair energies for the steps whose status is read, one without them.
  *
  * Between steps the host reads every member's status word with one copy
  * (kb_collect) to grow slot capacities before they can overflow and to
@@ -43,8 +44,8 @@ struct fmdx_batch
     BatchPlan plan;
     int dirty;
     int use_graph;
-    hipGraph_t graph;
-    hipGraphExec_t gexec;
+    hipGraph_t graph[2];       /* [energy]: the step without / with the pair energies */
+    hipGraphExec_t gexec[2];
     int graph_ok;
     int check_every;
 };
@@ -68,10 +69,11 @@ static void xcd_order(std::vector<int2> &v)
 static void graph_drop(fmdx_batch_t *b)
 {
     if (b->graph_ok)
-    {
-        hipGraphExecDestroy(b->gexec);
-        hipGraphDestroy(b->graph);
-    }
+        for (int k = 0; k < 2; k++)
+        {
+            hipGraphExecDestroy(b->gexec[k]);
+            hipGraphDestroy(b->graph[k]);
+        }
     b->graph_ok = 0;
 }
 
@@ -164,10 +166,11 @@ static int rebuild(fmdx_batch_t *b)
     return FMDX_OK;
 }
 
-static hipError_t enqueue_step(fmdx_batch_t *b)
+/* one step; `energy` also sums the pair energies, which only a status read needs */
+static hipError_t enqueue_step(fmdx_batch_t *b, int energy)
 {
     hipError_t e = launchb_vv_start(b->plan, b->st);
-    if (e == hipSuccess) e = launchb_forces(b->plan, b->st, b->precision);
+    if (e == hipSuccess) e = launchb_forces(b->plan, b->st, b->precision, energy);
     if (e == hipSuccess) e = launchb_vv_finish(b->plan, b->st);
     return e;
 }
@@ -175,21 +178,34 @@ static hipError_t enqueue_step(fmdx_batch_t *b)
 static int capture(fmdx_batch_t *b)
 {
     graph_drop(b);
-    CK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-    hipError_t e = enqueue_step(b);
-    hipGraph_t g = nullptr;
-    hipError_t e2 = hipStreamEndCapture(b->st, &g);
-    if (e != hipSuccess || e2 != hipSuccess)
+    hipGraph_t g[2] = {nullptr, nullptr};
+    hipGraphExec_t x[2] = {nullptr, nullptr};
+    int ok = 1;
+    for (int k = 0; k < 2 && ok; k++)
     {
-        if (g) hipGraphDestroy(g);
+        if (hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        {
+            ok = 0;
+            break;
+        }
+        const hipError_t e = enqueue_step(b, k);
+        const hipError_t e2 = hipStreamEndCapture(b->st, &g[k]);
+        ok = e == hipSuccess && e2 == hipSuccess && hipGraphInstantiate(&x[k], g[k], nullptr, nullptr, 0) == hipSuccess;
+    }
+    if (!ok)
+    {
+        for (int k = 0; k < 2; k++)
+        {
+            if (x[k]) hipGraphExecDestroy(x[k]);
+            if (g[k]) hipGraphDestroy(g[k]);
+        }
         return FMDX_EHIP;
     }
-    if (hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0) != hipSuccess)
+    for (int k = 0; k < 2; k++)
     {
-        hipGraphDestroy(g);
-        return FMDX_EHIP;
+        b->graph[k] = g[k];
+        b->gexec[k] = x[k];
     }
-    b->graph = g;
     b->graph_ok = 1;
     return FMDX_OK;
 }
@@ -291,7 +307,7 @@ extern "C" int fmdx_batch_add(fmdx_batch_t *b, fmdx_system_t *s, double dt, int 
     if (r) return r;
     CK(hipStreamSynchronize(s->st));
     set_step(s->D, s->mass, -1, dt, thermostat, tau, T0);
-    s->D.energy = 1;   /* every batched step sums the energies (the status record carries them) */
+    s->D.energy = 1;   /* its own force calls sum the energies; batched steps do on status steps */
     s->member = 1;
     b->sys.push_back(s);
     b->dirty = 1;
@@ -343,9 +359,11 @@ extern "C" int fmdx_batch_step(fmdx_batch_t *b, int nsteps)
     if (b->use_graph && !b->graph_ok && (r = capture(b))) return r;
     for (int k = 0; k < nsteps; k++)
     {
-        if (b->use_graph) CK(hipGraphLaunch(b->gexec, b->st));
-        else CK(enqueue_step(b));
-        if ((k + 1) % b->check_every == 0 || k + 1 == nsteps)
+        /* the steps after which the status is read sum the pair energies */
+        const int energy = (k + 1) % b->check_every == 0 || k + 1 == nsteps;
+        if (b->use_graph) CK(hipGraphLaunch(b->gexec[energy], b->st));
+        else CK(enqueue_step(b, energy));
+        if (energy)
         {
             if ((r = collect(b))) return r;
             if ((r = grow(b))) return r;

@@ -230,7 +230,8 @@ hipError_t launchb_vv_start(const BatchPlan &P, hipStream_t st);
 hipError_t launchb_vv_finish(const BatchPlan &P, hipStream_t st);
 hipError_t launchb_thermo(const BatchPlan &P, hipStream_t st);
 hipError_t launchb_reduce_pe(const BatchPlan &P, hipStream_t st);
-hipError_t launchb_forces(const BatchPlan &P, hipStream_t st, int precision);
+/* energy: also the pair energies and each member's PE sum (steps whose status is read) */
+hipError_t launchb_forces(const BatchPlan &P, hipStream_t st, int precision, int energy);
 hipError_t launchb_collect(const BatchPlan &P, hipStream_t st, unsigned *status, double *thermo);
 hipError_t launch_add_velocity(const KDesc &D, hipStream_t st, int group, double vx, double vy, double vz);
 hipError_t launch_relayout(const KDesc &Dold, const KDesc &Dnew, hipStream_t st);

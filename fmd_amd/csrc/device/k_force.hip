@@ -1322,12 +1322,12 @@ __global__ __launch_bounds__(WG * LW) void k_cell_lds(KDesc D)
 }
 #endif
 
-template <typename R, int PASS, int CAT>
+template <typename R, int PASS, int CAT, bool EN>
 __global__ __launch_bounds__(NT) void kb_cell(BatchMap M)
 {
     const int2 it = M.items[blockIdx.x];
     const KDesc &D = M.descs[it.x];
-    b_cell<R, PASS, CAT, true>(D, it.y);
+    b_cell<R, PASS, CAT, EN>(D, it.y);
 }
 /* FP32 positions relative to the origin of the home cell, from the FP64 positions */
 static __device__ void b_relpos(const KDesc &D, int blk)
@@ -1479,7 +1479,7 @@ hipError_t launch_forces(const KDesc &D, hipStream_t st, int precision)
     return D.energy ? launch_reduce_pe(D, st) : hipSuccess;
 }
 
-template <typename R, int PASS>
+template <typename R, int PASS, bool EN>
 static void launchb_cells(const BatchPlan &P, hipStream_t st, const int2 *const *lists, const int *n)
 {
     const size_t ppb = sizeof(PairParam) * P.max_nkinds * P.max_nkinds;
@@ -1490,32 +1490,40 @@ static void launchb_cells(const BatchPlan &P, hipStream_t st, const int2 *const 
         const dim3 g(n[cat]), b(NT);
         switch (cat)
         {
-            case CAT_EAM1: hipLaunchKernelGGL((kb_cell<R, PASS, CAT_EAM1>), g, b, ppb, st, mc); break;
-            case CAT_LJ: hipLaunchKernelGGL((kb_cell<R, PASS, CAT_LJ>), g, b, ppb, st, mc); break;
-            case CAT_MORSE: hipLaunchKernelGGL((kb_cell<R, PASS, CAT_MORSE>), g, b, ppb, st, mc); break;
-            default: hipLaunchKernelGGL((kb_cell<R, PASS, CAT_GEN>), g, b, ppb, st, mc); break;
+            case CAT_EAM1: hipLaunchKernelGGL((kb_cell<R, PASS, CAT_EAM1, EN>), g, b, ppb, st, mc); break;
+            case CAT_LJ: hipLaunchKernelGGL((kb_cell<R, PASS, CAT_LJ, EN>), g, b, ppb, st, mc); break;
+            case CAT_MORSE: hipLaunchKernelGGL((kb_cell<R, PASS, CAT_MORSE, EN>), g, b, ppb, st, mc); break;
+            default: hipLaunchKernelGGL((kb_cell<R, PASS, CAT_GEN, EN>), g, b, ppb, st, mc); break;
         }
     }
 }
 
 /* every system of a batch: relative positions (mixed), the density pass over
-   the EAM systems, the force pass, energies; one launch per pair category */
-hipError_t launchb_forces(const BatchPlan &P, hipStream_t st, int precision)
+   the EAM systems, the force pass and, with `energy`, the pair energies and
+   their sums; one launch per pair category */
+template <bool EN>
+static void launchb_passes(const BatchPlan &P, hipStream_t st, int precision)
 {
-    if (P.nsys == 0) return hipSuccess;
     if (precision == FMDX_PREC_MIXED)
     {
         BatchMap ms = {P.descs, P.slot_items};
         hipLaunchKernelGGL(kb_relpos, dim3(P.n_slot), dim3(TPB), 0, st, ms);
-        launchb_cells<float, 1>(P, st, P.cell1, P.n_cell1);
-        launchb_cells<float, 0>(P, st, P.cell0, P.n_cell0);
+        launchb_cells<float, 1, true>(P, st, P.cell1, P.n_cell1);
+        launchb_cells<float, 0, EN>(P, st, P.cell0, P.n_cell0);
     }
     else
     {
-        launchb_cells<double, 1>(P, st, P.cell1, P.n_cell1);
-        launchb_cells<double, 0>(P, st, P.cell0, P.n_cell0);
+        launchb_cells<double, 1, true>(P, st, P.cell1, P.n_cell1);
+        launchb_cells<double, 0, EN>(P, st, P.cell0, P.n_cell0);
     }
+}
+
+hipError_t launchb_forces(const BatchPlan &P, hipStream_t st, int precision, int energy)
+{
+    if (P.nsys == 0) return hipSuccess;
+    if (energy) launchb_passes<true>(P, st, precision);
+    else launchb_passes<false>(P, st, precision);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return launchb_reduce_pe(P, st);
+    return energy ? launchb_reduce_pe(P, st) : hipSuccess;
 }
