@@ -199,6 +199,10 @@ struct AccD
     __device__ __forceinline__ void add(double x) { v += x; }
     __device__ __forceinline__ double get() const { return v; }
 };
+/* FP32 compensated (Kahan) sum.  The flush loops add the two to four pair
+   terms of one iteration in plain FP32 first and compensate once per
+   iteration: one rounding of a few terms' partial sum (relative 6e-8 of it)
+   instead of three extra operations per term. */
 struct AccF
 {
     float s, c;
@@ -592,16 +596,20 @@ static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T
             Q q[NP];
 #pragma unroll
             for (int j = 0; j < NP; j++) q[j] = rho_at(rb, o[j], (Q *)nullptr);
+            R part = (R)0;   /* FP32: the iteration's pair terms summed before one compensated add */
 #pragma unroll
             for (int j = 0; j < NP; j++)
             {
                 const R c = sp_val<R, Q>(q[j], b[j]);
-                acc[3].add(u[j] ? c : (R)0);
+                if constexpr (EX) part += u[j] ? c : (R)0;
+                else acc[3].add(u[j] ? c : (R)0);
             }
+            if constexpr (EX) acc[3].add(part);
         }
         else
         {
             FRec<R> q[NP];
+            R pxs = (R)0, pys = (R)0, pzs = (R)0, pes = (R)0;
 #pragma unroll
             for (int j = 0; j < NP; j++) q[j] = frec_at<EN>(rb, o[j], (R *)nullptr);
 #pragma unroll
@@ -612,14 +620,28 @@ static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T
                 const R rho = (c.r1 + bb * ((R)2 * c.r2 + (R)3 * bb * c.r3)) * inv;
                 const R ph = (c.p1 + bb * ((R)2 * c.p2 + (R)3 * bb * c.p3)) * inv;
                 const R mag = (R)2 * (Fpi * rho + fj[j] * rho + ph);
-                acc[0].add(u[j] ? -mag * dx[j] : (R)0);
-                acc[1].add(u[j] ? -mag * dy[j] : (R)0);
-                acc[2].add(u[j] ? -mag * dz[j] : (R)0);
-                if (EN)
+                const R e = EN ? (R)0.5 * (c.p0 + bb * (c.p1 + bb * (c.p2 + bb * c.p3))) : (R)0;
+                if constexpr (EX)
                 {
-                    const R e = (R)0.5 * (c.p0 + bb * (c.p1 + bb * (c.p2 + bb * c.p3)));
-                    acc[3].add(u[j] ? e : (R)0);
+                    pxs += u[j] ? -mag * dx[j] : (R)0;
+                    pys += u[j] ? -mag * dy[j] : (R)0;
+                    pzs += u[j] ? -mag * dz[j] : (R)0;
+                    if (EN) pes += u[j] ? e : (R)0;
                 }
+                else
+                {
+                    acc[0].add(u[j] ? -mag * dx[j] : (R)0);
+                    acc[1].add(u[j] ? -mag * dy[j] : (R)0);
+                    acc[2].add(u[j] ? -mag * dz[j] : (R)0);
+                    if (EN) acc[3].add(u[j] ? e : (R)0);
+                }
+            }
+            if constexpr (EX)
+            {
+                acc[0].add(pxs);
+                acc[1].add(pys);
+                acc[2].add(pzs);
+                if (EN) acc[3].add(pes);
             }
         }
     }
@@ -689,14 +711,25 @@ static __device__ __forceinline__ void flush_pair2(const KDesc &D, const TileT &
         R fac1, en1, fac2, en2;
         if (CAT == CAT_LJ) { lj_terms<R>(p, r21, fac1, en1); lj_terms<R>(p, r22, fac2, en2); }
         else { morse_terms<R>(p, r21, fac1, en1); morse_terms<R>(p, r22, fac2, en2); }
-        acc[0].add(u1 ? fac1 * dx1 : (R)0);
-        acc[1].add(u1 ? fac1 * dy1 : (R)0);
-        acc[2].add(u1 ? fac1 * dz1 : (R)0);
-        if (EN) acc[3].add(u1 ? en1 : (R)0);
-        acc[0].add(u2 ? fac2 * dx2 : (R)0);
-        acc[1].add(u2 ? fac2 * dy2 : (R)0);
-        acc[2].add(u2 ? fac2 * dz2 : (R)0);
-        if (EN) acc[3].add(u2 ? en2 : (R)0);
+        if constexpr (EX)
+        {
+            /* FP32: the two pair terms summed before one compensated add */
+            acc[0].add((u1 ? fac1 * dx1 : (R)0) + (u2 ? fac2 * dx2 : (R)0));
+            acc[1].add((u1 ? fac1 * dy1 : (R)0) + (u2 ? fac2 * dy2 : (R)0));
+            acc[2].add((u1 ? fac1 * dz1 : (R)0) + (u2 ? fac2 * dz2 : (R)0));
+            if (EN) acc[3].add((u1 ? en1 : (R)0) + (u2 ? en2 : (R)0));
+        }
+        else
+        {
+            acc[0].add(u1 ? fac1 * dx1 : (R)0);
+            acc[1].add(u1 ? fac1 * dy1 : (R)0);
+            acc[2].add(u1 ? fac1 * dz1 : (R)0);
+            if (EN) acc[3].add(u1 ? en1 : (R)0);
+            acc[0].add(u2 ? fac2 * dx2 : (R)0);
+            acc[1].add(u2 ? fac2 * dy2 : (R)0);
+            acc[2].add(u2 ? fac2 * dz2 : (R)0);
+            if (EN) acc[3].add(u2 ? en2 : (R)0);
+        }
     }
 }
 
