@@ -243,6 +243,11 @@ static __device__ __forceinline__ float fexp(float x) { return __builtin_amdgcn_
 static __device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
 static __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 static __device__ __forceinline__ double frcp(double x) { return 1.0 / x; }
+static __device__ __forceinline__ double ffract(double x) { return __builtin_amdgcn_fract(x); }
+
+/* tile position of scan trip g of stripe s: trips and stripes stay below 2^24, so the
+   full-rate 24-bit multiply is exact (a 32-bit multiply issues at a quarter of the rate) */
+static __device__ __forceinline__ int tpos(int s, int g, int S) { return s + (int)__umul24((unsigned)g, (unsigned)S); }
 static __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 /* LJ (lj.h:39-57) and Morse (morse.h:42-56) for one pair at r^2 inside the
@@ -533,7 +538,7 @@ static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T
             const int t = v ? __builtin_ctzll(word) : 0;
             word &= word - 1ull;
             R x, y, z;
-            T.load(s + (g0 + t) * S, x, y, z, fj[j]);
+            T.load(tpos(s, g0 + t, S), x, y, z, fj[j]);
             dx[j] = xi - x;
             dy[j] = yi - y;
             dz[j] = zi - z;
@@ -541,10 +546,17 @@ static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T
             u[j] = v && r2 < rx2;
             if (EX) xw |= (v && r2 >= rx2) ? 1ull << t : 0ull;
             r2 = u[j] ? r2 : (R)0;
+            /* interval k and the position b in it; r2 < rc^2 keeps k at most kmax + 1, whose
+               offset lies beyond the table and reads zeros (the table's appended zero knots) */
             const R tt = r2 * inv;
-            const int k = min((int)tt, kmax);
-            b[j] = tt - (R)k;
-            kk[j] = k;
+            const int k = (int)tt;
+            kk[j] = min(k, kmax);
+            /* FP64: fract(tt), one operation; FP32: r2 inv - k from the unrounded product, the more
+               accurate of the two where tt carries only 24 bits; the LDS table has no interval
+               beyond kmax */
+            if constexpr (LT) b[j] = tt - (R)kk[j];
+            else if constexpr (sizeof(R) == sizeof(double)) b[j] = ffract(tt);
+            else b[j] = __builtin_fmaf(r2, inv, -(float)k);
             o[j] = u[j] ? (unsigned)k * rec : RP_NONE;
         }
         if constexpr (LT)
@@ -698,8 +710,8 @@ static __device__ __forceinline__ void flush_pair2(const KDesc &D, const TileT &
         const int t2 = v2 ? __builtin_ctzll(word) : 0;
         word &= word - 1ull;
         R x1, y1, z1, f1, x2, y2, z2, f2;
-        T.load(s + (g0 + t1) * S, x1, y1, z1, f1);
-        T.load(s + (g0 + t2) * S, x2, y2, z2, f2);
+        T.load(tpos(s, g0 + t1, S), x1, y1, z1, f1);
+        T.load(tpos(s, g0 + t2, S), x2, y2, z2, f2);
         const R dx1 = xi - x1, dy1 = yi - y1, dz1 = zi - z1;
         const R dx2 = xi - x2, dy2 = yi - y2, dz2 = zi - z2;
         R r21 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
@@ -744,7 +756,7 @@ static __device__ unsigned long long band_bits(const TileT &T, unsigned long lon
         const int t = __builtin_ctzll(w);
         w &= w - 1ull;
         float x, y, z, f;
-        T.load(s + (g0 + t) * S, x, y, z, f);
+        T.load(tpos(s, g0 + t, S), x, y, z, f);
         const float dx = xi - x, dy = yi - y, dz = zi - z;
         if (dx * dx + dy * dy + dz * dz >= rx2) out |= 1ull << t;
     }
@@ -763,7 +775,7 @@ static __device__ __forceinline__ void flush_exact(const KDesc &D, const TileT &
         {
             const int t = __builtin_ctzll(xw);
             xw &= xw - 1ull;
-            const int m = s + (g0 + t) * S;
+            const int m = tpos(s, g0 + t, S);
             if constexpr (CAT == CAT_EAM1) exact_eam1<PASS, EN>(D, X, T.g[m], p1.rc2, acc);
             else if constexpr (CAT == CAT_LJ || CAT == CAT_MORSE) exact_pair1<CAT, EN>(D, X, T.g[m], p1, acc);
             else exact_pair<PASS, EN>(D, X, T.g[m], ppi[T.k[m]], acc);
@@ -1168,7 +1180,7 @@ static __device__ void b_cell(const KDesc &D, const int c)
                         {
                             const int t = __builtin_ctzll(word);
                             word &= word - 1ull;
-                            const int m = s + (g0 + t) * S;
+                            const int m = tpos(s, g0 + t, S);
                             R x, y, z, fj;
                             T.load(m, x, y, z, fj);
                             const PairParam &pq = CAT == CAT_GEN ? ppi[T.k[m]] : p1;
