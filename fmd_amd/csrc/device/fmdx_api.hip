@@ -278,15 +278,20 @@ extern "C" int fmdx_set_eam(fmdx_system_t *s, int nelem, int nrho, double drho, 
     cvt(hF, fF); cvt(hR, fR); cvt(hP, fP);
     /* element 0's force record per interval: the derivative coefficients of rho and phi
        first, the values last (a force pass without energy reads the first 48 / 24 bytes) */
+    /* the derivative coefficients are stored as d/dx = (c1 + 2 c2 b + 3 c3 b^2) / h, that is
+       (c1 / h, 2 c2 / h, 3 c3 / h), so that a derivative is two FMAs */
     std::vector<double4> hRF(2 * (size_t)m2);
     std::vector<float4> fRF(2 * (size_t)m2);
+    const double ih = 1.0 / dr2;
     for (long k = 0; k < m2; k++)
     {
         const double4 r = hR[k], p = hP[k];
-        hRF[2 * k] = make_double4(r.y, r.z, r.w, p.y);
-        hRF[2 * k + 1] = make_double4(p.z, p.w, r.x, p.x);
-        fRF[2 * k] = make_float4((float)r.y, (float)r.z, (float)r.w, (float)p.y);
-        fRF[2 * k + 1] = make_float4((float)p.z, (float)p.w, (float)r.x, (float)p.x);
+        const double r1 = r.y * ih, r2 = 2.0 * r.z * ih, r3 = 3.0 * r.w * ih;
+        const double p1 = p.y * ih, p2 = 2.0 * p.z * ih, p3 = 3.0 * p.w * ih;
+        hRF[2 * k] = make_double4(r1, r2, r3, p1);
+        hRF[2 * k + 1] = make_double4(p2, p3, r.x, p.x);
+        fRF[2 * k] = make_float4((float)r1, (float)r2, (float)r3, (float)p1);
+        fRF[2 * k + 1] = make_float4((float)p2, (float)p3, (float)r.x, (float)p.x);
     }
     /* element 0's derivative knots of rho and phi on the reference's r^2 grid: D_k = h y'(x_k)
        (the slope in b = x/h - k, from interval k, or the end of the last one) and Y_k = y''_k h^2/6,

@@ -462,12 +462,21 @@ static __device__ __forceinline__ float4 rho_at(__amdgpu_buffer_rsrc_t r, unsign
 #define RP_NONE 0x80000000u   /* an offset beyond any table */
 
 /* Force record of one interval of a single-element table: the derivative
-   coefficients of rho and phi first, the values last, so that a force pass
-   without energy reads 48 of the 64 bytes (FP64: three 16-byte loads instead
-   of four) and the FP32 pass 24 of 32 (a 16- and an 8-byte load):
+   coefficients of rho and phi first, scaled so that d/dx = r1 + b (r2 + b r3)
+   (r1 = c1 / h, r2 = 2 c2 / h, r3 = 3 c3 / h, b = x / h - k), the values c0
+   last, so that a force pass without energy reads 48 of the 64 bytes (FP64:
+   three 16-byte loads instead of four) and the FP32 pass 24 of 32 (a 16- and
+   an 8-byte load); the energy recovers c1..c3 from the scaled coefficients:
      FP64  (rho1 rho2)(rho3 phi1)(phi2 phi3)(rho0 phi0)
      FP32  (rho1 rho2 rho3 phi1)(phi2 phi3 rho0 phi0)            */
 template <typename R> struct FRec { R r1, r2, r3, p1, p2, p3, p0; };
+/* half the pair potential phi at position b of its interval, from the scaled coefficients:
+   phi = c0 + b (c1 + b (c2 + b c3)) with c1 = h p1, c2 = h p2 / 2, c3 = h p3 / 3 */
+template <typename R>
+static __device__ __forceinline__ R pair_value(const FRec<R> &c, R b, R h)
+{
+    return (R)0.5 * (c.p0 + b * h * (c.p1 + b * ((R)0.5 * c.p2 + b * ((R)(1.0 / 3.0) * c.p3))));
+}
 template <bool EN>
 static __device__ __forceinline__ FRec<double> frec_at(__amdgpu_buffer_rsrc_t r, unsigned off, double *)
 {
@@ -520,6 +529,7 @@ static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T
                                                    : (PASS == 0 ? FMDX_EAM_NP32_0 : FMDX_EAM_NP32_1);
     const EamDev &E = D.eam;
     const R inv = (R)E.inv_dr2;
+    const R hh = (R)E.dr2;
     const int kmax = E.nr2 - 2;
     /* bytes per interval: density record (Q), force record (2 Q) */
     const unsigned rec = PASS == 1 ? (unsigned)sizeof(Q) : 2u * (unsigned)sizeof(Q);
@@ -597,7 +607,7 @@ static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T
                     {
                         /* the pair energy from the force record's value coefficients in memory */
                         const FRec<R> c = frec_at<true>(rb, o[j], (R *)nullptr);
-                        const R e = (R)0.5 * (c.p0 + bb * (c.p1 + bb * (c.p2 + bb * c.p3)));
+                        const R e = pair_value(c, bb, hh);
                         acc[3].add(u[j] ? e : (R)0);
                     }
                 }
@@ -629,10 +639,10 @@ static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T
             {
                 const FRec<R> &c = q[j];
                 const R bb = b[j];
-                const R rho = (c.r1 + bb * ((R)2 * c.r2 + (R)3 * bb * c.r3)) * inv;
-                const R ph = (c.p1 + bb * ((R)2 * c.p2 + (R)3 * bb * c.p3)) * inv;
+                const R rho = c.r1 + bb * (c.r2 + bb * c.r3);
+                const R ph = c.p1 + bb * (c.p2 + bb * c.p3);
                 const R mag = (R)2 * (Fpi * rho + fj[j] * rho + ph);
-                const R e = EN ? (R)0.5 * (c.p0 + bb * (c.p1 + bb * (c.p2 + bb * c.p3))) : (R)0;
+                const R e = EN ? pair_value(c, bb, hh) : (R)0;
                 if constexpr (EX)
                 {
                     pxs += u[j] ? -mag * dx[j] : (R)0;
