@@ -605,9 +605,11 @@ static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T
                     acc[2].add(u[j] ? -mag * dz[j] : (R)0);
                     if (EN)
                     {
-                        /* the pair energy from the force record's value coefficients in memory */
-                        const FRec<R> c = frec_at<true>(rb, o[j], (R *)nullptr);
-                        const R e = pair_value(c, bb, hh);
+                        /* the pair energy from phi's value coefficients (c0..c3) in memory: one
+                           16-byte load, half the force record's offset */
+                        const __amdgpu_buffer_rsrc_t rv = rp_rsrc(Prec<R>::phi(E), (unsigned)(E.nr2 - 1) * 16u);
+                        const float4 qv = rho_at(rv, u[j] ? o[j] / 2u : RP_NONE, (float4 *)nullptr);
+                        const R e = (R)0.5 * sp_val<R, float4>(qv, bb);
                         acc[3].add(u[j] ? e : (R)0);
                     }
                 }
