@@ -861,7 +861,7 @@ static __device__ void b_cell(const KDesc &D, const int c)
         return;
     }
     __shared__ long nb_base_s[NWS][27];
-    __shared__ int nb_pre_s[NWS][28];
+    __shared__ __align__(16) int nb_pre_s[NWS][32];   /* 28 prefixes, then INT_MAX */
     __shared__ Shifts<R> SH_s[NWS];
     __shared__ Tile<R, WITH_FP, CAT == CAT_GEN, CT> T_s[NWS];
     long *const nb_base = nb_base_s[ws];
@@ -910,17 +910,38 @@ static __device__ void b_cell(const KDesc &D, const int c)
         const int wr[3] = {wx, wy, wz}, dj[3] = {jx, jy, jz}, hc[3] = {cx, cy, cz};
         SH.set(D, lane, wr, dj, hc);
     }
-    /* inclusive prefix of the neighbour counts over lanes 0..26 */
+    /* inclusive prefix of the neighbour counts over lanes 0..26 (lanes 27..31 hold 0):
+       within each row of 16 lanes by DPP row shifts (a lane whose source lies outside
+       its row adds 0), then lanes 16..31 add lane 15's total */
     int incl = my_cnt;
-    for (int o = 1; o < 32; o <<= 1)
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xf, 0xf, false);   /* row_shr:1 */
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x112, 0xf, 0xf, false);   /* row_shr:2 */
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x114, 0xf, 0xf, false);   /* row_shr:4 */
+    incl += __builtin_amdgcn_update_dpp(0, incl, 0x118, 0xf, 0xf, false);   /* row_shr:8 */
     {
-        const int v = __shfl_up(incl, o);
-        if (lane >= o) incl += v;
+        const int row0 = __builtin_amdgcn_readlane(incl, 15);
+        if ((lane & (WG - 1)) >= 16 && (lane & (WG - 1)) < 32) incl += row0;
     }
     if (lane < 27) nb_pre[lane + 1] = incl;
+    else if (lane < 31) nb_pre[lane + 1] = 0x7fffffff;
     if (lane == 0) nb_pre[0] = 0;
     csync<LT>();
     const int ncand = __builtin_amdgcn_readfirstlane(nb_pre[27]);
+    /* Neighbour cell of candidate g of the 27-cell list: the last prefix not above g
+       (empty cells share their successor's prefix and are passed over).  Two levels:
+       the block of eight prefixes from three wave-uniform pivots, then one 32-byte
+       read of that block: one LDS round trip per staging batch where a binary search
+       over the 27 prefixes takes five dependent ones (4M-atom copper step 3.50 -> 3.43
+       ms FP64, 2.91 -> 2.80 ms mixed). */
+    const int pv8 = __builtin_amdgcn_readfirstlane(nb_pre[8]);
+    const int pv16 = __builtin_amdgcn_readfirstlane(nb_pre[16]);
+    const int pv24 = __builtin_amdgcn_readfirstlane(nb_pre[24]);
+    auto locate = [&](int g) -> int {
+        const int blk = 8 * ((g >= pv8) + (g >= pv16) + (g >= pv24));
+        const int4 u = *reinterpret_cast<const int4 *>(nb_pre + blk);
+        const int4 v = *reinterpret_cast<const int4 *>(nb_pre + blk + 4);
+        return blk + (u.y <= g) + (u.z <= g) + (u.w <= g) + (v.x <= g) + (v.y <= g) + (v.z <= g) + (v.w <= g);
+    };
 
     const int nh = __builtin_amdgcn_readfirstlane(D.cnt[c]);
     const int ag = D.ag;
@@ -1018,39 +1039,50 @@ static __device__ void b_cell(const KDesc &D, const int c)
                 if (lane < nhc) selfpos[lane] = -1;
             }
             int batch = 0;
+            /* one batch of candidates as loaded: position, F' (and kind, which only the
+               generic kernel dispatches on) load together, whether or not the candidate
+               survives the culling: a load issued after the test would add one more round
+               trip to every batch (force pass 1.88 -> 1.80 ms FP64, 1.60 -> 1.52 ms mixed,
+               on the 4M-atom copper step) */
+            struct Cand
+            {
+                R x, y, z, f;
+                int kj, lo, gr;
+            };
+            auto fetch = [&](int g, Cand &q) {
+                q.x = q.y = q.z = q.f = (R)0;
+                q.kj = 0; q.lo = 0; q.gr = ag;
+                if (g < ncand)
+                {
+                    q.lo = locate(g);
+                    const long js = nb_base[q.lo] + (g - nb_pre[q.lo]);
+                    P::template load<WITH_FP>(D, js, q.x, q.y, q.z, q.f);
+                    if (CAT == CAT_GEN) q.kj = D.kind[js];
+                    if (ag >= 0) q.gr = D.grp[js];
+                }
+            };
             while (gpos < ncand && nt <= (int)CT - NT)
             {
                 const int g = gpos + lane;
+                Cand cur;
+                fetch(g, cur);
                 bool keep = false;
-                R x = 0, y = 0, z = 0, fpv = 0;
-                int kj = 0, lo = 0;
+                R x = 0, y = 0, z = 0;
+                const R fpv = cur.f;
+                const int kj = cur.kj, lo = cur.lo;
                 if (g < ncand)
                 {
                     /* staged position and culling test without FMA contraction, so that every
                        instantiation keeps the same candidates: the force pass replays the
                        density pass's hit words by tile position */
 #pragma clang fp contract(off)
-                    int hi = 27;
-                    while (hi - lo > 1)
-                    {
-                        const int mid = (lo + hi) >> 1;
-                        if (nb_pre[mid] <= g) lo = mid;
-                        else hi = mid;
-                    }
-                    const long js = nb_base[lo] + (g - nb_pre[lo]);
-                    P::template load<WITH_FP>(D, js, x, y, z, fpv);
-                    x = x + SH.sx(D, lo);
-                    y = y + SH.sy(D, lo);
-                    z = z + SH.sz(D, lo);
+                    x = cur.x + SH.sx(D, lo);
+                    y = cur.y + SH.sy(D, lo);
+                    z = cur.z + SH.sz(D, lo);
                     const R ex = fmax(fmax(bx0 - x, x - bx1), (R)0);
                     const R ey = fmax(fmax(by0 - y, y - by1), (R)0);
                     const R ez = fmax(fmax(bz0 - z, z - bz1), (R)0);
-                    /* the candidate's F' (and kind, which only the generic kernel dispatches on) load
-                       with its position, whether or not it survives the culling: a load issued after
-                       the test would add one more round trip to every batch (force pass 1.88 -> 1.80 ms
-                       FP64, 1.60 -> 1.52 ms mixed, on the 4M-atom copper step) */
-                    if (CAT == CAT_GEN) kj = D.kind[js];
-                    keep = ex * ex + ey * ey + ez * ez <= cull2 && (ag < 0 || D.grp[js] == ag);
+                    keep = ex * ex + ey * ey + ez * ez <= cull2 && (ag < 0 || cur.gr == ag);
                 }
                 const unsigned long long bal = __ballot(keep);
                 int off = nt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
