@@ -63,4 +63,7 @@
 #ifndef FMDX_EAM_NP32_1
 #define FMDX_EAM_NP32_1 4
 #endif
+#ifndef FMDX_SPLIT_MAX
+#define FMDX_SPLIT_MAX 64
+#endif
 #endif

@@ -965,7 +965,13 @@ static __device__ void b_cell(const KDesc &D, const int c)
     const R pinf = (R)__builtin_inf();
     const R qnan = (R)__builtin_nan("");
     const long hbase = (long)c * D.cap;
-    const bool one_block = nh > 0 && nh <= NT;
+    /* Home atoms per block.  One block holds up to one atom per lane, each atom with
+       64 / nh stripes, so that a cell of 38 atoms leaves 26 lanes idle through the scan
+       and the flush.  Cells of more than 32 atoms (up to FMDX_SPLIT_MAX) are taken in
+       blocks of 32 atoms, two stripes each, and the rest, the tile staged for each
+       block (38 atoms: nt / 2 + nt / 10 scan trips instead of nt). */
+    const int bsz = (FMDX_WAVES_PER_CELL == 1 && nh > WG / 2 && nh <= FMDX_SPLIT_MAX) ? WG / 2 : NT;
+    const bool one_block = nh > 0 && nh <= bsz;
     const int hw = __builtin_amdgcn_readfirstlane(hw_ld);
     /* a density pass without hit-word storage (hbits null) records nothing: the force pass rescans */
     const bool record = PASS == 1 && one_block && D.hbits != nullptr;
@@ -993,9 +999,9 @@ static __device__ void b_cell(const KDesc &D, const int c)
     const R cull2 = rcut2 * (sizeof(R) == sizeof(double) ? (R)(1.0 + 1e-12) : (R)(1.0 + 1e-5));
     double cell_acc = 0.0;
 
-    for (int h0 = 0; h0 < nh; h0 += NT)
+    for (int h0 = 0; h0 < nh; h0 += bsz)
     {
-        const int nhc = min(NT, nh - h0);
+        const int nhc = min(bsz, nh - h0);
         const int S = NT / nhc;
         const int a = lane % nhc, s = lane / nhc;
         const long is = hbase + h0 + a;

@@ -178,3 +178,37 @@ def test_degenerate_boxes_1_and_2_cells(rng, precision):
     assert list(o.nc) == [1, 2, 3]
     tol = (1e-10, 1e-10) if precision == F.FMDX_PREC_FP64 else (1e-5, 1e-6)
     _compare(o, d, *tol)
+
+
+@pytest.mark.parametrize("precision", [F.FMDX_PREC_FP64, F.FMDX_PREC_MIXED])
+@pytest.mark.parametrize("ucells,a,lo,hi", [(7, 3.615, 33, 64), (6, 3.45, 65, 256)])
+def test_large_cells_split_into_home_blocks(rng, precision, ucells, a, lo, hi):
+    # Morse copper (rc 7 A) on 7^3 FCC unit cells (3 cells per edge of 8.4 A, 35 to 62
+    # atoms per cell) and on 6^3 compressed ones (2 cells per edge of 10.4 A, 108 atoms
+    # per cell): the kernels take cells of 33 to 64 atoms in a block of 32 and a block
+    # of the rest, and larger cells in blocks of 64
+    x = lattice_fcc((ucells,) * 3, a, rng, jitter=0.2)
+    L = [ucells * a] * 3
+    o, d = build(L, [1, 1, 1], [63.546], ["Cu"], [("morse", 0, 0, 0.3429, 1.3588, 2.866, 7.0)], x,
+                 precision=precision)
+    counts = o.atoms()["cell_count"]
+    assert counts.min() >= lo and counts.max() <= hi, (counts.min(), counts.max())
+    tol = (1e-10, 1e-10) if precision == F.FMDX_PREC_FP64 else (1e-5, 1e-6)
+    _compare(o, d, *tol)
+
+
+@pytest.mark.parametrize("precision", [F.FMDX_PREC_FP64, F.FMDX_PREC_MIXED])
+def test_eam_cells_split_into_home_blocks(rng, cu01, precision):
+    # Cu01 EAM on 4^3 FCC unit cells: 2 cells per edge of 7.2 A, 32 atoms per cell before
+    # the jitter moves some across (the lattice planes are shifted onto the cell faces):
+    # cells of more than 32 atoms are taken in two blocks,
+    # whose density pass records no hit words, so that the force pass scans them again
+    a = 3.615
+    L = [4 * a] * 3
+    x = np.mod(lattice_fcc((4, 4, 4), a, rng, jitter=0.25, origin=(-0.2 * a,) * 3), L[0])
+    o, d = build(L, [1, 1, 1], [63.546], ["Cu"], [("eam", cu01, [(0, 0)])], x, precision=precision)
+    counts = o.atoms()["cell_count"]
+    assert counts.max() > 32 and counts.min() <= 32, (counts.min(), counts.max())
+    tol = (1e-10, 1e-10) if precision == F.FMDX_PREC_FP64 else (1e-5, 1e-6)
+    got, ref = _compare(o, d, *tol)
+    assert rel_err(got["aux"], ref["aux"]) < tol[0]
