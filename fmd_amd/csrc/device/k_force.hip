@@ -199,9 +199,10 @@ struct AccD
     __device__ __forceinline__ void add(double x) { v += x; }
     __device__ __forceinline__ double get() const { return v; }
 };
-/* FP32 compensated (Kahan) sum.  The flush loops add the two to four pair
+/* FP32 compensated (Kahan) sum.  The EAM flush loops add the two to four pair
    terms of one iteration in plain FP32 first and compensate once per
-   iteration: one rounding of a few terms' partial sum (relative 6e-8 of it)
+   iteration, the LJ and Morse flushes once per hit word (up to 64 trips):
+   one rounding per term of a short partial sum (relative 6e-8 of it)
    instead of three extra operations per term. */
 struct AccF
 {
@@ -713,6 +714,10 @@ static __device__ __forceinline__ void flush_pair2(const KDesc &D, const TileT &
     constexpr bool EX = sizeof(R) == sizeof(float);   /* FP32: a pair in the rounding band below rc^2 sets ex */
     const R rc2 = (R)p.rc2;
     const R rx2 = EX ? (R)(p.rc2 * (1.0 - RX_BAND)) : rc2;
+    /* FP32: the pair terms of the whole word are summed in plain FP32 and compensated
+       once, after the word (batched LJ/Morse ensemble 2.73 -> 2.63 ms per step against one
+       compensated add per iteration, same max mixed/FP64 force error 7.1e-6) */
+    R wpx = (R)0, wpy = (R)0, wpz = (R)0, wpe = (R)0;
     while (__ballot(word != 0ull))
     {
         const bool v1 = word != 0ull;
@@ -737,11 +742,10 @@ static __device__ __forceinline__ void flush_pair2(const KDesc &D, const TileT &
         else { morse_terms<R>(p, r21, fac1, en1); morse_terms<R>(p, r22, fac2, en2); }
         if constexpr (EX)
         {
-            /* FP32: the two pair terms summed before one compensated add */
-            acc[0].add((u1 ? fac1 * dx1 : (R)0) + (u2 ? fac2 * dx2 : (R)0));
-            acc[1].add((u1 ? fac1 * dy1 : (R)0) + (u2 ? fac2 * dy2 : (R)0));
-            acc[2].add((u1 ? fac1 * dz1 : (R)0) + (u2 ? fac2 * dz2 : (R)0));
-            if (EN) acc[3].add((u1 ? en1 : (R)0) + (u2 ? en2 : (R)0));
+            wpx += (u1 ? fac1 * dx1 : (R)0) + (u2 ? fac2 * dx2 : (R)0);
+            wpy += (u1 ? fac1 * dy1 : (R)0) + (u2 ? fac2 * dy2 : (R)0);
+            wpz += (u1 ? fac1 * dz1 : (R)0) + (u2 ? fac2 * dz2 : (R)0);
+            if (EN) wpe += (u1 ? en1 : (R)0) + (u2 ? en2 : (R)0);
         }
         else
         {
@@ -754,6 +758,13 @@ static __device__ __forceinline__ void flush_pair2(const KDesc &D, const TileT &
             acc[2].add(u2 ? fac2 * dz2 : (R)0);
             if (EN) acc[3].add(u2 ? en2 : (R)0);
         }
+    }
+    if constexpr (EX)
+    {
+        acc[0].add(wpx);
+        acc[1].add(wpy);
+        acc[2].add(wpz);
+        if (EN) acc[3].add(wpe);
     }
 }
 
