@@ -742,9 +742,12 @@ static __device__ __forceinline__ void flush_pair2(const KDesc &D, const TileT &
         else { morse_terms<R>(p, r21, fac1, en1); morse_terms<R>(p, r22, fac2, en2); }
         if constexpr (EX)
         {
-            wpx += (u1 ? fac1 * dx1 : (R)0) + (u2 ? fac2 * dx2 : (R)0);
-            wpy += (u1 ? fac1 * dy1 : (R)0) + (u2 ? fac2 * dy2 : (R)0);
-            wpz += (u1 ? fac1 * dz1 : (R)0) + (u2 ? fac2 * dz2 : (R)0);
+            /* an empty slot's factor is zeroed and multiplies a finite separation (tile
+               entries, padding included, and home coordinates are finite) */
+            const R f1 = u1 ? fac1 : (R)0, f2 = u2 ? fac2 : (R)0;
+            wpx = __builtin_fmaf(f2, dx2, __builtin_fmaf(f1, dx1, wpx));
+            wpy = __builtin_fmaf(f2, dy2, __builtin_fmaf(f1, dy1, wpy));
+            wpz = __builtin_fmaf(f2, dz2, __builtin_fmaf(f1, dz1, wpz));
             if (EN) wpe += (u1 ? en1 : (R)0) + (u2 ? en2 : (R)0);
         }
         else
@@ -964,17 +967,17 @@ static __device__ void b_cell(const KDesc &D, const int c)
        selects candidates: every selected pair is tested again with the differences
        (flush_eam1, pair_hit), so the margin only has to exceed the cancellation
        error; cell-relative coordinates stay below 4 sqrt(3) rc (cells are narrower
-       than 2 rc), which bounds that error by ~2e-5 rc^2.  No NaN may reach d, whose
-       sign bit would be arbitrary: a lane without a home atom scans with zero
-       multipliers and |x_i|^2 = +inf, and padding entries are (0, 0, 0, +inf), so
-       both give d = +inf.  The other tiles keep the difference form, with the
-       margin folded into the first FMA, d = dz^2 + (dy^2 + (dx^2 - rc^2 (1 + m))),
-       and the same sign shift (m = 1e-12 in FP64); there a lane without a home atom
-       scans from x_i = +inf and padding sits at -inf, so that d = +inf again. */
+       than 2 rc), which bounds that error by ~2e-5 rc^2.  No NaN may reach the d of
+       a live home atom, whose sign bit would be arbitrary: padding entries are
+       (0, 0, 0, +inf), which gives d = +inf.  The other tiles keep the difference
+       form, with the margin folded into the first FMA,
+       d = dz^2 + (dy^2 + (dx^2 - rc^2 (1 + m))), and the same sign shift (m = 1e-12
+       in FP64); there padding sits at -inf, so that d = +inf again.  A lane without
+       a live home atom scans with zero multipliers (norm form) or from +inf
+       (difference form), and its word is cleared whatever it read. */
     constexpr bool NSCAN = sizeof(R) == sizeof(float) && !WITH_FP;
     const R rscan2 = rcut2 * (R)(sizeof(R) == sizeof(double) ? 1.0 + 1e-12 : 1.0 + 2.5e-4);
     const R pinf = (R)__builtin_inf();
-    const R qnan = (R)__builtin_nan("");
     const long hbase = (long)c * D.cap;
     /* Home atoms per block.  One block holds up to one atom per lane, each atom with
        64 / nh stripes, so that a cell of 38 atoms leaves 26 lanes idle through the scan
@@ -1017,8 +1020,10 @@ static __device__ void b_cell(const KDesc &D, const int c)
         const int a = lane % nhc, s = lane / nhc;
         const long is = hbase + h0 + a;
         const ExactCtx X = {nb_base, nb_pre, SH.w, is};
-        /* a lane without an active home atom scans with NaN coordinates: no hits */
-        R xi = qnan, yi = qnan, zi = qnan, Fpi = 0;
+        /* a lane without an active home atom (live false) scans so that it finds no
+           candidate, from finite coordinates, so that the flushes may multiply its
+           empty slots' zero factors by finite separations */
+        R xi = 0, yi = 0, zi = 0, Fpi = 0;
         int ki = 0;
         bool act = false;
         if (s < S)
@@ -1032,6 +1037,7 @@ static __device__ void b_cell(const KDesc &D, const int c)
             }
             if (PASS == 0 && D.has_eam) Fpi = (R)D.aux[is];
         }
+        const bool live = act && s < S;
         const PairParam *ppi = pp + ki * D.nkinds;
         Acc acc[4];
         for (int q = 0; q < 4; q++) acc[q].zero();
@@ -1175,7 +1181,6 @@ static __device__ void b_cell(const KDesc &D, const int c)
                     int m = s + (g0 + ng - 1) * S;
                     if constexpr (NSCAN)
                     {
-                        const bool live = xi == xi;   /* a home atom on this lane */
                         const float mx = live ? -2.0f * xi : 0.0f, my = live ? -2.0f * yi : 0.0f;
                         const float mz = live ? -2.0f * zi : 0.0f;
                         const float nim = live ? (xi * xi + yi * yi + zi * zi) - rscan2 : __builtin_inff();
@@ -1197,7 +1202,6 @@ static __device__ void b_cell(const KDesc &D, const int c)
                     }
                     else
                     {
-                        const bool live = xi == xi;   /* a home atom on this lane */
                         const R sx = live ? xi : pinf, sy = live ? yi : pinf, sz = live ? zi : pinf;
                         const auto *q = T.at(m);
 #pragma unroll 4
@@ -1215,7 +1219,12 @@ static __device__ void b_cell(const KDesc &D, const int c)
                             lo = __builtin_amdgcn_alignbit(lo, sign_word(d), 31);
                         }
                     }
-                    word = ((unsigned long long)hi << 32) | lo;
+                    /* A lane without a live home atom keeps no hits: the lanes beyond the
+                       stripes (s >= S) scan entries past the padded tile, whose stale LDS
+                       contents can read as hits, and nothing of theirs may reach the
+                       flushes, whose exact FP64 pairs turn a tile entry's list place into a
+                       global slot index. */
+                    word = live ? ((unsigned long long)hi << 32) | lo : 0ull;
                     /* the home atom meets itself on one trip of one stripe: clear that bit */
                     const int ds = selft - s - g0 * S;
                     if (ds >= 0 && ds < ng * S && ds % S == 0) word &= ~(1ull << (ds / S));

@@ -82,5 +82,5 @@ def test_lds_table_variant_fits_one_workgroup_per_cu(tmp_path):
         assert get("max_flat_workgroup_size") == 16 * 64, name
         assert get("group_segment_fixed_size") + 48 + 5002 * 16 <= 160 * 1024, name
         assert get("vgpr_count") <= 128, name
-        assert get("private_segment_fixed_size") <= 32, name
+        assert get("private_segment_fixed_size") <= 64, name
     assert found >= 3   # density pass, force pass with and without the pair energy
