@@ -125,6 +125,17 @@ template <bool FP, bool KD, int CT> struct Tile<double, FP, KD, CT>
         if constexpr (FP) f = p[m].w;
         else f = 0.0;
     }
+    /* entry t of a lane's word: the lane's first entry m0, stripes S entries apart, as a
+       byte offset t * S * size (one 24-bit multiply-add per entry) */
+    __device__ __forceinline__ unsigned stride(int S) const { return (unsigned)S * (unsigned)sizeof(p[0]); }
+    __device__ __forceinline__ void load(int m0, unsigned st, int t, double &x, double &y, double &zz, double &f) const
+    {
+        const auto &e = *reinterpret_cast<const typename RecOf<FP>::T *>(
+            reinterpret_cast<const char *>(p + m0) + __umul24((unsigned)t, st));
+        x = e.x; y = e.y; zz = e.z;
+        if constexpr (FP) f = e.w;
+        else f = 0.0;
+    }
 };
 template <bool FP, bool KD, int CT> struct Tile<float, FP, KD, CT>
 {
@@ -148,6 +159,12 @@ template <bool FP, bool KD, int CT> struct Tile<float, FP, KD, CT>
         x = a.x; y = a.y; zz = a.z; f = a.w;
     }
     __device__ __forceinline__ const float4 *at(int m) const { return p + m; }
+    __device__ __forceinline__ unsigned stride(int S) const { return (unsigned)S * (unsigned)sizeof(float4); }
+    __device__ __forceinline__ void load(int m0, unsigned st, int t, float &x, float &y, float &zz, float &f) const
+    {
+        const float4 a = *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(p + m0) + __umul24((unsigned)t, st));
+        x = a.x; y = a.y; zz = a.z; f = a.w;
+    }
 };
 
 /* Image shift of each of the 27 neighbour cells, added to the staged
@@ -536,6 +553,8 @@ static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T
     const unsigned rec = PASS == 1 ? (unsigned)sizeof(Q) : 2u * (unsigned)sizeof(Q);
     const __amdgpu_buffer_rsrc_t rb = rp_rsrc(PASS == 1 ? (const void *)Prec<R>::rho(E) : Prec<R>::rf(E),
                                               (unsigned)(E.nr2 - 1) * rec);
+    const int m0 = tpos(s, g0, S);   /* the lane's entry of the word's first trip */
+    const unsigned st = T.stride(S);
     while (__ballot(word != 0ull))
     {
         R dx[NP], dy[NP], dz[NP], fj[NP], b[NP];
@@ -549,7 +568,7 @@ static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T
             const int t = v ? __builtin_ctzll(word) : 0;
             word &= word - 1ull;
             R x, y, z;
-            T.load(tpos(s, g0 + t, S), x, y, z, fj[j]);
+            T.load(m0, st, t, x, y, z, fj[j]);
             dx[j] = xi - x;
             dy[j] = yi - y;
             dz[j] = zi - z;
@@ -718,7 +737,9 @@ static __device__ __forceinline__ void flush_pair2(const KDesc &D, const TileT &
        once, after the word (batched LJ/Morse ensemble 2.73 -> 2.63 ms per step against one
        compensated add per iteration, same max mixed/FP64 force error 7.1e-6) */
     R wpx = (R)0, wpy = (R)0, wpz = (R)0, wpe = (R)0;
-    while (__ballot(word != 0ull))
+    const int m0 = tpos(s, g0, S);   /* the lane's entry of the word's first trip */
+    const unsigned st = T.stride(S);
+    if (__ballot(word != 0ull)) do
     {
         const bool v1 = word != 0ull;
         const int t1 = v1 ? __builtin_ctzll(word) : 0;
@@ -727,14 +748,16 @@ static __device__ __forceinline__ void flush_pair2(const KDesc &D, const TileT &
         const int t2 = v2 ? __builtin_ctzll(word) : 0;
         word &= word - 1ull;
         R x1, y1, z1, f1, x2, y2, z2, f2;
-        T.load(tpos(s, g0 + t1, S), x1, y1, z1, f1);
-        T.load(tpos(s, g0 + t2, S), x2, y2, z2, f2);
+        T.load(m0, st, t1, x1, y1, z1, f1);
+        T.load(m0, st, t2, x2, y2, z2, f2);
         const R dx1 = xi - x1, dy1 = yi - y1, dz1 = zi - z1;
         const R dx2 = xi - x2, dy2 = yi - y2, dz2 = zi - z2;
         R r21 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
         R r22 = dx2 * dx2 + dy2 * dy2 + dz2 * dz2;
-        const bool u1 = v1 && r21 < rx2, u2 = v2 && r22 < rx2;
-        if (EX) ex = ex || (v1 && r21 >= rx2) || (v2 && r22 >= rx2);
+        const bool in1 = r21 < rx2, in2 = r22 < rx2;
+        const bool u1 = v1 && in1, u2 = v2 && in2;
+        /* the separations of a live lane's slots are finite: not below rx2 is at least rx2 */
+        if (EX) ex = ex || (v1 && !in1) || (v2 && !in2);
         r21 = u1 ? r21 : rc2;
         r22 = u2 ? r22 : rc2;
         R fac1, en1, fac2, en2;
@@ -761,7 +784,7 @@ static __device__ __forceinline__ void flush_pair2(const KDesc &D, const TileT &
             acc[2].add(u2 ? fac2 * dz2 : (R)0);
             if (EN) acc[3].add(u2 ? en2 : (R)0);
         }
-    }
+    } while (__ballot(word != 0ull));
     if constexpr (EX)
     {
         acc[0].add(wpx);
