@@ -6,8 +6,8 @@
  *     candidate tile with coalesced SoA loads.  Periodic images are shifted
  *     into the home cell's frame on the way in, as the reference's ghost
  *     copies are (md-ghost.c:85-107).  Candidates outside the active group
- *     and the padding of the last trip get NaN coordinates, so that they
- *     never match.  A list longer than one tile is processed tile by tile.
+ *     are not staged, and the padding of the last trip lies where it never
+ *     matches.  A list longer than one tile is processed tile by tile.
  *  2. Scan.  Lanes are split into (home atom a, stripe s): with n home atoms
  *     each atom gets S = 64/n lanes and lane (a,s) tests candidates s, s+S,
  *     s+2S, ... of the tile.  The trip count is wave-uniform, and every 64
@@ -645,7 +645,8 @@ static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T
             for (int j = 0; j < NP; j++)
             {
                 const R c = sp_val<R, Q>(q[j], b[j]);
-                if constexpr (EX) part += u[j] ? c : (R)0;
+                /* FP32: an empty slot read a zero record at b = 0, so that c = 0 */
+                if constexpr (EX) part += c;
                 else acc[3].add(u[j] ? c : (R)0);
             }
             if constexpr (EX) acc[3].add(part);
@@ -667,10 +668,12 @@ static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T
                 const R e = EN ? pair_value(c, bb, hh) : (R)0;
                 if constexpr (EX)
                 {
-                    pxs += u[j] ? -mag * dx[j] : (R)0;
-                    pys += u[j] ? -mag * dy[j] : (R)0;
-                    pzs += u[j] ? -mag * dz[j] : (R)0;
-                    if (EN) pes += u[j] ? e : (R)0;
+                    /* an empty slot read a zero record: mag = 0 and e = 0, times a finite
+                       separation (tile entries and padding are finite) */
+                    pxs = __builtin_fmaf(-mag, dx[j], pxs);
+                    pys = __builtin_fmaf(-mag, dy[j], pys);
+                    pzs = __builtin_fmaf(-mag, dz[j], pzs);
+                    if (EN) pes += e;
                 }
                 else
                 {
@@ -995,12 +998,16 @@ static __device__ void b_cell(const KDesc &D, const int c)
        (0, 0, 0, +inf), which gives d = +inf.  The other tiles keep the difference
        form, with the margin folded into the first FMA,
        d = dz^2 + (dy^2 + (dx^2 - rc^2 (1 + m))), and the same sign shift (m = 1e-12
-       in FP64); there padding sits at -inf, so that d = +inf again.  A lane without
+       in FP64); there padding sits at (-1e30, -1e30, -1e30) (FP64: -1e300), whose
+       squared distance overflows, so that d = +inf again.  A lane without
        a live home atom scans with zero multipliers (norm form) or from +inf
        (difference form), and its word is cleared whatever it read. */
     constexpr bool NSCAN = sizeof(R) == sizeof(float) && !WITH_FP;
     const R rscan2 = rcut2 * (R)(sizeof(R) == sizeof(double) ? 1.0 + 1e-12 : 1.0 + 2.5e-4);
     const R pinf = (R)__builtin_inf();
+    /* padding of the difference-form tiles: finite, but so far that its squared
+       distance from any home atom overflows to +inf */
+    const R pfar = sizeof(R) == sizeof(float) ? (R)1e30f : (R)1e300;
     const long hbase = (long)c * D.cap;
     /* Home atoms per block.  One block holds up to one atom per lane, each atom with
        64 / nh stripes, so that a cell of 38 atoms leaves 26 lanes idle through the scan
@@ -1185,7 +1192,7 @@ static __device__ void b_cell(const KDesc &D, const int c)
             for (int m = nt + lane; m < npad; m += NT)
             {
                 if constexpr (NSCAN) T.put(m, (R)0, (R)0, (R)0, pinf, NOKIND);
-                else T.put(m, -pinf, -pinf, -pinf, (R)0, NOKIND);
+                else T.put(m, -pfar, -pfar, -pfar, (R)0, NOKIND);
             }
             csync<LT>();
             if constexpr (FMDX_WAVES_PER_CELL > 1) selft = selfpos[a];
