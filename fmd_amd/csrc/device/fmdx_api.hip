@@ -87,6 +87,8 @@ extern "C" int fmdx_system_create(fmdx_system_t **out, const double l[3], const 
     CK(hipGetDevice(&s->dev));
     if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) { free(s); return FMDX_EHIP; }
     s->precision = precision;
+    s->D.xq_keep = precision == FMDX_PREC_MIXED;
+    s->xq_fresh = 0;
     KDesc &D = s->D;
     D.nc0 = nc[0]; D.nc1 = nc[1]; D.nc2 = nc[2];
     D.ncell = (int)ncell;
@@ -163,6 +165,8 @@ extern "C" int fmdx_set_precision(fmdx_system_t *s, int precision)
     if (!s || (precision != FMDX_PREC_FP64 && precision != FMDX_PREC_MIXED)) return FMDX_EARG;
     if (precision == FMDX_PREC_MIXED && s->D.has_eam && !s->rhoc32) return FMDX_ESTATE;
     s->precision = precision;
+    s->D.xq_keep = precision == FMDX_PREC_MIXED;
+    s->xq_fresh = 0;
     return FMDX_OK;
 }
 
@@ -375,6 +379,7 @@ extern "C" int fmdx_upload(fmdx_system_t *s, unsigned natoms, const unsigned *ce
 {
     if (!s || !cell_count) return FMDX_EARG;
     if (s->member) return FMDX_ESTATE;
+    s->xq_fresh = 0;
     KDesc &D = s->D;
     std::vector<int> start(D.ncell + 1);
     unsigned maxc = 0;
@@ -540,11 +545,15 @@ extern "C" int fmdx_update_forces_ex(fmdx_system_t *s, int active_group, int ttm
     s->D.ag = active_group;
     s->D.energy = (flags & FMDX_FORCES_ENERGY) ? 1 : 0;
     s->D.ttm = ttm_dev(s, ttm_friction);
+    /* the FP32 cell-relative positions are formed here unless the integrator step just
+       before left them in place */
+    const bool relpos = !s->xq_fresh;
+    s->xq_fresh = 0;
     if (s->profile && s->prof_n < NPROF)
     {
         hipEvent_t *e = &s->pev[3 * s->prof_n];
         CK(hipEventRecord(e[0], s->st));
-        CK(launch_pass1(s->D, s->st, s->precision));
+        CK(launch_pass1(s->D, s->st, s->precision, relpos));
         CK(hipEventRecord(e[1], s->st));
         CK(launch_pass0(s->D, s->st, s->precision));
         CK(hipEventRecord(e[2], s->st));
@@ -552,7 +561,7 @@ extern "C" int fmdx_update_forces_ex(fmdx_system_t *s, int active_group, int ttm
         s->prof_n++;
     }
     else
-        CK(launch_forces(s->D, s->st, s->precision));
+        CK(launch_forces(s->D, s->st, s->precision, relpos));
     return FMDX_OK;
 }
 
@@ -560,7 +569,9 @@ extern "C" int fmdx_vv_start(fmdx_system_t *s, int active_group, double dt, int 
 {
     if (!s || !s->uploaded) return FMDX_ESTATE;
     set_step(s->D, s->mass, active_group, dt, thermostat, tau, T0);
+    s->xq_fresh = 0;
     CK(launch_vv_start(s->D, s->st));
+    s->xq_fresh = s->D.xq_keep;
     return FMDX_OK;
 }
 
@@ -576,13 +587,16 @@ extern "C" int fmdx_se_step(fmdx_system_t *s, int active_group, double dt)
 {
     if (!s || !s->uploaded) return FMDX_ESTATE;
     set_step(s->D, s->mass, active_group, dt, 0, 1.0, 0.0);
+    s->xq_fresh = 0;
     CK(launch_se_step(s->D, s->st));
+    s->xq_fresh = s->D.xq_keep;
     return FMDX_OK;
 }
 
 extern "C" int fmdx_refresh(fmdx_system_t *s)
 {
     if (!s || !s->uploaded) return FMDX_ESTATE;
+    s->xq_fresh = 0;
     CK(launch_flag_only(s->D, s->st));
     return FMDX_OK;
 }
@@ -699,6 +713,7 @@ extern "C" int fmdx_set_capacity(fmdx_system_t *s, unsigned cap)
     CK(hipStreamSynchronize(s->st));
     free_slots(s->D);
     s->D = N;
+    s->xq_fresh = 0;   /* the new layout's FP32 positions are formed at the next force call */
     return FMDX_OK;
 }
 

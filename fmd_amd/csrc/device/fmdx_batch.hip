@@ -81,7 +81,11 @@ static int upload_descs(fmdx_batch_t *b)
 {
     const int n = (int)b->sys.size();
     std::vector<KDesc> h(n > 0 ? n : 1);
-    for (int i = 0; i < n; i++) h[i] = b->sys[i]->D;
+    for (int i = 0; i < n; i++)
+    {
+        h[i] = b->sys[i]->D;
+        h[i].xq_keep = b->precision == FMDX_PREC_MIXED;
+    }
     if (n) CK(hipMemcpyAsync(b->descs, h.data(), sizeof(KDesc) * n, hipMemcpyHostToDevice, b->st));
     /* the copy reads h, which ends with this call */
     CK(hipStreamSynchronize(b->st));
@@ -170,7 +174,10 @@ static int rebuild(fmdx_batch_t *b)
 static hipError_t enqueue_step(fmdx_batch_t *b, int energy)
 {
     hipError_t e = launchb_vv_start(b->plan, b->st);
-    if (e == hipSuccess) e = launchb_forces(b->plan, b->st, b->precision, energy);
+    /* mixed: the start step and the re-binning left every member's FP32 cell-relative
+       positions in place (the descriptors carry xq_keep), so the force passes need no
+       separate pass forming them */
+    if (e == hipSuccess) e = launchb_forces(b->plan, b->st, b->precision, energy, b->precision != FMDX_PREC_MIXED);
     if (e == hipSuccess) e = launchb_vv_finish(b->plan, b->st);
     return e;
 }

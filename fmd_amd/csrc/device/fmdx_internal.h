@@ -83,6 +83,7 @@ struct KDesc
     /* slots */
     double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz, *aux;
     float4 *xq;   /* mixed build: x y z relative to the home cell origin (FP32), and F'(rho) in w */
+    int xq_keep;  /* mixed build: the integrators write xq and the re-binning carries it along */
     int *grp;
     unsigned *aid;
     uint8_t *kind;
@@ -129,6 +130,18 @@ struct KDesc
     double hdtm[MAXK];     /* dt * 0.5 / mass per kind (integrators.c:67) */
     TtmDev ttm;            /* TTM friction source (forces.c:292-330) */
 };
+
+/* FP32 position relative to the origin of cell (cx, cy, cz), formed in FP64 without FMA
+   contraction and rounded once: the mixed build's force kernels read it (xq x y z; w is
+   F'(rho), written by the density pass).  The integrators, the re-binning and k_relpos all
+   form it here, so that each gives the same bits. */
+static __device__ __forceinline__ float4 rel_pos(const KDesc &D, int cx, int cy, int cz, double x, double y,
+                                                 double z)
+{
+#pragma clang fp contract(off)
+    const double ox = cx * D.h0, oy = cy * D.h1, oz = cz * D.h2;
+    return make_float4((float)(x - ox), (float)(y - oy), (float)(z - oz), 0.0f);
+}
 
 /* many systems in one launch: block b works on block items[b].y of system items[b].x */
 struct BatchMap
@@ -223,15 +236,17 @@ hipError_t launch_thermo(const KDesc &D, hipStream_t st);
 hipError_t launch_flag_only(const KDesc &D, hipStream_t st);
 hipError_t launch_bin(const KDesc &D, hipStream_t st);
 hipError_t launch_reduce_pe(const KDesc &D, hipStream_t st);
-hipError_t launch_forces(const KDesc &D, hipStream_t st, int precision);
-hipError_t launch_pass1(const KDesc &D, hipStream_t st, int precision);
+/* relpos: form the FP32 cell-relative positions first (mixed build), unless the
+   last integrator step and re-binning left them in place */
+hipError_t launch_forces(const KDesc &D, hipStream_t st, int precision, bool relpos = true);
+hipError_t launch_pass1(const KDesc &D, hipStream_t st, int precision, bool relpos = true);
 hipError_t launch_pass0(const KDesc &D, hipStream_t st, int precision);
 hipError_t launchb_vv_start(const BatchPlan &P, hipStream_t st);
 hipError_t launchb_vv_finish(const BatchPlan &P, hipStream_t st);
 hipError_t launchb_thermo(const BatchPlan &P, hipStream_t st);
 hipError_t launchb_reduce_pe(const BatchPlan &P, hipStream_t st);
 /* energy: also the pair energies and each member's PE sum (steps whose status is read) */
-hipError_t launchb_forces(const BatchPlan &P, hipStream_t st, int precision, int energy);
+hipError_t launchb_forces(const BatchPlan &P, hipStream_t st, int precision, int energy, bool relpos = true);
 hipError_t launchb_collect(const BatchPlan &P, hipStream_t st, unsigned *status, double *thermo);
 hipError_t launch_add_velocity(const KDesc &D, hipStream_t st, int group, double vx, double vy, double vz);
 hipError_t launch_relayout(const KDesc &Dold, const KDesc &Dnew, hipStream_t st);

@@ -18,6 +18,7 @@ struct fmdx_system
     unsigned natoms_uploaded;
     int uploaded;
     int member;            /* 1 while a batch holds this system's device pointers */
+    int xq_fresh;          /* mixed: the last integrator step left the FP32 cell-relative positions in place */
     double mass[MAXK];
     PairParam hp[MAXK * MAXK];
     PairParam *dpairs;

@@ -1479,8 +1479,7 @@ static __device__ void b_relpos(const KDesc &D, int blk)
 {
     each_slot(D, blk, [&](long slot, int c) {
         const int cz = c % D.nc2, t = c / D.nc2, cy = t % D.nc1, cx = t / D.nc1;
-        D.xq[slot] = make_float4((float)(D.x[slot] - cx * D.h0), (float)(D.y[slot] - cy * D.h1),
-                                 (float)(D.z[slot] - cz * D.h2), 0.0f);
+        D.xq[slot] = rel_pos(D, cx, cy, cz, D.x[slot], D.y[slot], D.z[slot]);
     });
 }
 __global__ __launch_bounds__(256) void k_relpos(KDesc D) { b_relpos(D, blockIdx.x); }
@@ -1592,11 +1591,11 @@ static void launch_cell(const KDesc &D, hipStream_t st)
 
 /* pass 1 (EAM density + F'(rho)) and pass 0 (forces + energy), launched separately so that
    callers can time each kernel with events on the stream */
-hipError_t launch_pass1(const KDesc &D, hipStream_t st, int precision)
+hipError_t launch_pass1(const KDesc &D, hipStream_t st, int precision, bool relpos)
 {
     if (precision == FMDX_PREC_MIXED)
     {
-        hipLaunchKernelGGL(k_relpos, dim3((D.ncell + CPB - 1) / CPB), dim3(TPB), 0, st, D);
+        if (relpos) hipLaunchKernelGGL(k_relpos, dim3((D.ncell + CPB - 1) / CPB), dim3(TPB), 0, st, D);
         if (D.has_eam) launch_cell<float, 1>(D, st);
         else (void)launch_fill(D.cell_emb, D.ncell, 0.0, st);
     }
@@ -1615,9 +1614,9 @@ hipError_t launch_pass0(const KDesc &D, hipStream_t st, int precision)
     return hipGetLastError();
 }
 
-hipError_t launch_forces(const KDesc &D, hipStream_t st, int precision)
+hipError_t launch_forces(const KDesc &D, hipStream_t st, int precision, bool relpos)
 {
-    hipError_t e = launch_pass1(D, st, precision);
+    hipError_t e = launch_pass1(D, st, precision, relpos);
     if (e != hipSuccess) return e;
     e = launch_pass0(D, st, precision);
     if (e != hipSuccess) return e;
@@ -1647,12 +1646,12 @@ static void launchb_cells(const BatchPlan &P, hipStream_t st, const int2 *const 
    the EAM systems, the force pass and, with `energy`, the pair energies and
    their sums; one launch per pair category */
 template <bool EN>
-static void launchb_passes(const BatchPlan &P, hipStream_t st, int precision)
+static void launchb_passes(const BatchPlan &P, hipStream_t st, int precision, bool relpos)
 {
     if (precision == FMDX_PREC_MIXED)
     {
         BatchMap ms = {P.descs, P.slot_items};
-        hipLaunchKernelGGL(kb_relpos, dim3(P.n_slot), dim3(TPB), 0, st, ms);
+        if (relpos) hipLaunchKernelGGL(kb_relpos, dim3(P.n_slot), dim3(TPB), 0, st, ms);
         launchb_cells<float, 1, true>(P, st, P.cell1, P.n_cell1);
         launchb_cells<float, 0, EN>(P, st, P.cell0, P.n_cell0);
     }
@@ -1663,11 +1662,11 @@ static void launchb_passes(const BatchPlan &P, hipStream_t st, int precision)
     }
 }
 
-hipError_t launchb_forces(const BatchPlan &P, hipStream_t st, int precision, int energy)
+hipError_t launchb_forces(const BatchPlan &P, hipStream_t st, int precision, int energy, bool relpos)
 {
     if (P.nsys == 0) return hipSuccess;
-    if (energy) launchb_passes<true>(P, st, precision);
-    else launchb_passes<false>(P, st, precision);
+    if (energy) launchb_passes<true>(P, st, precision, relpos);
+    else launchb_passes<false>(P, st, precision, relpos);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return energy ? launchb_reduce_pe(P, st) : hipSuccess;

@@ -47,6 +47,16 @@ static __device__ __forceinline__ void cell_coords(const KDesc &D, int c, int &c
     cx = t / D.nc1;
 }
 
+/* an integrator's new position: its cell-relative FP32 copy, for the cell the atom is in
+   (an atom that leaves gets its copy for the new cell when the re-binning places it) */
+static __device__ __forceinline__ void keep_rel(const KDesc &D, int c, long slot, double x, double y, double z)
+{
+    if (!D.xq_keep) return;
+    int cx, cy, cz;
+    cell_coords(D, c, cx, cy, cz);
+    D.xq[slot] = rel_pos(D, cx, cy, cz, x, y, z);
+}
+
 /* destination of one atom after it moved; misc.c:281-300 and md-ghost.c:827-869 */
 static __device__ void bin_flag(const KDesc &D, int c, long slot, double x, double y, double z)
 {
@@ -140,6 +150,7 @@ static __device__ void b_vv_start(const KDesc &D, int blk)
             D.vx[slot] = vx; D.vy[slot] = vy; D.vz[slot] = vz;
             D.x[slot] = x; D.y[slot] = y; D.z[slot] = z;
         }
+        keep_rel(D, c, slot, x, y, z);
         bin_flag(D, c, slot, x, y, z);
     });
 }
@@ -166,6 +177,7 @@ static __device__ void b_se_step(const KDesc &D, int blk)
             D.vx[slot] = vx; D.vy[slot] = vy; D.vz[slot] = vz;
             D.x[slot] = x; D.y[slot] = y; D.z[slot] = z;
         }
+        keep_rel(D, c, slot, x, y, z);
         bin_flag(D, c, slot, x, y, z);
     });
 }
@@ -366,18 +378,23 @@ __global__ __launch_bounds__(64) void kb_collect(const KDesc *descs, unsigned *s
 
 /* ----------------------------------------------------------- binning */
 
+/* a stayer moved within its cell: its cell-relative copy moves with it */
 static __device__ __forceinline__ void copy_slot(const KDesc &D, long from, long to)
 {
     D.x[to] = D.x[from]; D.y[to] = D.y[from]; D.z[to] = D.z[from];
     D.vx[to] = D.vx[from]; D.vy[to] = D.vy[from]; D.vz[to] = D.vz[from];
     D.grp[to] = D.grp[from]; D.aid[to] = D.aid[from]; D.kind[to] = D.kind[from];
+    if (D.xq_keep) D.xq[to] = D.xq[from];
 }
 
-static __device__ __forceinline__ void copy_staged(const KDesc &D, int mid, long to)
+/* an arrival (or a stayer staged with the early arrivals) placed in cell (cx, cy, cz):
+   its cell-relative copy is formed for that cell */
+static __device__ __forceinline__ void copy_staged(const KDesc &D, int mid, long to, int cx, int cy, int cz)
 {
     D.x[to] = D.mvx[mid]; D.y[to] = D.mvy[mid]; D.z[to] = D.mvz[mid];
     D.vx[to] = D.mvvx[mid]; D.vy[to] = D.mvvy[mid]; D.vz[to] = D.mvvz[mid];
     D.grp[to] = D.mvgrp[mid]; D.aid[to] = D.mvaid[mid]; D.kind[to] = D.mvkind[mid];
+    if (D.xq_keep) D.xq[to] = rel_pos(D, cx, cy, cz, D.mvx[mid], D.mvy[mid], D.mvz[mid]);
 }
 
 /* The replays below walk a cell's move flags in a data-dependent order, one
@@ -538,6 +555,8 @@ static __device__ void b_bin_apply(const KDesc &D, int blk)
 
 static __device__ int bin_apply_cell(const KDesc &D, int c, int nl, int na, int (*rows)[MVL + 1])
 {
+    int cx, cy, cz;
+    cell_coords(D, c, cx, cy, cz);
     const int nc = D.cnt[c];
     const int A = D.nearly[c];
     const long base = (long)c * D.cap;
@@ -571,7 +590,7 @@ static __device__ int bin_apply_cell(const KDesc &D, int c, int nl, int na, int 
             if (cur >= nc)
             {
                 int q = cur - nc;
-                if (q < nb && i < D.cap) copy_staged(D, D.bid[base + q], base + i);
+                if (q < nb && i < D.cap) copy_staged(D, D.bid[base + q], base + i, cx, cy, cz);
             }
             else if (cur != i)
                 copy_slot(D, base + cur, base + i);
@@ -582,7 +601,7 @@ static __device__ int bin_apply_cell(const KDesc &D, int c, int nl, int na, int 
     int total = n;
     for (int q = A; q < na; q++)
     {
-        if (q < nb && total < D.cap) copy_staged(D, D.bid[base + q], base + total);
+        if (q < nb && total < D.cap) copy_staged(D, D.bid[base + q], base + total, cx, cy, cz);
         total++;
     }
     if (total > D.cap)
@@ -624,6 +643,7 @@ __global__ __launch_bounds__(TPB) void k_relayout(KDesc A, KDesc B)
         B.vx[t] = A.vx[slot]; B.vy[t] = A.vy[slot]; B.vz[t] = A.vz[slot];
         B.fx[t] = A.fx[slot]; B.fy[t] = A.fy[slot]; B.fz[t] = A.fz[slot];
         B.aux[t] = A.aux[slot];
+        B.xq[t] = A.xq[slot];
         B.grp[t] = A.grp[slot]; B.aid[t] = A.aid[slot]; B.kind[t] = A.kind[slot];
     });
 }
