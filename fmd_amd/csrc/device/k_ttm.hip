@@ -360,12 +360,16 @@ __global__ __launch_bounds__(T3PB) void k_ttm_solve3(TtmState S, double time, do
     for (int a = threadIdx.x; a < S.n; a += T3PB) Te[a] = S.Te[a];
     const bool calc_source = fabs(time - S.laser_t0) < S.laser_tdiff;
     const double dt = mddt / S.ratio;
-    const double h2[3] = {S.tcellh_x * S.tcellh_x, S.tcellh_y * S.tcellh_y, S.tcellh_z * S.tcellh_z};
+    /* the divisors that do not change during the call are inverted once: 1 / h^2 per
+       dimension and K / Ti per owned cell (one FP64 division per cell and sub-step is left,
+       dt / (C Te); the single workgroup's sub-steps are bound by its VALU) */
+    const double ih2[3] = {1.0 / (S.tcellh_x * S.tcellh_x), 1.0 / (S.tcellh_y * S.tcellh_y),
+                           1.0 / (S.tcellh_z * S.tcellh_z)};
     /* other: the cell's value in the second buffer of the reference's swap; a
        cell that is not updated in a sub-step takes it, as a swap would give.
        The owned cells' neighbours (x-, y-, z-, x+, y+, z+), Ti and whether they
        are interior are fixed for the call. */
-    double xacc[T3OWN], nt[T3OWN], other[T3OWN], depth[T3OWN], tiv[T3OWN];
+    double xacc[T3OWN], nt[T3OWN], other[T3OWN], depth[T3OWN], tiv[T3OWN], kti[T3OWN];
     int nbi[T3OWN][6];
     bool inner[T3OWN];
     for (int k = 0; k < T3OWN; k++)
@@ -373,6 +377,7 @@ __global__ __launch_bounds__(T3PB) void k_ttm_solve3(TtmState S, double time, do
         xacc[k] = 0.0;
         depth[k] = 0.0;
         tiv[k] = 0.0;
+        kti[k] = 0.0;
         inner[k] = false;
         const int a = threadIdx.x + k * T3PB;
         for (int d = 0; d < 6; d++) nbi[k][d] = a;
@@ -382,6 +387,7 @@ __global__ __launch_bounds__(T3PB) void k_ttm_solve3(TtmState S, double time, do
         const int ix = col / S.ncy, iy = col % S.ncy;
         inner[k] = iz >= 1 && iz <= S.ncz;
         tiv[k] = S.Ti[a];
+        kti[k] = S.K / tiv[k];
         if (inner[k])
         {
             nbi[k][0] = (((ix + S.ncx - 1) % S.ncx) * S.ncy + iy) * S.nzg + iz;
@@ -411,7 +417,7 @@ __global__ __launch_bounds__(T3PB) void k_ttm_solve3(TtmState S, double time, do
             for (int k = 0; k < T3OWN; k++)
             {
                 const int a = threadIdx.x + k * T3PB;
-                if (a < S.n) kap[a] = S.K * Te[a] / tiv[k];
+                if (a < S.n) kap[a] = kti[k] * Te[a];
             }
             __syncthreads();
         }
@@ -431,7 +437,7 @@ __global__ __launch_bounds__(T3PB) void k_ttm_solve3(TtmState S, double time, do
             double flux = 0.0;
             if (S.kmode == 0)
             {
-                for (int d = 0; d < 3; d++) flux += S.K * (Te[hi[d]] - 2 * Te[a] + Te[lo[d]]) / h2[d];
+                for (int d = 0; d < 3; d++) flux += S.K * (Te[hi[d]] - 2 * Te[a] + Te[lo[d]]) * ih2[d];
             }
             else
             {
@@ -440,7 +446,7 @@ __global__ __launch_bounds__(T3PB) void k_ttm_solve3(TtmState S, double time, do
                 {
                     const double KeL = 0.5 * (ka + kap[lo[d]]);
                     const double KeR = 0.5 * (ka + kap[hi[d]]);
-                    flux += (KeR * (Te[hi[d]] - Te[a]) - KeL * (Te[a] - Te[lo[d]])) / h2[d];
+                    flux += (KeR * (Te[hi[d]] - Te[a]) - KeL * (Te[a] - Te[lo[d]])) * ih2[d];
                 }
             }
             const double source = calc_source ? spc * depth[k] : 0.0;
