@@ -41,6 +41,7 @@ static void free_slots(KDesc &D)
     dfree(D.x); dfree(D.y); dfree(D.z); dfree(D.vx); dfree(D.vy); dfree(D.vz);
     dfree(D.fx); dfree(D.fy); dfree(D.fz); dfree(D.aux);
     dfree(D.xq);
+    dfree(D.np);
     dfree(D.grp); dfree(D.aid); dfree(D.kind);
     dfree(D.mv); dfree(D.code); dfree(D.bkey); dfree(D.bid);
     dfree(D.mvx); dfree(D.mvy); dfree(D.mvz); dfree(D.mvvx); dfree(D.mvvy); dfree(D.mvvz);
@@ -167,7 +168,7 @@ extern "C" int fmdx_set_precision(fmdx_system_t *s, int precision)
     s->precision = precision;
     s->D.xq_keep = precision == FMDX_PREC_MIXED;
     s->xq_fresh = 0;
-    return FMDX_OK;
+    return ensure_n3(s, precision);
 }
 
 static void refresh_pair_summary(fmdx_system_t *s)
@@ -217,7 +218,7 @@ extern "C" int fmdx_set_pair(fmdx_system_t *s, int k1, int k2, int cat, double r
     refresh_pair_summary(s);
     CK(hipMemcpyAsync(s->dpairs, s->hp, sizeof(PairParam) * nk * nk, hipMemcpyHostToDevice, s->st));
     CK(hipStreamSynchronize(s->st));
-    return FMDX_OK;
+    return ensure_n3(s, s->precision);
 }
 
 /* reference spline (y, yDD) on knots x_k = k*h -> per-interval coefficients in b = x/h - k */
@@ -447,7 +448,7 @@ extern "C" int fmdx_upload(fmdx_system_t *s, unsigned natoms, const unsigned *ce
     dfree(dstart); dfree(dx); dfree(dv); dfree(dg); dfree(da); dfree(dk);
     s->natoms_uploaded = natoms;
     s->uploaded = 1;
-    return FMDX_OK;
+    return ensure_n3(s, s->precision);
 }
 
 extern "C" int fmdx_natoms(fmdx_system_t *s, unsigned *n)
@@ -537,11 +538,23 @@ extern "C" int fmdx_update_forces(fmdx_system_t *s, int active_group, int ttm_fr
     return fmdx_update_forces_ex(s, active_group, ttm_friction, FMDX_FORCES_ENERGY);
 }
 
+/* Allocated as soon as a system with slots qualifies (upload, precision, potentials,
+   capacity), so that the allocation-free entry point finds them and takes the same
+   kernels as fmdx_update_forces; the regular entry point and the batch engine also
+   allocate on first need. */
+int ensure_n3(fmdx_system_t *s, int precision)
+{
+    if (!s->uploaded || !n3_wanted(s->D, precision) || s->D.np) return FMDX_OK;
+    return dalloc(&s->D.np, (size_t)39 * s->D.ncell * s->D.cap);
+}
+
 extern "C" int fmdx_update_forces_ex(fmdx_system_t *s, int active_group, int ttm_friction, int flags)
 {
     if (!s || !s->uploaded) return FMDX_ESTATE;
     if (s->D.has_eam && !s->rhoc) return FMDX_ESTATE;
     if (flags & ~FMDX_FORCES_ENERGY) return FMDX_EARG;
+    int r = ensure_n3(s, s->precision);
+    if (r) return r;
     s->D.ag = active_group;
     s->D.energy = (flags & FMDX_FORCES_ENERGY) ? 1 : 0;
     s->D.ttm = ttm_dev(s, ttm_friction);
@@ -676,6 +689,7 @@ extern "C" int fmdx_memory(fmdx_system_t *s, size_t *bytes)
     b += (size_t)D.ncell * (6 * sizeof(int) + 2 * sizeof(double) + sizeof(unsigned long long));
     if (D.hbits) b += (size_t)D.ncell * HBITS_WORDS * force_threads_per_cell() * sizeof(unsigned long long);
     if (s->rhoc) b += (size_t)(D.eam.nrho + 6 * D.eam.nr2) * (sizeof(double4) + sizeof(float4));
+    if (D.np) b += ns * 39 * sizeof(float);
     *bytes = b;
     return FMDX_OK;
 }
@@ -698,6 +712,7 @@ extern "C" int fmdx_set_capacity(fmdx_system_t *s, unsigned cap)
     KDesc N = s->D;
     N.x = N.y = N.z = N.vx = N.vy = N.vz = N.fx = N.fy = N.fz = N.aux = nullptr;
     N.xq = nullptr;
+    N.np = nullptr;   /* the partner sums of the new layout are allocated at the next force call */
     N.grp = nullptr; N.aid = nullptr; N.kind = nullptr; N.mv = nullptr; N.code = nullptr;
     N.bkey = nullptr; N.bid = nullptr;
     N.mvx = N.mvy = N.mvz = N.mvvx = N.mvvy = N.mvvz = nullptr;
@@ -714,7 +729,7 @@ extern "C" int fmdx_set_capacity(fmdx_system_t *s, unsigned cap)
     free_slots(s->D);
     s->D = N;
     s->xq_fresh = 0;   /* the new layout's FP32 positions are formed at the next force call */
-    return FMDX_OK;
+    return ensure_n3(s, s->precision);
 }
 
 /* ------------------------------------------------------------------ TTM */

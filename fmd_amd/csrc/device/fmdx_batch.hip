@@ -149,6 +149,12 @@ static int rebuild(fmdx_batch_t *b)
         all.insert(all.end(), lists[k]->begin(), lists[k]->end());
     }
     if (total) CK(hipMemcpyAsync(b->items, all.data(), sizeof(int2) * total, hipMemcpyHostToDevice, b->st));
+    /* members whose force passes evaluate each pair once need their partner sums */
+    for (int i = 0; i < n; i++)
+    {
+        const int e = ensure_n3(b->sys[i], b->precision);
+        if (e) return e;
+    }
     int r = upload_descs(b);
     if (r) return r;
     BatchPlan &P = b->plan;
@@ -163,6 +169,7 @@ static int rebuild(fmdx_batch_t *b)
         P.cell1[c] = ptr[7 + c]; P.n_cell1[c] = (int)c1[c].size();
     }
     P.max_nkinds = maxk;
+    P.n3 = b->precision == FMDX_PREC_MIXED;   /* every LJ and Morse member holds partner sums */
     b->hstatus.assign((size_t)std::max(n, 1) * 8, 0);
     b->hthermo.assign((size_t)std::max(n, 1) * 16, 0.0);
     CK(hipStreamSynchronize(b->st));
@@ -247,6 +254,7 @@ static int grow(fmdx_batch_t *b)
         int r = fmdx_set_capacity(s, nc);
         s->member = 1;
         if (r) return r;
+        if ((r = ensure_n3(s, b->precision))) return r;   /* the new layout's partner sums */
         changed = 1;
     }
     if (changed)

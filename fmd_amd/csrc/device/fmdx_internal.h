@@ -83,6 +83,8 @@ struct KDesc
     /* slots */
     double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz, *aux;
     float4 *xq;   /* mixed build: x y z relative to the home cell origin (FP32), and F'(rho) in w */
+    float *np;    /* mixed build, single-kind LJ or Morse, each pair once: partner sums
+                     [3][13][ncell*cap] (x, y, z; forward direction; slot), or null */
     int xq_keep;  /* mixed build: the integrators write xq and the re-binning carries it along */
     int *grp;
     unsigned *aid;
@@ -225,6 +227,7 @@ struct BatchPlan
     const int2 *cell1[4];        /* (system, cell) for the EAM density pass, per pair category */
     int n_cell1[4];
     int max_nkinds;
+    int n3;                      /* the LJ and Morse members evaluate each pair once (all have partner sums) */
 };
 
 /* kernel launchers (defined in the kernel translation units); the step
@@ -247,6 +250,8 @@ hipError_t launchb_thermo(const BatchPlan &P, hipStream_t st);
 hipError_t launchb_reduce_pe(const BatchPlan &P, hipStream_t st);
 /* energy: also the pair energies and each member's PE sum (steps whose status is read) */
 hipError_t launchb_forces(const BatchPlan &P, hipStream_t st, int precision, int energy, bool relpos = true);
+/* each pair evaluated once (partner sums, then a fold) for this system in this precision */
+bool n3_wanted(const KDesc &D, int precision);
 hipError_t launchb_collect(const BatchPlan &P, hipStream_t st, unsigned *status, double *thermo);
 hipError_t launch_add_velocity(const KDesc &D, hipStream_t st, int group, double vx, double vy, double vz);
 hipError_t launch_relayout(const KDesc &Dold, const KDesc &Dnew, hipStream_t st);

@@ -79,4 +79,8 @@ TtmDev ttm_dev(fmdx_system_t *s, int on);
 /* pair-category specialisation of the force kernels for a system (k_force.hip) */
 int force_category(const KDesc &D);
 
+/* the partner-sum arrays of a system whose force passes evaluate each pair once in this
+   precision (allocated on first need; fmdx_api.hip) */
+int ensure_n3(fmdx_system_t *s, int precision);
+
 #endif /* FMDX_SYS_H */

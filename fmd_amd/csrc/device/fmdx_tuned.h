@@ -66,4 +66,10 @@
 #ifndef FMDX_SPLIT_MAX
 #define FMDX_SPLIT_MAX 64
 #endif
+#ifndef FMDX_N3
+#define FMDX_N3 0
+#endif
+#ifndef FMDX_TILE32_N3
+#define FMDX_TILE32_N3 256
+#endif
 #endif

@@ -697,9 +697,33 @@ static __device__ __forceinline__ void flush_eam1(const KDesc &D, const TileT &T
 /* Single-kind LJ or Morse: the set bits of a hit word two per iteration
    without branches, as flush_eam1 does; a slot without a pair evaluates the
    formulas at r^2 = rc^2 and adds exact zeros. */
-/* one exact single-kind LJ or Morse pair (lj.h:26-58, morse.h:26-58) */
-template <int CAT, bool EN, typename A>
-static __device__ __forceinline__ void exact_pair1(const KDesc &D, const ExactCtx &X, int g, const PairParam &p, A *acc)
+/* Each pair once (FMDX_N3, FP32 single-kind LJ and Morse): the home cell stages itself
+   and its 13 forward neighbours; a pair with a candidate of a forward cell adds the
+   force on the home atom to that atom's sums and its opposite to the candidate's entry
+   of the tile's partner sums (LDS, one wave: a fixed order), and its whole energy to
+   the home atom.  Pairs within the home cell are met from both sides, as before: home
+   atom's force only, half the energy each.  Tile entries below jmin are home-cell
+   candidates. */
+struct JSide
+{
+    float *jx, *jy, *jz;
+    int jmin;
+};
+
+/* forward direction f (0..12): the offsets after (0, 0, 0) in (x, y, z) order,
+   (1, *, *), (0, 1, *), (0, 0, 1) */
+static __device__ __forceinline__ void n3_offset(int f, int &ox, int &oy, int &oz)
+{
+    ox = f < 9 ? 1 : 0;
+    oy = f < 9 ? f / 3 - 1 : (f < 12 ? 1 : 0);
+    oz = f < 9 ? f % 3 - 1 : (f < 12 ? f - 10 : 1);
+}
+
+/* one exact single-kind LJ or Morse pair (lj.h:26-58, morse.h:26-58); N3: m is the
+   candidate's tile entry */
+template <int CAT, bool EN, bool N3, typename A>
+static __device__ __forceinline__ void exact_pair1(const KDesc &D, const ExactCtx &X, int g, const PairParam &p, A *acc,
+                                                   int m, const JSide &J)
 {
     double dx, dy, dz;
     exact_sep(D, X, g, dx, dy, dz);
@@ -708,10 +732,21 @@ static __device__ __forceinline__ void exact_pair1(const KDesc &D, const ExactCt
     double fac, en;
     if (CAT == CAT_LJ) lj_terms<double>(p, r2, fac, en);
     else morse_terms<double>(p, r2, fac, en);
-    acc[0].add((float)(fac * dx));
-    acc[1].add((float)(fac * dy));
-    acc[2].add((float)(fac * dz));
-    if (EN) acc[3].add((float)en);
+    const float fx = (float)(fac * dx), fy = (float)(fac * dy), fz = (float)(fac * dz);
+    acc[0].add(fx);
+    acc[1].add(fy);
+    acc[2].add(fz);
+    const bool fw = N3 && m >= J.jmin;
+    if constexpr (N3)
+    {
+        if (fw)
+        {
+            atomicAdd(&J.jx[m], -fx);
+            atomicAdd(&J.jy[m], -fy);
+            atomicAdd(&J.jz[m], -fz);
+        }
+    }
+    if (EN) acc[3].add(fw ? (float)(2.0 * en) : (float)en);
 }
 
 /* one exact pair of any category (the generic dispatch of pair_hit in FP64) */
@@ -728,10 +763,10 @@ static __device__ __forceinline__ void exact_pair(const KDesc &D, const ExactCtx
     for (int q = 0; q < 4; q++) acc[q].add((float)t[q].get());
 }
 
-template <typename R, int CAT, bool EN, typename TileT, typename A>
+template <typename R, int CAT, bool EN, bool N3, typename TileT, typename A>
 static __device__ __forceinline__ void flush_pair2(const KDesc &D, const TileT &T, unsigned long long word, int s,
                                                    int g0, int S, R xi, R yi, R zi, const PairParam &p,
-                                                   bool &ex, A *acc)
+                                                   bool &ex, A *acc, const JSide &J)
 {
     constexpr bool EX = sizeof(R) == sizeof(float);   /* FP32: a pair in the rounding band below rc^2 sets ex */
     const R rc2 = (R)p.rc2;
@@ -774,7 +809,28 @@ static __device__ __forceinline__ void flush_pair2(const KDesc &D, const TileT &
             wpx = __builtin_fmaf(f2, dx2, __builtin_fmaf(f1, dx1, wpx));
             wpy = __builtin_fmaf(f2, dy2, __builtin_fmaf(f1, dy1, wpy));
             wpz = __builtin_fmaf(f2, dz2, __builtin_fmaf(f1, dz1, wpz));
-            if (EN) wpe += (u1 ? en1 : (R)0) + (u2 ? en2 : (R)0);
+            if constexpr (N3)
+            {
+                /* the forward candidates' opposite forces, only from lanes with such a pair: the
+                   empty slots of a stripe all name one entry, and adds of theirs to it would
+                   serialise in the LDS */
+                const int m1 = m0 + (int)__umul24((unsigned)t1, (unsigned)S), m2 = m0 + (int)__umul24((unsigned)t2, (unsigned)S);
+                const bool w1 = u1 && m1 >= J.jmin, w2 = u2 && m2 >= J.jmin;
+                if (w1)
+                {
+                    atomicAdd(&J.jx[m1], -f1 * dx1);
+                    atomicAdd(&J.jy[m1], -f1 * dy1);
+                    atomicAdd(&J.jz[m1], -f1 * dz1);
+                }
+                if (w2)
+                {
+                    atomicAdd(&J.jx[m2], -f2 * dx2);
+                    atomicAdd(&J.jy[m2], -f2 * dy2);
+                    atomicAdd(&J.jz[m2], -f2 * dz2);
+                }
+                if (EN) wpe += (u1 ? (w1 ? (R)2 * en1 : en1) : (R)0) + (u2 ? (w2 ? (R)2 * en2 : en2) : (R)0);
+            }
+            else if (EN) wpe += (u1 ? en1 : (R)0) + (u2 ? en2 : (R)0);
         }
         else
         {
@@ -817,9 +873,10 @@ static __device__ unsigned long long band_bits(const TileT &T, unsigned long lon
 
 /* the pairs of the exact shell collected from one hit word (bit t: trip t of
    this lane's stripe), evaluated in FP64 after the word's FP32 flush */
-template <int PASS, int CAT, bool EN, typename TileT, typename A>
+template <int PASS, int CAT, bool EN, bool N3, typename TileT, typename A>
 static __device__ __forceinline__ void flush_exact(const KDesc &D, const TileT &T, unsigned long long xw, int s, int g0, int S,
-                                   const ExactCtx &X, const PairParam *ppi, const PairParam &p1, A *acc)
+                                   const ExactCtx &X, const PairParam *ppi, const PairParam &p1, A *acc,
+                                   const JSide &J)
 {
     while (__ballot(xw != 0ull))
     {
@@ -829,7 +886,7 @@ static __device__ __forceinline__ void flush_exact(const KDesc &D, const TileT &
             xw &= xw - 1ull;
             const int m = tpos(s, g0 + t, S);
             if constexpr (CAT == CAT_EAM1) exact_eam1<PASS, EN>(D, X, T.g[m], p1.rc2, acc);
-            else if constexpr (CAT == CAT_LJ || CAT == CAT_MORSE) exact_pair1<CAT, EN>(D, X, T.g[m], p1, acc);
+            else if constexpr (CAT == CAT_LJ || CAT == CAT_MORSE) exact_pair1<CAT, EN, N3>(D, X, T.g[m], p1, acc, m, J);
             else exact_pair<PASS, EN>(D, X, T.g[m], ppi[T.k[m]], acc);
         }
     }
@@ -870,13 +927,19 @@ static __host__ __device__ __forceinline__ unsigned lt_offset(int nkinds)
    spline table resident in LDS (k_cell_lds): the calling wave is one of LW
    in its workgroup, each with its own tile, and the table follows the pair
    parameters in the dynamic LDS. */
-template <typename R, int PASS, int CAT, bool EN, bool LT = false>
+template <typename R, int PASS, int CAT, bool EN, bool LT = false, bool N3 = false>
 static __device__ void b_cell(const KDesc &D, const int c)
 {
     typedef typename AccOf<R>::T Acc;
     typedef Prec<R> P;
     static_assert(!LT || (sizeof(R) == sizeof(float) && CAT == CAT_EAM1 && FMDX_WAVES_PER_CELL == 1),
                   "LDS tables: FP32 single-element EAM, one wave per cell");
+    static_assert(!N3 || (sizeof(R) == sizeof(float) && (CAT == CAT_LJ || CAT == CAT_MORSE) && PASS == 0 && !LT &&
+                          FMDX_WAVES_PER_CELL == 1),
+                  "each pair once: FP32 single-kind LJ or Morse, one wave per cell");
+    /* neighbour cells staged: all 27, or (N3) the home cell and its 13 forward neighbours;
+       the list entry of the home cell */
+    constexpr int NNB = N3 ? 14 : 27, HOME = N3 ? 0 : 13;
     /* candidates per tile, per precision and pair category (fmdx_tuned.h): the FP32 EAM kernels
        keep a smaller tile, so that the density pass's LDS (tile plus the list places of the exact
        pairs) still lets 20 workgroups share a CU */
@@ -884,6 +947,7 @@ static __device__ void b_cell(const KDesc &D, const int c)
     {
         CT = LT ? FMDX_EAM_LDS_TILE
            : sizeof(R) == sizeof(double) ? (int)P::CT
+           : N3 ? FMDX_TILE32_N3
            : CAT == CAT_EAM1 ? FMDX_TILE32_EAM : CAT == CAT_LJ ? FMDX_TILE32_LJ
            : CAT == CAT_MORSE ? FMDX_TILE32_MORSE : (int)P::CT
     };
@@ -904,6 +968,8 @@ static __device__ void b_cell(const KDesc &D, const int c)
     __shared__ __align__(16) int nb_pre_s[NWS][32];   /* 28 prefixes, then INT_MAX */
     __shared__ Shifts<R> SH_s[NWS];
     __shared__ Tile<R, WITH_FP, CAT == CAT_GEN, CT> T_s[NWS];
+    /* N3: the partner sums of the tile's entries (x, y, z) */
+    __shared__ float jside_s[N3 ? 3 : 1][N3 ? CT + NT : 1];
     long *const nb_base = nb_base_s[ws];
     int *const nb_pre = nb_pre_s[ws];
     Shifts<R> &SH = SH_s[ws];
@@ -933,9 +999,16 @@ static __device__ void b_cell(const KDesc &D, const int c)
         tiles_ld = D.htile[c];
     }
     int my_cnt = 0;
-    if (lane < 27)
+    if (lane < NNB)
     {
-        int jx = cx + lane / 9 - 1, jy = cy + (lane / 3) % 3 - 1, jz = cz + lane % 3 - 1;
+        int ox = lane / 9 - 1, oy = (lane / 3) % 3 - 1, oz = lane % 3 - 1;
+        if constexpr (N3)
+        {
+            /* entry 0 the home cell, entry 1 + f forward direction f */
+            if (lane == 0) ox = oy = oz = 0;
+            else n3_offset(lane - 1, ox, oy, oz);
+        }
+        int jx = cx + ox, jy = cy + oy, jz = cz + oz;
         int wx = 0, wy = 0, wz = 0;
         bool ok = true;
         if (jx < 0) { if (D.pbc0) { jx += D.nc0; wx = -1; } else ok = false; }
@@ -962,11 +1035,11 @@ static __device__ void b_cell(const KDesc &D, const int c)
         const int row0 = __builtin_amdgcn_readlane(incl, 15);
         if ((lane & (WG - 1)) >= 16 && (lane & (WG - 1)) < 32) incl += row0;
     }
-    if (lane < 27) nb_pre[lane + 1] = incl;
+    if (lane < NNB) nb_pre[lane + 1] = incl;
     else if (lane < 31) nb_pre[lane + 1] = 0x7fffffff;
     if (lane == 0) nb_pre[0] = 0;
     csync<LT>();
-    const int ncand = __builtin_amdgcn_readfirstlane(nb_pre[27]);
+    const int ncand = __builtin_amdgcn_readfirstlane(nb_pre[NNB]);
     /* Neighbour cell of candidate g of the 27-cell list: the last prefix not above g
        (empty cells share their successor's prefix and are passed over).  Two levels:
        the block of eight prefixes from three wave-uniform pivots, then one 32-byte
@@ -1014,7 +1087,9 @@ static __device__ void b_cell(const KDesc &D, const int c)
        and the flush.  Cells of more than 32 atoms (up to FMDX_SPLIT_MAX) are taken in
        blocks of 32 atoms, two stripes each, and the rest, the tile staged for each
        block (38 atoms: nt / 2 + nt / 10 scan trips instead of nt). */
-    const int bsz = (FMDX_WAVES_PER_CELL == 1 && nh > WG / 2 && nh <= FMDX_SPLIT_MAX) ? WG / 2 : NT;
+    const int bsz = (!N3 && FMDX_WAVES_PER_CELL == 1 && nh > WG / 2 && nh <= FMDX_SPLIT_MAX) ? WG / 2 : NT;
+    /* N3: partner sums per forward direction d (0..12) and slot, x then y then z arrays */
+    const long nslot = (long)D.ncell * D.cap;
     const bool one_block = nh > 0 && nh <= bsz;
     const int hw = __builtin_amdgcn_readfirstlane(hw_ld);
     /* a density pass without hit-word storage (hbits null) records nothing: the force pass rescans */
@@ -1084,9 +1159,10 @@ static __device__ void b_cell(const KDesc &D, const int c)
                candidate farther than the cutoff from the cell's box cannot pair
                with any home atom; survivors are compacted in list order */
             int nt = 0;
+            int jmin = 0;   /* N3: home-cell candidates staged in this round (the tile's first entries) */
             /* tile position of this lane's home atom in this round, -1 while it is not in the tile */
             int selft = -1;
-            const int gself = nb_pre[13] + h0 + a;   /* its place in the 27-cell candidate list */
+            const int gself = nb_pre[HOME] + h0 + a;   /* its place in the candidate list */
             if constexpr (FMDX_WAVES_PER_CELL > 1)
             {
                 if (lane < nhc) selfpos[lane] = -1;
@@ -1101,14 +1177,16 @@ static __device__ void b_cell(const KDesc &D, const int c)
             {
                 R x, y, z, f;
                 int kj, lo, gr;
+                long js;
             };
             auto fetch = [&](int g, Cand &q) {
                 q.x = q.y = q.z = q.f = (R)0;
-                q.kj = 0; q.lo = 0; q.gr = ag;
+                q.kj = 0; q.lo = 0; q.gr = ag; q.js = 0;
                 if (g < ncand)
                 {
                     q.lo = locate(g);
                     const long js = nb_base[q.lo] + (g - nb_pre[q.lo]);
+                    q.js = js;
                     P::template load<WITH_FP>(D, js, q.x, q.y, q.z, q.f);
                     if (CAT == CAT_GEN) q.kj = D.kind[js];
                     if (ag >= 0) q.gr = D.grp[js];
@@ -1158,19 +1236,39 @@ static __device__ void b_cell(const KDesc &D, const int c)
                 {
                     T.put(off, x, y, z, NSCAN ? x * x + y * y + z * z : fpv, kj);
                     if constexpr (sizeof(R) == sizeof(float)) T.g[off] = (unsigned short)g;
+                    if constexpr (N3)
+                    {
+                        jside_s[0][off] = 0.0f;
+                        jside_s[1][off] = 0.0f;
+                        jside_s[2][off] = 0.0f;
+                    }
                 }
-                /* the lane that staged this lane's home atom (list entry 13 is the home
+                if constexpr (N3)
+                {
+                    /* home-cell candidates come first in list order; a forward candidate that
+                       is not staged gets a zero partner sum from this cell (written once, by
+                       the first block of home atoms) */
+                    jmin += __popcll(__ballot(keep && lo == HOME));
+                    if (h0 == 0 && g < ncand && !keep && lo != HOME)
+                    {
+                        float *const pj = D.np + (long)(lo - 1) * nslot + cur.js;
+                        pj[0] = 0.0f;
+                        pj[13 * nslot] = 0.0f;
+                        pj[26 * nslot] = 0.0f;
+                    }
+                }
+                /* the lane that staged this lane's home atom (list entry HOME is the home
                    cell itself, not an image) hands over the tile position it wrote */
                 if constexpr (FMDX_WAVES_PER_CELL == 1)
                 {
                     const int src = gself - gpos;
-                    const int got = __shfl((keep && lo == 13) ? off : -1, src & (WG - 1));
+                    const int got = __shfl((keep && lo == HOME) ? off : -1, src & (WG - 1));
                     if (src >= 0 && src < WG) selft = got;
                 }
                 else
                 {
-                    const int hk = g - nb_pre[13] - h0;
-                    if (keep && lo == 13 && hk >= 0 && hk < nhc) selfpos[hk] = (short)off;
+                    const int hk = g - nb_pre[HOME] - h0;
+                    if (keep && lo == HOME && hk >= 0 && hk < nhc) selfpos[hk] = (short)off;
                 }
                 nt += kept;
                 gpos += NT;
@@ -1197,6 +1295,7 @@ static __device__ void b_cell(const KDesc &D, const int c)
             csync<LT>();
             if constexpr (FMDX_WAVES_PER_CELL > 1) selft = selfpos[a];
             if (s >= S) selft = -1;
+            const JSide J = {jside_s[0], jside_s[N3 ? 1 : 0], jside_s[N3 ? 2 : 0], jmin};
             for (int g0 = 0; g0 < trips; g0 += 64)
             {
                 const int ng = min(64, trips - g0);
@@ -1269,7 +1368,7 @@ static __device__ void b_cell(const KDesc &D, const int c)
                        find its pairs rather than tracked pair by pair */
                     const unsigned long long w0 = word;
                     bool ex = false;
-                    flush_pair2<R, CAT, EN>(D, T, word, s, g0, S, xi, yi, zi, p1, ex, acc);
+                    flush_pair2<R, CAT, EN, N3>(D, T, word, s, g0, S, xi, yi, zi, p1, ex, acc, J);
                     if constexpr (sizeof(R) == sizeof(float))
                         if (__ballot(ex)) xw = band_bits(T, w0, s, g0, S, xi, yi, zi, (R)(p1.rc2 * (1.0 - RX_BAND)));
                 }
@@ -1302,9 +1401,36 @@ static __device__ void b_cell(const KDesc &D, const int c)
                     }
                 if constexpr (sizeof(R) == sizeof(float))
                 {
-                    if (__ballot(xw != 0ull)) flush_exact<PASS, CAT, EN>(D, T, xw, s, g0, S, X, ppi, p1, acc);
+                    if (__ballot(xw != 0ull)) flush_exact<PASS, CAT, EN, N3>(D, T, xw, s, g0, S, X, ppi, p1, acc, J);
                 }
                 wi++;
+            }
+            if constexpr (N3)
+            {
+                /* the round's forward candidates hand their partner sums over, each to the
+                   entry of its slot for the direction it lies in; a second block of home atoms
+                   (cells of more than 64) adds to what the first wrote */
+                csync<LT>();
+                if (h0 > 0) __threadfence();
+                for (int m = jmin + lane; m < nt; m += NT)
+                {
+                    const int g = T.g[m];
+                    const int lo = locate(g);
+                    float *const pj = D.np + (long)(lo - 1) * nslot + nb_base[lo] + (g - nb_pre[lo]);
+                    const float vx = jside_s[0][m], vy = jside_s[1][m], vz = jside_s[2][m];
+                    if (h0 == 0)
+                    {
+                        pj[0] = vx;
+                        pj[13 * nslot] = vy;
+                        pj[26 * nslot] = vz;
+                    }
+                    else
+                    {
+                        pj[0] += vx;
+                        pj[13 * nslot] += vy;
+                        pj[26 * nslot] += vz;
+                    }
+                }
             }
         }
         if (record && lane == 0)
@@ -1434,8 +1560,11 @@ static __device__ __forceinline__ int xcd_cell(const KDesc &D, int b, int n)
     return (px * D.nc1 + py) * D.nc2 + pz;
 }
 
-template <typename R, int PASS, int CAT, bool EN>
-__global__ __launch_bounds__(NT) void k_cell(KDesc D) { b_cell<R, PASS, CAT, EN>(D, xcd_cell(D, blockIdx.x, gridDim.x)); }
+template <typename R, int PASS, int CAT, bool EN, bool N3 = false>
+__global__ __launch_bounds__(NT) void k_cell(KDesc D)
+{
+    b_cell<R, PASS, CAT, EN, false, N3>(D, xcd_cell(D, blockIdx.x, gridDim.x));
+}
 
 #if FMDX_EAM_LDS
 /* FP32 single-element EAM pass with its spline table resident in LDS: one
@@ -1467,12 +1596,58 @@ __global__ __launch_bounds__(WG * LW) void k_cell_lds(KDesc D)
 }
 #endif
 
-template <typename R, int PASS, int CAT, bool EN>
+template <typename R, int PASS, int CAT, bool EN, bool N3 = false>
 __global__ __launch_bounds__(NT) void kb_cell(BatchMap M)
 {
     const int2 it = M.items[blockIdx.x];
     const KDesc &D = M.descs[it.x];
-    b_cell<R, PASS, CAT, EN>(D, it.y);
+    b_cell<R, PASS, CAT, EN, false, N3>(D, it.y);
+}
+
+/* N3: a slot's force is its own home-side sum plus the partner sums left for it by the
+   home cells of its 13 backward directions, in direction order (FP64 sums of FP32
+   terms); a direction without a home cell (an open face) or with an empty one adds
+   nothing (an empty home cell writes no partner sums) */
+static __device__ void b_fold(const KDesc &D, int blk)
+{
+    if (!D.np) return;
+    const long nslot = (long)D.ncell * D.cap;
+    each_slot(D, blk, [&](long slot, int c) {
+        const int cz = c % D.nc2, t = c / D.nc2, cy = t % D.nc1, cx = t / D.nc1;
+        double fx = D.fx[slot], fy = D.fy[slot], fz = D.fz[slot];
+        for (int f = 0; f < 13; f++)
+        {
+            int ox, oy, oz;
+            n3_offset(f, ox, oy, oz);
+            int hx = cx - ox, hy = cy - oy, hz = cz - oz;
+            bool ok = true;
+            if (hx < 0) { if (D.pbc0) hx += D.nc0; else ok = false; }
+            if (hy < 0) { if (D.pbc1) hy += D.nc1; else ok = false; }
+            else if (hy >= D.nc1) { if (D.pbc1) hy -= D.nc1; else ok = false; }
+            if (hz < 0) { if (D.pbc2) hz += D.nc2; else ok = false; }
+            else if (hz >= D.nc2) { if (D.pbc2) hz -= D.nc2; else ok = false; }
+            if (!ok || D.cnt[((long)hx * D.nc1 + hy) * D.nc2 + hz] == 0) continue;
+            const float *const pj = D.np + (long)f * nslot + slot;
+            fx += (double)pj[0];
+            fy += (double)pj[13 * nslot];
+            fz += (double)pj[26 * nslot];
+        }
+        if (!(__builtin_isfinite(fx) && __builtin_isfinite(fy) && __builtin_isfinite(fz)))
+        {
+            set_err(D, FMDX_DEV_ERR_FORCE);
+            note_first(&D.counters[4], (unsigned)c + 1u);
+        }
+        D.fx[slot] = fx;
+        D.fy[slot] = fy;
+        D.fz[slot] = fz;
+    });
+}
+__global__ __launch_bounds__(256) void k_fold(KDesc D) { b_fold(D, blockIdx.x); }
+__global__ __launch_bounds__(256) void kb_fold(BatchMap M)
+{
+    const int2 it = M.items[blockIdx.x];
+    const KDesc &D = M.descs[it.x];
+    b_fold(D, it.y);
 }
 /* FP32 positions relative to the origin of the home cell, from the FP64 positions */
 static __device__ void b_relpos(const KDesc &D, int blk)
@@ -1575,10 +1750,28 @@ static void launch_cell_en(const KDesc &D, hipStream_t st)
 #endif
             hipLaunchKernelGGL((k_cell<R, PASS, CAT_EAM1, EN>), g, b, lds, st, D);
             break;
-        case CAT_LJ: hipLaunchKernelGGL((k_cell<R, PASS, CAT_LJ, EN>), g, b, lds, st, D); break;
-        case CAT_MORSE: hipLaunchKernelGGL((k_cell<R, PASS, CAT_MORSE, EN>), g, b, lds, st, D); break;
+        case CAT_LJ:
+        case CAT_MORSE:
+            if constexpr (sizeof(R) == sizeof(float) && PASS == 0 && FMDX_N3 && FMDX_WAVES_PER_CELL == 1)
+                if (D.np)
+                {
+                    if (force_category(D) == CAT_LJ) hipLaunchKernelGGL((k_cell<R, PASS, CAT_LJ, EN, true>), g, b, lds, st, D);
+                    else hipLaunchKernelGGL((k_cell<R, PASS, CAT_MORSE, EN, true>), g, b, lds, st, D);
+                    hipLaunchKernelGGL(k_fold, dim3((D.ncell + CPB - 1) / CPB), dim3(TPB), 0, st, D);
+                    break;
+                }
+            if (force_category(D) == CAT_LJ) hipLaunchKernelGGL((k_cell<R, PASS, CAT_LJ, EN>), g, b, lds, st, D);
+            else hipLaunchKernelGGL((k_cell<R, PASS, CAT_MORSE, EN>), g, b, lds, st, D);
+            break;
         default: hipLaunchKernelGGL((k_cell<R, PASS, CAT_GEN, EN>), g, b, lds, st, D); break;
     }
+}
+
+/* each pair evaluated once for this system: mixed precision, single-kind LJ or Morse */
+bool n3_wanted(const KDesc &D, int precision)
+{
+    const int cat = force_category(D);
+    return FMDX_N3 && FMDX_WAVES_PER_CELL == 1 && precision == FMDX_PREC_MIXED && (cat == CAT_LJ || cat == CAT_MORSE);
 }
 
 /* the density pass has one form; the force pass takes the pair energy when D.energy is set */
@@ -1635,11 +1828,34 @@ static void launchb_cells(const BatchPlan &P, hipStream_t st, const int2 *const 
         switch (cat)
         {
             case CAT_EAM1: hipLaunchKernelGGL((kb_cell<R, PASS, CAT_EAM1, EN>), g, b, ppb, st, mc); break;
-            case CAT_LJ: hipLaunchKernelGGL((kb_cell<R, PASS, CAT_LJ, EN>), g, b, ppb, st, mc); break;
-            case CAT_MORSE: hipLaunchKernelGGL((kb_cell<R, PASS, CAT_MORSE, EN>), g, b, ppb, st, mc); break;
+            case CAT_LJ:
+                if constexpr (sizeof(R) == sizeof(float) && PASS == 0 && FMDX_N3 && FMDX_WAVES_PER_CELL == 1)
+                    if (P.n3)
+                    {
+                        hipLaunchKernelGGL((kb_cell<R, PASS, CAT_LJ, EN, true>), g, b, ppb, st, mc);
+                        break;
+                    }
+                hipLaunchKernelGGL((kb_cell<R, PASS, CAT_LJ, EN>), g, b, ppb, st, mc);
+                break;
+            case CAT_MORSE:
+                if constexpr (sizeof(R) == sizeof(float) && PASS == 0 && FMDX_N3 && FMDX_WAVES_PER_CELL == 1)
+                    if (P.n3)
+                    {
+                        hipLaunchKernelGGL((kb_cell<R, PASS, CAT_MORSE, EN, true>), g, b, ppb, st, mc);
+                        break;
+                    }
+                hipLaunchKernelGGL((kb_cell<R, PASS, CAT_MORSE, EN>), g, b, ppb, st, mc);
+                break;
             default: hipLaunchKernelGGL((kb_cell<R, PASS, CAT_GEN, EN>), g, b, ppb, st, mc); break;
         }
     }
+    /* each pair once: the partner sums folded into the forces (members without them pass) */
+    if constexpr (sizeof(R) == sizeof(float) && PASS == 0 && FMDX_N3 && FMDX_WAVES_PER_CELL == 1)
+        if (P.n3)
+        {
+            BatchMap ms = {P.descs, P.slot_items};
+            hipLaunchKernelGGL(kb_fold, dim3(P.n_slot), dim3(TPB), 0, st, ms);
+        }
 }
 
 /* every system of a batch: relative positions (mixed), the density pass over

@@ -84,3 +84,27 @@ def test_lds_table_variant_fits_one_workgroup_per_cu(tmp_path):
         assert get("vgpr_count") <= 128, name
         assert get("private_segment_fixed_size") <= 64, name
     assert found >= 3   # density pass, force pass with and without the pair energy
+
+
+def test_each_pair_once_variant_compiles(tmp_path):
+    """The FP32 LJ and Morse passes that evaluate each pair once (FMDX_N3, off by default:
+    measured slower than the two-sided passes) still compile, within the registers and LDS
+    that let 15 workgroups share a CU, and take their partner sums with LDS atomics."""
+    import re
+    import subprocess
+    out = str(tmp_path / "k_force_n3.s")
+    subprocess.run([B.HIPCC] + B.HIP_FLAGS + ["-DFMDX_N3=1", "--cuda-device-only", "-S",
+                    os.path.join(B.DEV_SRC, A.TUNED_FILE), "-o", out], check=True, capture_output=True)
+    text = open(out).read()
+    meta = text[text.index("amdhsa.kernels:"):]
+    found = 0
+    for blk in re.split(r"\n  - ", meta)[1:]:
+        name = re.search(r"\.name:\s+(\S+)", blk)
+        if not name or not re.search(r"(k|kb)_cellIfLi0ELi[23]ELb[01]ELb1E", name.group(1)):
+            continue
+        get = lambda k: int(re.search(r"\." + k + r":\s+(\d+)", blk).group(1))
+        found += 1
+        assert get("vgpr_count") <= 128, name.group(1)
+        assert get("group_segment_fixed_size") * 15 <= 160 * 1024, name.group(1)
+    assert found >= 8   # LJ and Morse, with and without the energy, single system and batch
+    assert "ds_add_f32" in text
