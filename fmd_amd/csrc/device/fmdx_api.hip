@@ -9,8 +9,42 @@
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
+#include <mutex>
+#include <utility>
 #include <vector>
 #include "fmdx_sys.h"
+
+/* Streams of destroyed systems are kept for the next system created on the same device:
+   creating and destroying a HIP stream cost about 0.2 and 0.45 ms, which a server that
+   builds and retires a system per job pays every job.  A stream enters the pool idle. */
+static std::mutex g_stream_mu;
+static std::vector<std::pair<int, hipStream_t>> g_streams;
+
+static hipError_t stream_take(int dev, hipStream_t *st)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_stream_mu);
+        for (size_t i = g_streams.size(); i-- > 0;)
+            if (g_streams[i].first == dev)
+            {
+                *st = g_streams[i].second;
+                g_streams.erase(g_streams.begin() + i);
+                return hipSuccess;
+            }
+    }
+    return hipStreamCreateWithFlags(st, hipStreamNonBlocking);
+}
+
+static void stream_give(int dev, hipStream_t st)
+{
+    if (hipStreamSynchronize(st) != hipSuccess)
+    {
+        hipStreamDestroy(st);
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_stream_mu);
+    g_streams.emplace_back(dev, st);
+}
 
 extern "C" const char *fmdx_strerror(int code)
 {
@@ -35,40 +69,66 @@ extern "C" int fmdx_device_count(int *n)
     return FMDX_OK;
 }
 
-/* per-slot arrays of a layout with capacity cap */
+/* per-slot arrays of a layout with capacity cap (ns = ncell * cap slots), mover staging
+   for nmov atoms and the per-block thermo partials, carved from D.slot_mem */
+static void slot_layout(KDesc &D, Carve &c, size_t ns, size_t nmov, size_t npart)
+{
+    c(&D.x, ns); c(&D.y, ns); c(&D.z, ns);
+    c(&D.vx, ns); c(&D.vy, ns); c(&D.vz, ns);
+    c(&D.fx, ns); c(&D.fy, ns); c(&D.fz, ns);
+    c(&D.aux, ns);
+    c(&D.xq, ns);
+    c(&D.grp, ns); c(&D.aid, ns); c(&D.kind, ns);
+    c(&D.mv, ns); c(&D.code, ns);
+    c(&D.bkey, ns); c(&D.bid, ns);
+    c(&D.mvx, nmov); c(&D.mvy, nmov); c(&D.mvz, nmov);
+    c(&D.mvvx, nmov); c(&D.mvvy, nmov); c(&D.mvvz, nmov);
+    c(&D.mvgrp, nmov); c(&D.mvaid, nmov); c(&D.mvkind, nmov);
+    c(&D.part, npart);
+}
+
 static void free_slots(KDesc &D)
 {
-    dfree(D.x); dfree(D.y); dfree(D.z); dfree(D.vx); dfree(D.vy); dfree(D.vz);
-    dfree(D.fx); dfree(D.fy); dfree(D.fz); dfree(D.aux);
-    dfree(D.xq);
     dfree(D.np);
-    dfree(D.grp); dfree(D.aid); dfree(D.kind);
-    dfree(D.mv); dfree(D.code); dfree(D.bkey); dfree(D.bid);
-    dfree(D.mvx); dfree(D.mvy); dfree(D.mvz); dfree(D.mvvx); dfree(D.mvvy); dfree(D.mvvz);
-    dfree(D.mvgrp); dfree(D.mvaid); dfree(D.mvkind);
-    dfree(D.part);
+    carve_free(&D.slot_mem, [&](Carve &c) { slot_layout(D, c, 0, 0, 0); });
 }
 
 static int alloc_slots(KDesc &D, int cap, unsigned nmov)
 {
-    size_t ns = (size_t)D.ncell * cap;
+    const size_t ns = (size_t)D.ncell * cap;
+    const size_t nblk = ((size_t)D.ncell + CPB - 1) / CPB + 1;
     D.cap = cap;
-    int r = 0;
-    r |= dalloc(&D.x, ns); r |= dalloc(&D.y, ns); r |= dalloc(&D.z, ns);
-    r |= dalloc(&D.vx, ns); r |= dalloc(&D.vy, ns); r |= dalloc(&D.vz, ns);
-    r |= dalloc(&D.fx, ns); r |= dalloc(&D.fy, ns); r |= dalloc(&D.fz, ns);
-    r |= dalloc(&D.aux, ns);
-    r |= dalloc(&D.xq, ns);
-    r |= dalloc(&D.grp, ns); r |= dalloc(&D.aid, ns); r |= dalloc(&D.kind, ns);
-    r |= dalloc(&D.mv, ns); r |= dalloc(&D.code, ns);
-    r |= dalloc(&D.bkey, ns); r |= dalloc(&D.bid, ns);
-    r |= dalloc(&D.mvx, nmov); r |= dalloc(&D.mvy, nmov); r |= dalloc(&D.mvz, nmov);
-    r |= dalloc(&D.mvvx, nmov); r |= dalloc(&D.mvvy, nmov); r |= dalloc(&D.mvvz, nmov);
-    r |= dalloc(&D.mvgrp, nmov); r |= dalloc(&D.mvaid, nmov); r |= dalloc(&D.mvkind, nmov);
-    size_t nblk = ((size_t)D.ncell + CPB - 1) / CPB + 1;
-    r |= dalloc(&D.part, nblk * 8);
-    if (r) { free_slots(D); return FMDX_ENOMEM; }
-    return FMDX_OK;
+    return carve_alloc(&D.slot_mem, [&](Carve &c) { slot_layout(D, c, ns, nmov, nblk * 8); });
+}
+
+/* per-cell arrays, counters, reduction partials and the pair table, carved from s->cell_mem */
+static void cell_layout(fmdx_system_t *s, Carve &c, size_t ncell)
+{
+    KDesc &D = s->D;
+    c(&D.cnt, ncell);
+    c(&D.nleave, ncell); c(&D.narr, ncell);
+    c(&D.nearly, ncell); c(&D.fill, ncell);
+    c(&D.counters, NCOUNTERS);
+    c(&D.cell_pe, ncell); c(&D.cell_emb, ncell);
+    c(&D.htrips, ncell);
+    c(&D.htile, ncell);
+    c(&D.epart, 2 * NPART_MAX);
+    c(&D.thermo, 16);
+    c(&s->dpairs, MAXK * MAXK);
+}
+
+/* the EAM tables, carved from s->tab_mem: nF knots of F, nR of rho, nP of phi (each in
+   FP64 and FP32), nRF of the r^2 tables and nK of the FP32 derivative knots */
+struct TabSizes
+{
+    size_t nF, nR, nP, nRF, nK;
+};
+
+static void tab_layout(fmdx_system_t *s, Carve &c, const TabSizes &z)
+{
+    c(&s->rf, z.nRF); c(&s->rf32, z.nRF); c(&s->kn32, z.nK);
+    c(&s->Fc, z.nF); c(&s->rhoc, z.nR); c(&s->phic, z.nP);
+    c(&s->Fc32, z.nF); c(&s->rhoc32, z.nR); c(&s->phic32, z.nP);
 }
 
 extern "C" int fmdx_system_create(fmdx_system_t **out, const double l[3], const int pbc[3], const int nc[3],
@@ -86,7 +146,7 @@ extern "C" int fmdx_system_create(fmdx_system_t **out, const double l[3], const 
     fmdx_system_t *s = (fmdx_system_t *)calloc(1, sizeof(fmdx_system_t));
     if (!s) return FMDX_ENOMEM;
     CK(hipGetDevice(&s->dev));
-    if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) { free(s); return FMDX_EHIP; }
+    if (stream_take(s->dev, &s->st) != hipSuccess) { free(s); return FMDX_EHIP; }
     s->precision = precision;
     s->D.xq_keep = precision == FMDX_PREC_MIXED;
     s->xq_fresh = 0;
@@ -105,18 +165,7 @@ extern "C" int fmdx_system_create(fmdx_system_t **out, const double l[3], const 
         s->mass[k] = D.mass[k];
         D.elem[k] = -1;
     }
-    int r = 0;
-    r |= dalloc(&D.cnt, ncell);
-    r |= dalloc(&D.nleave, ncell); r |= dalloc(&D.narr, ncell);
-    r |= dalloc(&D.nearly, ncell); r |= dalloc(&D.fill, ncell);
-    r |= dalloc(&D.counters, NCOUNTERS);
-    r |= dalloc(&D.cell_pe, ncell); r |= dalloc(&D.cell_emb, ncell);
-    r |= dalloc(&D.htrips, ncell);
-    r |= dalloc(&D.htile, ncell);
-    r |= dalloc(&D.epart, 2 * NPART_MAX);
-    r |= dalloc(&D.thermo, 16);
-    r |= dalloc(&s->dpairs, MAXK * MAXK);
-    if (r) { fmdx_system_destroy(s); return FMDX_ENOMEM; }
+    if (carve_alloc(&s->cell_mem, [&](Carve &c) { cell_layout(s, c, ncell); })) { fmdx_system_destroy(s); return FMDX_ENOMEM; }
     hipMemsetAsync(D.cnt, 0, sizeof(int) * ncell, s->st);
     hipMemsetAsync(D.nleave, 0, sizeof(int) * ncell, s->st);
     hipMemsetAsync(D.narr, 0, sizeof(int) * ncell, s->st);
@@ -145,18 +194,14 @@ extern "C" int fmdx_system_destroy(fmdx_system_t *s)
     if (s->st) hipStreamSynchronize(s->st);
     KDesc &D = s->D;
     free_slots(D);
-    dfree(D.cnt); dfree(D.nleave); dfree(D.narr); dfree(D.nearly); dfree(D.fill);
-    dfree(D.counters); dfree(D.cell_pe); dfree(D.cell_emb); dfree(D.thermo);
-    dfree(D.hbits); dfree(D.htrips); dfree(D.htile); dfree(D.epart);
-    dfree(s->dpairs);
-    dfree(s->Fc); dfree(s->rhoc); dfree(s->phic);
-    dfree(s->Fc32); dfree(s->rhoc32); dfree(s->phic32);
-    dfree(s->rf); dfree(s->rf32); dfree(s->kn32);
+    carve_free(&s->cell_mem, [&](Carve &c) { cell_layout(s, c, 0); });
+    dfree(D.hbits);
+    carve_free(&s->tab_mem, [&](Carve &c) { tab_layout(s, c, TabSizes{}); });
     for (int i = 0; i < s->nttm_mem; i++) hipFree(s->ttm_mem[i]);
     for (int i = 0; i < s->nev; i++) hipEventDestroy(s->ev[i]);
     for (int i = 0; i < NPROF * 3; i++)
         if (s->pev[i]) hipEventDestroy(s->pev[i]);
-    if (s->st) hipStreamDestroy(s->st);
+    if (s->st) stream_give(s->dev, s->st);
     free(s);
     return FMDX_OK;
 }
@@ -316,13 +361,9 @@ extern "C" int fmdx_set_eam(fmdx_system_t *s, int nelem, int nrho, double drho, 
     }
     /* tables of an earlier call may still be read by work queued on the stream */
     CK(hipStreamSynchronize(s->st));
-    dfree(s->Fc); dfree(s->rhoc); dfree(s->phic); dfree(s->Fc32); dfree(s->rhoc32); dfree(s->phic32);
-    dfree(s->rf); dfree(s->rf32); dfree(s->kn32);
-    int r = 0;
-    r |= dalloc(&s->rf, hRF.size()); r |= dalloc(&s->rf32, fRF.size()); r |= dalloc(&s->kn32, fK.size());
-    r |= dalloc(&s->Fc, hF.size()); r |= dalloc(&s->rhoc, hR.size()); r |= dalloc(&s->phic, hP.size());
-    r |= dalloc(&s->Fc32, fF.size()); r |= dalloc(&s->rhoc32, fR.size()); r |= dalloc(&s->phic32, fP.size());
-    if (r) return FMDX_ENOMEM;
+    const TabSizes tz{hF.size(), hR.size(), hP.size(), hRF.size(), fK.size()};
+    carve_free(&s->tab_mem, [&](Carve &c) { tab_layout(s, c, tz); });
+    if (carve_alloc(&s->tab_mem, [&](Carve &c) { tab_layout(s, c, tz); })) return FMDX_ENOMEM;
     CK(hipMemcpyAsync(s->Fc, hF.data(), hF.size() * sizeof(double4), hipMemcpyHostToDevice, s->st));
     CK(hipMemcpyAsync(s->rhoc, hR.data(), hR.size() * sizeof(double4), hipMemcpyHostToDevice, s->st));
     CK(hipMemcpyAsync(s->phic, hP.data(), hP.size() * sizeof(double4), hipMemcpyHostToDevice, s->st));
@@ -406,16 +447,18 @@ extern "C" int fmdx_upload(fmdx_system_t *s, unsigned natoms, const unsigned *ce
     free_slots(D);
     int r = alloc_slots(D, cap, natoms + 64);
     if (r) return r;
-    int *dstart = nullptr;
-    double *dx = nullptr, *dv = nullptr;
-    int *dg = nullptr;
-    unsigned *da = nullptr;
-    uint8_t *dk = nullptr;
-    r = 0;
-    r |= dalloc(&dstart, D.ncell + 1);
-    r |= dalloc(&dx, 3 * (size_t)natoms); r |= dalloc(&dv, 3 * (size_t)natoms);
-    r |= dalloc(&dg, natoms); r |= dalloc(&da, natoms); r |= dalloc(&dk, natoms);
-    if (r) return FMDX_ENOMEM;
+    int *dstart;
+    double *dx, *dv;
+    int *dg;
+    unsigned *da;
+    uint8_t *dk;
+    StageMem stage;
+    auto stage_layout = [&](Carve &c) {
+        c(&dstart, D.ncell + 1);
+        c(&dx, 3 * (size_t)natoms); c(&dv, 3 * (size_t)natoms);
+        c(&dg, natoms); c(&da, natoms); c(&dk, natoms);
+    };
+    if (carve_alloc(&stage.p, stage_layout)) return FMDX_ENOMEM;
     std::vector<int> g(natoms ? natoms : 1, 0);
     std::vector<unsigned> aid(natoms ? natoms : 1, 0);
     std::vector<double> vz(3 * (size_t)(natoms ? natoms : 1), 0.0);
@@ -445,7 +488,6 @@ extern "C" int fmdx_upload(fmdx_system_t *s, unsigned natoms, const unsigned *ce
     unsigned ctr[NCOUNTERS] = {0, 0, maxc};
     CK(hipMemcpyAsync(D.counters, ctr, sizeof ctr, hipMemcpyHostToDevice, s->st));
     CK(hipStreamSynchronize(s->st));
-    dfree(dstart); dfree(dx); dfree(dv); dfree(dg); dfree(da); dfree(dk);
     s->natoms_uploaded = natoms;
     s->uploaded = 1;
     return ensure_n3(s, s->precision);
@@ -484,16 +526,18 @@ extern "C" int fmdx_download(fmdx_system_t *s, unsigned *natoms, unsigned *cell_
     if (natoms) *natoms = n;
     if (cell_count) memcpy(cell_count, cnt.data(), sizeof(unsigned) * D.ncell);
     if (!x && !v && !f && !aux && !group && !atomid && !kind) return FMDX_OK;
-    int *dstart = nullptr;
-    double *dx = nullptr, *dv = nullptr, *df = nullptr, *daux = nullptr;
-    int *dg = nullptr;
-    unsigned *da = nullptr;
-    uint8_t *dk = nullptr;
-    int r = 0;
-    r |= dalloc(&dstart, D.ncell + 1);
-    r |= dalloc(&dx, 3 * (size_t)n); r |= dalloc(&dv, 3 * (size_t)n); r |= dalloc(&df, 3 * (size_t)n);
-    r |= dalloc(&daux, n); r |= dalloc(&dg, n); r |= dalloc(&da, n); r |= dalloc(&dk, n);
-    if (r) return FMDX_ENOMEM;
+    int *dstart;
+    double *dx, *dv, *df, *daux;
+    int *dg;
+    unsigned *da;
+    uint8_t *dk;
+    StageMem stage;
+    auto stage_layout = [&](Carve &c) {
+        c(&dstart, D.ncell + 1);
+        c(&dx, 3 * (size_t)n); c(&dv, 3 * (size_t)n); c(&df, 3 * (size_t)n);
+        c(&daux, n); c(&dg, n); c(&da, n); c(&dk, n);
+    };
+    if (carve_alloc(&stage.p, stage_layout)) return FMDX_ENOMEM;
     CK(hipMemcpyAsync(dstart, start.data(), sizeof(int) * (D.ncell + 1), hipMemcpyHostToDevice, s->st));
     CK(launch_gather(D, s->st, dstart, dx, dv, df, daux, dg, da, dk));
     if (n)
@@ -510,7 +554,6 @@ extern "C" int fmdx_download(fmdx_system_t *s, unsigned *natoms, unsigned *cell_
     CK(hipStreamSynchronize(s->st));
     if (kind)
         for (unsigned i = 0; i < n; i++) kind[i] = k8[i];
-    dfree(dstart); dfree(dx); dfree(dv); dfree(df); dfree(daux); dfree(dg); dfree(da); dfree(dk);
     return FMDX_OK;
 }
 
@@ -710,13 +753,7 @@ extern "C" int fmdx_set_capacity(fmdx_system_t *s, unsigned cap)
     CK(hipStreamSynchronize(s->st));
     if (cap < ctr[2]) return FMDX_EARG;
     KDesc N = s->D;
-    N.x = N.y = N.z = N.vx = N.vy = N.vz = N.fx = N.fy = N.fz = N.aux = nullptr;
-    N.xq = nullptr;
-    N.np = nullptr;   /* the partner sums of the new layout are allocated at the next force call */
-    N.grp = nullptr; N.aid = nullptr; N.kind = nullptr; N.mv = nullptr; N.code = nullptr;
-    N.bkey = nullptr; N.bid = nullptr;
-    N.mvx = N.mvy = N.mvz = N.mvvx = N.mvvy = N.mvvz = nullptr;
-    N.mvgrp = nullptr; N.mvaid = nullptr; N.mvkind = nullptr; N.part = nullptr;
+    N.np = nullptr;   /* the partner sums of the new layout are allocated below */
     int r = alloc_slots(N, (int)cap, s->natoms_uploaded + 64);
     if (r) return r;
     const long nslot = (long)N.ncell * N.cap;

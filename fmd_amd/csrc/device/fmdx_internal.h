@@ -86,6 +86,7 @@ struct KDesc
     float *np;    /* mixed build, single-kind LJ or Morse, each pair once: partner sums
                      [3][13][ncell*cap] (x, y, z; forward direction; slot), or null */
     int xq_keep;  /* mixed build: the integrators write xq and the re-binning carries it along */
+    char *slot_mem;   /* the one allocation the per-slot arrays, mover staging and part are carved from */
     int *grp;
     unsigned *aid;
     uint8_t *kind;

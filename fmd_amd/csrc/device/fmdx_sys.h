@@ -22,9 +22,11 @@ struct fmdx_system
     double mass[MAXK];
     PairParam hp[MAXK * MAXK];
     PairParam *dpairs;
+    char *cell_mem;        /* per-cell arrays, counters, reductions and dpairs: one allocation */
     /* EAM */
     double4 *Fc, *rhoc, *phic, *rf;
     float4 *Fc32, *rhoc32, *phic32, *rf32, *kn32;
+    char *tab_mem;         /* the nine EAM tables: one allocation */
     /* TTM */
     int ttm_on;
     TtmState T;
@@ -59,6 +61,57 @@ static inline void dfree(T *&p)
 {
     if (p) hipFree((void *)p);
     p = nullptr;
+}
+
+/* Arrays that are allocated and released together are carved out of one device
+   allocation, each on a 256-byte boundary: hipFree waits for the device and costs about
+   0.1 ms, so a system's ~45 arrays took ~5 ms to release one by one.  A group is laid
+   out by one function called twice with the same sizes: first over no base, which only
+   sums the sizes (and leaves every pointer null), then over the allocation. */
+struct Carve
+{
+    char *base;
+    size_t off;
+    explicit Carve(char *b = nullptr) : base(b), off(0) {}
+    template <class T>
+    void operator()(T **p, size_t n)
+    {
+        if (n == 0) n = 1;
+        *p = base ? (T *)(base + off) : nullptr;
+        off += (n * sizeof(T) + 255) & ~(size_t)255;
+    }
+};
+
+template <class F>
+static inline int carve_alloc(char **mem, F layout)
+{
+    Carve sizing;
+    layout(sizing);
+    *mem = nullptr;
+    if (hipMalloc((void **)mem, sizing.off) != hipSuccess)
+    {
+        *mem = nullptr;
+        return FMDX_ENOMEM;
+    }
+    Carve c(*mem);
+    layout(c);
+    return FMDX_OK;
+}
+
+/* a staging group released when the call returns, error returns included */
+struct StageMem
+{
+    char *p = nullptr;
+    ~StageMem() { dfree(p); }
+};
+
+/* releases a group and nulls its arrays */
+template <class F>
+static inline void carve_free(char **mem, F layout)
+{
+    dfree(*mem);
+    Carve none;
+    layout(none);
 }
 
 /* step parameters of a system; md->timestep * 0.5 / mass (integrators.c:67)

@@ -6,7 +6,9 @@ one batch (libfmdx.so's batch engine) and, between step quanta:
 
 * admits arrived jobs in arrival order while the device-memory budget and
   the member limit allow (admission by footprint: an estimate from the atom
-  count before the job is built, its measured footprint afterwards);
+  count before the job is built, its measured footprint afterwards), builds
+  them on a builder thread while the batch steps, and adds each built job to
+  the batch at the next quantum boundary;
 * advances all members together by a quantum of steps that never passes
   the end of any member's step count;
 * retires members that reached their step count, and members whose device
@@ -23,6 +25,7 @@ first come first served.
 import collections
 import ctypes as C
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -50,9 +53,35 @@ class Result:
         self.x = None
 
 
+class _Built:
+    """a build that already ran (the loop without a builder thread)"""
+
+    def __init__(self, fn, *a):
+        self.job, self.exc = None, None
+        try:
+            self.job = fn(*a)
+        except BaseException as e:   # re-raised by result(), where a future would raise it
+            self.exc = e
+
+    def done(self):
+        return True
+
+    def result(self):
+        if self.exc is not None:
+            raise self.exc
+        return self.job
+
+
 class Server:
+    """`builders` threads build arrived jobs (atoms, device system, upload) while the GPU runs
+    the batch's steps: ctypes releases the GIL inside the library's calls, and a quantum of
+    steps waits on the device most of its time.  A job joins the batch at the first quantum
+    boundary after its build has finished, in arrival order.  The library works on the
+    calling thread's current device; with one process per GPU that is device 0 in every
+    thread.  builders=0 builds at the quantum boundary on the loop's own thread."""
+
     def __init__(self, build, precision=FMDX_PREC_MIXED, budget_bytes=16 << 30, max_jobs=512, quantum=25,
-                 keep_positions=False, check_interval=8):
+                 keep_positions=False, check_interval=8, builders=1):
         self.build = build
         self.precision = precision
         self.budget = budget_bytes
@@ -60,60 +89,72 @@ class Server:
         self.quantum = quantum
         self.keep_positions = keep_positions
         self.check_interval = check_interval
+        self.builders = builders
 
     def run(self, specs, clock=time.perf_counter, sleep=time.sleep):
         """specs: dicts with 'index', 'arrival' (s from the start) and 'nsteps'; returns Results in index order"""
         b = Batch(self.precision)
         b.set_check_interval(self.check_interval)
+        pool = ThreadPoolExecutor(max_workers=self.builders) if self.builders > 0 else None
         pending = collections.deque(sorted(specs, key=lambda s: s["arrival"]))
+        building = collections.deque()   # [spec, future, estimated bytes]
         active = []      # [job, remaining steps, Result, bytes]
         results = {}
-        used = 0
+        used = 0         # measured bytes of the members plus the estimates of the jobs in build
         t0 = clock()
-        while pending or active:
-            now = clock() - t0
-            while pending and pending[0]["arrival"] <= now and len(active) < self.max_jobs:
-                sp = pending[0]
-                est = self._estimate(sp)
-                if active and used + est > self.budget:
-                    break
-                pending.popleft()
-                r = Result(sp["index"], sp["arrival"])
-                job = self.build(sp, self.precision)
-                b.add(job.dev, job.dt)
-                r.admit = clock() - t0
-                r.natoms, r.nsteps = job.natoms, job.nsteps
-                nbytes = memory_of(job.dev)
-                used += nbytes
-                active.append([job, job.nsteps, r, nbytes])
+        try:
+            while pending or building or active:
                 now = clock() - t0
-            if not active:
-                if pending:
-                    sleep(max(0.0, pending[0]["arrival"] - (clock() - t0)))
-                continue
-            q = min(self.quantum, min(a[1] for a in active))
-            if q > 0:
-                b.step(q)
-                for a in active:
-                    a[1] -= q
-            st = b.status()
-            done = [i for i, a in enumerate(active) if a[1] == 0 or st[i].err]
-            t_fin = clock() - t0
-            for i in reversed(done):
-                job, rem, r, nbytes = active[i]
-                t = st[i]
-                r.finish = t_fin
-                r.latency = t_fin - r.arrival
-                r.natoms_end, r.temperature, r.pe, r.err = t.natoms, t.temperature, t.pe, t.err
-                r.failed = bool(t.err)
-                if self.keep_positions:
-                    r.x = job.f.atoms()
-                b.remove(job.dev)
-                job.free()
-                used -= nbytes
-                results[r.index] = r
-                del active[i]
-        b.close()
+                while pending and pending[0]["arrival"] <= now and len(active) + len(building) < self.max_jobs:
+                    sp = pending[0]
+                    est = self._estimate(sp)
+                    if (active or building) and used + est > self.budget:
+                        break
+                    pending.popleft()
+                    fut = pool.submit(self.build, sp, self.precision) if pool else _Built(self.build, sp, self.precision)
+                    building.append([sp, fut, est])
+                    used += est
+                # builds join in arrival order; with nothing to step, wait for the next one
+                while building and (building[0][1].done() or not active):
+                    sp, fut, est = building.popleft()
+                    job = fut.result()
+                    r = Result(sp["index"], sp["arrival"])
+                    b.add(job.dev, job.dt)
+                    r.admit = clock() - t0
+                    r.natoms, r.nsteps = job.natoms, job.nsteps
+                    nbytes = memory_of(job.dev)
+                    used += nbytes - est
+                    active.append([job, job.nsteps, r, nbytes])
+                if not active:
+                    if pending and not building:
+                        sleep(max(0.0, pending[0]["arrival"] - (clock() - t0)))
+                    continue
+                q = min(self.quantum, min(a[1] for a in active))
+                if q > 0:
+                    b.step(q)
+                    for a in active:
+                        a[1] -= q
+                st = b.status()
+                done = [i for i, a in enumerate(active) if a[1] == 0 or st[i].err]
+                t_fin = clock() - t0
+                for i in reversed(done):
+                    job, rem, r, nbytes = active[i]
+                    t = st[i]
+                    r.finish = t_fin
+                    r.latency = t_fin - r.arrival
+                    r.natoms_end, r.temperature, r.pe, r.err = t.natoms, t.temperature, t.pe, t.err
+                    r.failed = bool(t.err)
+                    if self.keep_positions:
+                        r.x = job.f.atoms()
+                    b.remove(job.dev)
+                    job.free()
+                    used -= nbytes
+                    results[r.index] = r
+                    del active[i]
+        finally:
+            if pool:
+                pool.shutdown(wait=True)
+            b.close()
         return [results[k] for k in sorted(results)]
 
     def _estimate(self, sp):
