@@ -24,6 +24,7 @@ FMDX_NOALLOC_MAX_CELLS = 590000
 FMDX_FORCES_ENERGY = 1
 FMDX_PREC_FP64 = 0
 FMDX_PREC_MIXED = 1
+FMDX_COMPACT_OFF, FMDX_COMPACT_AUTO, FMDX_COMPACT_TIGHT = 0, 1, 2
 POT_LJ, POT_EAM, POT_MORSE = 1, 2, 3
 MASS_UNIT = 9.6485332907e+03
 KB = 8.6173303e-5
@@ -125,6 +126,7 @@ def load_fmdx():
         lib.fmdx_batch_size.argtypes = [P, C.POINTER(C.c_int)]
         lib.fmdx_batch_use_graph.argtypes = [P, C.c_int]
         lib.fmdx_batch_set_check_interval.argtypes = [P, C.c_int]
+        lib.fmdx_batch_set_compaction.argtypes = [P, C.c_int]
         lib.fmdx_batch_step.argtypes = [P, C.c_int]
         lib.fmdx_batch_status.argtypes = [P, C.POINTER(Thermo)]
         lib.fmdx_batch_sync.argtypes = [P]
@@ -343,6 +345,10 @@ class Batch:
 
     def set_check_interval(self, steps):
         _ck(self.lib.fmdx_batch_set_check_interval(self.h, steps), "fmdx_batch_set_check_interval")
+
+    def set_compaction(self, mode):
+        """FMDX_COMPACT_OFF, FMDX_COMPACT_AUTO (default) or FMDX_COMPACT_TIGHT"""
+        _ck(self.lib.fmdx_batch_set_compaction(self.h, mode), "fmdx_batch_set_compaction")
 
     def step(self, n=1):
         _ck(self.lib.fmdx_batch_step(self.h, n), "fmdx_batch_step")

@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <vector>
 #include "fmdx_sys.h"
+#include "fmdx_tuned.h"
 
 struct fmdx_batch
 {
@@ -36,11 +37,16 @@ struct fmdx_batch
     int descs_cap;
     int2 *items;
     size_t items_cap;
+    char *occ_mem;             /* compacted lists: lengths, member categories, the lists */
+    size_t occ_cap;            /* list entries */
+    int occ_sys_cap;
     unsigned *dstatus;
     double *dthermo;
     int status_cap;
     std::vector<unsigned> hstatus;
     std::vector<double> hthermo;
+    int hocc[16];              /* lengths of the compacted lists, as last read */
+    int compact_mode;          /* FMDX_COMPACT_* */
     BatchPlan plan;
     int dirty;
     int use_graph;
@@ -90,6 +96,49 @@ static int upload_descs(fmdx_batch_t *b)
     /* the copy reads h, which ends with this call */
     CK(hipStreamSynchronize(b->st));
     return FMDX_OK;
+}
+
+/* Whether the members' occupied cells, as last counted, are few enough for the compacted
+   lists to pay: below 60 % of their cells.  Fuller batches take the full lists, whose
+   launches skip the list lengths and the compaction itself. */
+static bool sparse(const fmdx_batch_t *b)
+{
+    const BatchPlan &P = b->plan;
+    long occ = 0, all = 0;
+    for (int c = 0; c < 4; c++)
+    {
+        occ += b->hocc[1 + c];
+        all += P.n_cell0[c];
+    }
+    return occ * 10 < all * 6;
+}
+
+/* Grids of the compacted lists from their lengths: room for a quarter more (and 64
+   items; none when tight), at most one workgroup per item of the full list.  A grid is resized when its
+   list outgrows it or shrinks below half of it; returns whether any grid changed (the
+   captured step then no longer matches). */
+static int fit_grids(fmdx_batch_t *b, int force)
+{
+    BatchPlan &P = b->plan;
+    if (!P.compact) return 0;
+    const int tight = b->compact_mode == FMDX_COMPACT_TIGHT;
+    auto fit = [&](int &g, int n, int full) {
+        const int cap = std::max(8, (full + 7) / 8 * 8);
+        const int room = tight ? 0 : n / 4 + 64;
+        const int want = std::min(cap, std::max(8, (n + room + 7) / 8 * 8));
+        if (force || n > g || (want < g / 2))
+        {
+            if (g != want) { g = want; return 1; }
+        }
+        return 0;
+    };
+    int changed = fit(P.g_slot, b->hocc[0], P.n_slot);
+    for (int c = 0; c < 4; c++)
+    {
+        changed |= fit(P.g_cell0[c], b->hocc[1 + c], P.n_cell0[c]);
+        changed |= fit(P.g_cell1[c], b->hocc[5 + c], P.n_cell1[c]);
+    }
+    return changed;
 }
 
 static int rebuild(fmdx_batch_t *b)
@@ -149,6 +198,42 @@ static int rebuild(fmdx_batch_t *b)
         all.insert(all.end(), lists[k]->begin(), lists[k]->end());
     }
     if (total) CK(hipMemcpyAsync(b->items, all.data(), sizeof(int2) * total, hipMemcpyHostToDevice, b->st));
+    /* the compacted lists hold at most every slot group and cell of the full ones */
+    BatchPlan &P = b->plan;
+    P.compact = FMDX_COMPACT && n > 0 && b->compact_mode != FMDX_COMPACT_OFF;
+    size_t nocc = slot.size();
+    for (int c = 0; c < 4; c++) nocc += c0[c].size() + c1[c].size();
+    auto occ_layout = [&](Carve &cv, size_t cap, size_t nsys) {
+        cv(&P.occ_n, 16);
+        cv((int **)&P.sys_cat, nsys);
+        cv(&P.occ_slot, cap);   /* the whole list area */
+    };
+    if (P.compact && (nocc > b->occ_cap || n > b->occ_sys_cap))
+    {
+        const size_t cap = std::max(nocc, 2 * b->occ_cap);
+        const int scap = std::max(n, 2 * b->occ_sys_cap);
+        dfree(b->occ_mem);
+        b->occ_cap = 0;
+        b->occ_sys_cap = 0;
+        if (carve_alloc(&b->occ_mem, [&](Carve &cv) { occ_layout(cv, cap, scap); })) return FMDX_ENOMEM;
+        b->occ_cap = cap;
+        b->occ_sys_cap = scap;
+    }
+    if (P.compact)
+    {
+        /* one carve of the whole list area: slot groups first, then each category's lists */
+        int2 *at = P.occ_slot + slot.size();
+        for (int c = 0; c < 4; c++)
+        {
+            P.occ_cell0[c] = at;
+            at += c0[c].size();
+            P.occ_cell1[c] = at;
+            at += c1[c].size();
+        }
+        std::vector<int> cats(n);
+        for (int i = 0; i < n; i++) cats[i] = force_category(b->sys[i]->D);
+        CK(hipMemcpyAsync((void *)P.sys_cat, cats.data(), sizeof(int) * n, hipMemcpyHostToDevice, b->st));
+    }
     /* members whose force passes evaluate each pair once need their partner sums */
     for (int i = 0; i < n; i++)
     {
@@ -157,7 +242,6 @@ static int rebuild(fmdx_batch_t *b)
     }
     int r = upload_descs(b);
     if (r) return r;
-    BatchPlan &P = b->plan;
     P.descs = b->descs;
     P.nsys = n;
     P.slot_items = ptr[0]; P.n_slot = (int)slot.size();
@@ -172,7 +256,12 @@ static int rebuild(fmdx_batch_t *b)
     P.n3 = b->precision == FMDX_PREC_MIXED;   /* every LJ and Morse member holds partner sums */
     b->hstatus.assign((size_t)std::max(n, 1) * 8, 0);
     b->hthermo.assign((size_t)std::max(n, 1) * 16, 0.0);
+    /* the first step's start and kick walk the lists of the members' current cells */
+    CK(launchb_occ(P, b->st));
+    if (P.compact) CK(hipMemcpyAsync(b->hocc, P.occ_n, sizeof(int) * 9, hipMemcpyDeviceToHost, b->st));
     CK(hipStreamSynchronize(b->st));
+    if (b->compact_mode == FMDX_COMPACT_AUTO) P.compact = P.compact && sparse(b);
+    fit_grids(b, 1);
     b->dirty = 0;
     return FMDX_OK;
 }
@@ -232,7 +321,16 @@ static int collect(fmdx_batch_t *b)
     CK(launchb_collect(b->plan, b->st, b->dstatus, b->dthermo));
     CK(hipMemcpyAsync(b->hstatus.data(), b->dstatus, sizeof(unsigned) * 8 * n, hipMemcpyDeviceToHost, b->st));
     CK(hipMemcpyAsync(b->hthermo.data(), b->dthermo, sizeof(double) * 16 * n, hipMemcpyDeviceToHost, b->st));
+    if (b->plan.compact)
+        CK(hipMemcpyAsync(b->hocc, b->plan.occ_n, sizeof(int) * 9, hipMemcpyDeviceToHost, b->st));
     CK(hipStreamSynchronize(b->st));
+    if (b->plan.compact && b->compact_mode == FMDX_COMPACT_AUTO && !sparse(b))
+    {
+        b->plan.compact = 0;   /* the members filled their cells: the full lists from here on */
+        graph_drop(b);
+    }
+    else if (fit_grids(b, 0))
+        graph_drop(b);   /* captured again before the next step */
     return FMDX_OK;
 }
 
@@ -284,6 +382,7 @@ extern "C" int fmdx_batch_create(fmdx_batch_t **out, int precision)
     b->precision = precision;
     b->use_graph = 1;
     b->check_every = 8;
+    b->compact_mode = FMDX_COMPACT_AUTO;
     b->dirty = 1;
     *out = b;
     return FMDX_OK;
@@ -297,6 +396,7 @@ extern "C" void fmdx_batch_destroy(fmdx_batch_t *b)
     graph_drop(b);
     dfree(b->descs);
     dfree(b->items);
+    dfree(b->occ_mem);
     dfree(b->dstatus);
     dfree(b->dthermo);
     hipStreamDestroy(b->st);
@@ -363,6 +463,17 @@ extern "C" int fmdx_batch_set_check_interval(fmdx_batch_t *b, int steps)
     return FMDX_OK;
 }
 
+extern "C" int fmdx_batch_set_compaction(fmdx_batch_t *b, int mode)
+{
+    if (!b || mode < FMDX_COMPACT_OFF || mode > FMDX_COMPACT_TIGHT) return FMDX_EARG;
+    if (mode != b->compact_mode)
+    {
+        b->compact_mode = mode;
+        b->dirty = 1;   /* lists, grids and the captured step are laid out again */
+    }
+    return FMDX_OK;
+}
+
 /* Velocity-Verlet steps of every member (VelocityVerlet_takeOneStep,
    integrators.c:181-211, with each member's own dt and thermostat). */
 extern "C" int fmdx_batch_step(fmdx_batch_t *b, int nsteps)
@@ -371,11 +482,11 @@ extern "C" int fmdx_batch_step(fmdx_batch_t *b, int nsteps)
     int r;
     if (b->dirty && (r = rebuild(b))) return r;
     if (b->sys.empty()) return FMDX_OK;
-    if (b->use_graph && !b->graph_ok && (r = capture(b))) return r;
     for (int k = 0; k < nsteps; k++)
     {
         /* the steps after which the status is read sum the pair energies */
         const int energy = (k + 1) % b->check_every == 0 || k + 1 == nsteps;
+        if (b->use_graph && !b->graph_ok && (r = capture(b))) return r;
         if (b->use_graph) CK(hipGraphLaunch(b->gexec[energy], b->st));
         else CK(enqueue_step(b, energy));
         if (energy)

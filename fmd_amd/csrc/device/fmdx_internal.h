@@ -146,12 +146,51 @@ static __device__ __forceinline__ float4 rel_pos(const KDesc &D, int cx, int cy,
     return make_float4((float)(x - ox), (float)(y - oy), (float)(z - oz), 0.0f);
 }
 
-/* many systems in one launch: block b works on block items[b].y of system items[b].x */
+/* many systems in one launch: block b works on block items[b].y of system items[b].x.
+   With `count` set, the list was compacted on the device and holds *count items (see
+   each_item); `skip` items at its head are another launch's. */
 struct BatchMap
 {
     const KDesc *descs;
     const int2 *items;
+    const int *count;
+    int skip;
 };
+
+/* f(descriptor, block) for the items of the workgroup.  A full list: the workgroup's own
+   item.  A compacted list: the items from `skip` on are split into one contiguous run per
+   XCD (workgroup b runs on XCD b % 8, so that each XCD's L2 holds the neighbours of
+   nearby cells); LOOP: the workgroup takes every (gridDim/8)-th item of its XCD's run;
+   otherwise it takes at most one, and the items beyond gridDim are left to a LOOP launch
+   with skip = gridDim. */
+template <bool LOOP, class F>
+static __device__ __forceinline__ void each_item(const BatchMap &M, F f)
+{
+    int base = (int)blockIdx.x, first = 0, len = 1, step = 1;
+    if (M.count)
+    {
+        int n = *M.count - M.skip;
+        if (!LOOP) n = n < (int)gridDim.x ? n : (int)gridDim.x;
+        const int x = (int)(blockIdx.x & 7), q = n >> 3, r = n & 7;
+        base = M.skip + x * q + (x < r ? x : r);
+        len = q + (x < r ? 1 : 0);
+        first = (int)(blockIdx.x >> 3);
+        step = (int)(gridDim.x >> 3);
+    }
+    if (!LOOP)
+    {
+        if (first >= len) return;
+        const int2 it = M.items[base + first];
+        f(M.descs[it.x], it.y);
+        return;
+    }
+    for (int i = first; i < len; i += step)
+    {
+        const int2 it = M.items[base + i];
+        f(M.descs[it.x], it.y);
+        __syncthreads();   /* the next item reuses the workgroup's LDS */
+    }
+}
 
 /* TTM state resident on the device: n = ncx*ncy*(ncz+2) turi cells, index
    (ix*ncy + iy)*(ncz+2) + iz, ghosts in z only; dim3 selects the grid solver */
@@ -229,6 +268,18 @@ struct BatchPlan
     int n_cell1[4];
     int max_nkinds;
     int n3;                      /* the LJ and Morse members evaluate each pair once (all have partner sums) */
+    /* the step over occupied cells only: after each re-binning kb_occ writes the lists below
+       and their lengths occ_n[0] (slot groups), [1 + cat] (force pass), [5 + cat] (density pass) */
+    int compact;
+    /* grids that walk the compacted lists (multiples of 8), sized by the host from the list
+       lengths it last read, with room to grow; a list that outgrows its grid is still walked
+       whole, some workgroups taking more than one item */
+    int g_slot, g_cell0[4], g_cell1[4];
+    const int *sys_cat;          /* [nsys] pair category of each member's force pass */
+    int *occ_n;                  /* [9] */
+    int2 *occ_slot;              /* capacity n_slot */
+    int2 *occ_cell0[4];          /* capacity n_cell0[cat] */
+    int2 *occ_cell1[4];          /* capacity n_cell1[cat] */
 };
 
 /* kernel launchers (defined in the kernel translation units); the step
@@ -254,6 +305,8 @@ hipError_t launchb_forces(const BatchPlan &P, hipStream_t st, int precision, int
 /* each pair evaluated once (partner sums, then a fold) for this system in this precision */
 bool n3_wanted(const KDesc &D, int precision);
 hipError_t launchb_collect(const BatchPlan &P, hipStream_t st, unsigned *status, double *thermo);
+/* compact: the occupied-cell lists from the members' current cell counts */
+hipError_t launchb_occ(const BatchPlan &P, hipStream_t st);
 hipError_t launch_add_velocity(const KDesc &D, hipStream_t st, int group, double vx, double vy, double vz);
 hipError_t launch_relayout(const KDesc &Dold, const KDesc &Dnew, hipStream_t st);
 hipError_t launch_scatter(const KDesc &D, hipStream_t st, const int *cell_start, const double *x, const double *v,

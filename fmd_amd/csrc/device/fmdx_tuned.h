@@ -72,4 +72,9 @@
 #ifndef FMDX_TILE32_N3
 #define FMDX_TILE32_N3 256
 #endif
+/* batch steps over the occupied cells only (lists compacted on the device after each
+   re-binning) */
+#ifndef FMDX_COMPACT
+#define FMDX_COMPACT 1
+#endif
 #endif

@@ -928,7 +928,7 @@ static __host__ __device__ __forceinline__ unsigned lt_offset(int nkinds)
    in its workgroup, each with its own tile, and the table follows the pair
    parameters in the dynamic LDS. */
 template <typename R, int PASS, int CAT, bool EN, bool LT = false, bool N3 = false>
-static __device__ void b_cell(const KDesc &D, const int c)
+static __device__ __forceinline__ void b_cell(const KDesc &D, const int c)
 {
     typedef typename AccOf<R>::T Acc;
     typedef Prec<R> P;
@@ -1596,12 +1596,12 @@ __global__ __launch_bounds__(WG * LW) void k_cell_lds(KDesc D)
 }
 #endif
 
-template <typename R, int PASS, int CAT, bool EN, bool N3 = false>
+/* LOOP: the cells of a compacted list beyond the grid of the one-cell-per-workgroup
+   launch (M.skip), when the list has outgrown it since the host last sized it */
+template <typename R, int PASS, int CAT, bool EN, bool N3 = false, bool LOOP = false>
 __global__ __launch_bounds__(NT) void kb_cell(BatchMap M)
 {
-    const int2 it = M.items[blockIdx.x];
-    const KDesc &D = M.descs[it.x];
-    b_cell<R, PASS, CAT, EN, false, N3>(D, it.y);
+    each_item<LOOP>(M, [&](const KDesc &D, int c) { b_cell<R, PASS, CAT, EN, false, N3>(D, c); });
 }
 
 /* N3: a slot's force is its own home-side sum plus the partner sums left for it by the
@@ -1645,9 +1645,7 @@ static __device__ void b_fold(const KDesc &D, int blk)
 __global__ __launch_bounds__(256) void k_fold(KDesc D) { b_fold(D, blockIdx.x); }
 __global__ __launch_bounds__(256) void kb_fold(BatchMap M)
 {
-    const int2 it = M.items[blockIdx.x];
-    const KDesc &D = M.descs[it.x];
-    b_fold(D, it.y);
+    each_item<true>(M, [&](const KDesc &D, int blk) { b_fold(D, blk); });
 }
 /* FP32 positions relative to the origin of the home cell, from the FP64 positions */
 static __device__ void b_relpos(const KDesc &D, int blk)
@@ -1660,9 +1658,7 @@ static __device__ void b_relpos(const KDesc &D, int blk)
 __global__ __launch_bounds__(256) void k_relpos(KDesc D) { b_relpos(D, blockIdx.x); }
 __global__ __launch_bounds__(256) void kb_relpos(BatchMap M)
 {
-    const int2 it = M.items[blockIdx.x];
-    const KDesc &D = M.descs[it.x];
-    b_relpos(D, it.y);
+    each_item<true>(M, [&](const KDesc &D, int blk) { b_relpos(D, blk); });
 }
 
 static inline size_t pp_bytes(const KDesc &D) { return sizeof(PairParam) * D.nkinds * D.nkinds; }
@@ -1816,45 +1812,65 @@ hipError_t launch_forces(const KDesc &D, hipStream_t st, int precision, bool rel
     return D.energy ? launch_reduce_pe(D, st) : hipSuccess;
 }
 
+/* one category's pass over a list: one workgroup per cell, and, on a compacted list, a
+   small looping launch for the cells beyond that grid (it exits at once when there are none) */
+#define GRID_MORE 256
+template <typename R, int PASS, int CAT, bool EN, bool N3>
+static void launchb_cat(const BatchMap &mc, dim3 g, size_t ppb, hipStream_t st)
+{
+    hipLaunchKernelGGL((kb_cell<R, PASS, CAT, EN, N3>), g, dim3(NT), ppb, st, mc);
+    if (mc.count)
+    {
+        BatchMap mo = mc;
+        mo.skip = (int)g.x;
+        hipLaunchKernelGGL((kb_cell<R, PASS, CAT, EN, N3, true>), dim3(GRID_MORE), dim3(NT), ppb, st, mo);
+    }
+}
+
+/* the pass over every cell of the members (lists, n), or, compacted, over the occupied
+   ones (occ, occ_n[at + cat]) with the host-sized grids */
 template <typename R, int PASS, bool EN>
-static void launchb_cells(const BatchPlan &P, hipStream_t st, const int2 *const *lists, const int *n)
+static void launchb_cells(const BatchPlan &P, hipStream_t st, const int2 *const *lists, const int *n,
+                          int2 *const *occ, const int *grid, int at)
 {
     const size_t ppb = sizeof(PairParam) * P.max_nkinds * P.max_nkinds;
+    constexpr bool N3OK = sizeof(R) == sizeof(float) && PASS == 0 && FMDX_N3 && FMDX_WAVES_PER_CELL == 1;
     for (int cat = 0; cat < 4; cat++)
     {
         if (n[cat] == 0) continue;
-        BatchMap mc = {P.descs, lists[cat]};
-        const dim3 g(n[cat]), b(NT);
+        BatchMap mc = {P.descs, lists[cat], nullptr, 0};
+        dim3 g(n[cat]);
+        if (P.compact)
+        {
+            mc = {P.descs, occ[cat], P.occ_n + at + cat, 0};
+            g = dim3((unsigned)grid[cat]);
+        }
         switch (cat)
         {
-            case CAT_EAM1: hipLaunchKernelGGL((kb_cell<R, PASS, CAT_EAM1, EN>), g, b, ppb, st, mc); break;
+            case CAT_EAM1: launchb_cat<R, PASS, CAT_EAM1, EN, false>(mc, g, ppb, st); break;
             case CAT_LJ:
-                if constexpr (sizeof(R) == sizeof(float) && PASS == 0 && FMDX_N3 && FMDX_WAVES_PER_CELL == 1)
-                    if (P.n3)
-                    {
-                        hipLaunchKernelGGL((kb_cell<R, PASS, CAT_LJ, EN, true>), g, b, ppb, st, mc);
-                        break;
-                    }
-                hipLaunchKernelGGL((kb_cell<R, PASS, CAT_LJ, EN>), g, b, ppb, st, mc);
+                if (N3OK && P.n3) launchb_cat<R, PASS, CAT_LJ, EN, N3OK>(mc, g, ppb, st);
+                else launchb_cat<R, PASS, CAT_LJ, EN, false>(mc, g, ppb, st);
                 break;
             case CAT_MORSE:
-                if constexpr (sizeof(R) == sizeof(float) && PASS == 0 && FMDX_N3 && FMDX_WAVES_PER_CELL == 1)
-                    if (P.n3)
-                    {
-                        hipLaunchKernelGGL((kb_cell<R, PASS, CAT_MORSE, EN, true>), g, b, ppb, st, mc);
-                        break;
-                    }
-                hipLaunchKernelGGL((kb_cell<R, PASS, CAT_MORSE, EN>), g, b, ppb, st, mc);
+                if (N3OK && P.n3) launchb_cat<R, PASS, CAT_MORSE, EN, N3OK>(mc, g, ppb, st);
+                else launchb_cat<R, PASS, CAT_MORSE, EN, false>(mc, g, ppb, st);
                 break;
-            default: hipLaunchKernelGGL((kb_cell<R, PASS, CAT_GEN, EN>), g, b, ppb, st, mc); break;
+            default: launchb_cat<R, PASS, CAT_GEN, EN, false>(mc, g, ppb, st); break;
         }
     }
     /* each pair once: the partner sums folded into the forces (members without them pass) */
     if constexpr (sizeof(R) == sizeof(float) && PASS == 0 && FMDX_N3 && FMDX_WAVES_PER_CELL == 1)
         if (P.n3)
         {
-            BatchMap ms = {P.descs, P.slot_items};
-            hipLaunchKernelGGL(kb_fold, dim3(P.n_slot), dim3(TPB), 0, st, ms);
+            BatchMap ms = {P.descs, P.slot_items, nullptr, 0};
+            dim3 gs(P.n_slot);
+            if (P.compact)
+            {
+                ms = {P.descs, P.occ_slot, P.occ_n, 0};
+                gs = dim3((unsigned)P.g_slot);
+            }
+            hipLaunchKernelGGL(kb_fold, gs, dim3(TPB), 0, st, ms);
         }
 }
 
@@ -1866,15 +1882,15 @@ static void launchb_passes(const BatchPlan &P, hipStream_t st, int precision, bo
 {
     if (precision == FMDX_PREC_MIXED)
     {
-        BatchMap ms = {P.descs, P.slot_items};
+        BatchMap ms = {P.descs, P.slot_items, nullptr, 0};
         if (relpos) hipLaunchKernelGGL(kb_relpos, dim3(P.n_slot), dim3(TPB), 0, st, ms);
-        launchb_cells<float, 1, true>(P, st, P.cell1, P.n_cell1);
-        launchb_cells<float, 0, EN>(P, st, P.cell0, P.n_cell0);
+        launchb_cells<float, 1, true>(P, st, P.cell1, P.n_cell1, P.occ_cell1, P.g_cell1, 5);
+        launchb_cells<float, 0, EN>(P, st, P.cell0, P.n_cell0, P.occ_cell0, P.g_cell0, 1);
     }
     else
     {
-        launchb_cells<double, 1, true>(P, st, P.cell1, P.n_cell1);
-        launchb_cells<double, 0, EN>(P, st, P.cell0, P.n_cell0);
+        launchb_cells<double, 1, true>(P, st, P.cell1, P.n_cell1, P.occ_cell1, P.g_cell1, 5);
+        launchb_cells<double, 0, EN>(P, st, P.cell0, P.n_cell0, P.occ_cell0, P.g_cell0, 1);
     }
 }
 

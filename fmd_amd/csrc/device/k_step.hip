@@ -34,9 +34,7 @@
     __global__ __launch_bounds__(TPB) void k_##NAME(KDesc D) { b_##NAME(D, blockIdx.x); } \
     __global__ __launch_bounds__(TPB) void kb_##NAME(BatchMap M)                       \
     {                                                                                  \
-        const int2 it = M.items[blockIdx.x];                                           \
-        const KDesc &D = M.descs[it.x];                                                 \
-        b_##NAME(D, it.y);                                                             \
+        each_item<true>(M, [&](const KDesc &D, int blk) { b_##NAME(D, blk); });         \
     }
 
 static __device__ __forceinline__ void cell_coords(const KDesc &D, int c, int &cx, int &cy, int &cz)
@@ -831,23 +829,113 @@ hipError_t launch_fill(double *p, long n, double v, hipStream_t st)
     return hipGetLastError();
 }
 
+/* ------------------------------------------- occupied cells of a batch */
+
+/* wave-aggregated append of v to list where pred holds: one atomic per wave */
+static __device__ __forceinline__ void append(int2 *list, int *len, bool pred, int2 v)
+{
+    const unsigned long long m = __ballot(pred);
+    if (!m) return;
+    const int lane = (int)(threadIdx.x % WAVE);
+    const int first = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == first) base = atomicAdd(len, __popcll(m));
+    base = __shfl(base, first);
+    if (pred) list[base + __popcll(m & ((1ull << lane) - 1ull))] = v;
+}
+
+__global__ __launch_bounds__(64) void kb_occ_clear(int *occ_n)
+{
+    if (threadIdx.x < 9) occ_n[threadIdx.x] = 0;
+}
+
+/* After a re-binning, over the (system, block of TPB cells) items: each occupied cell
+   into its system's force-pass list (and density-pass list, EAM), each group of CPB
+   cells holding an atom into the O(N) list.  An empty cell gets the words the force
+   passes write for one (no pair or embedding energy, no recorded hits) and an empty
+   group the zero thermo partials the kick writes for it, so that the reductions over
+   every cell and group read what they read without the compaction; each system's
+   mover counter is reset for the next re-binning.  The order of a list decides only
+   which workgroup takes an item, never a result. */
+__global__ __launch_bounds__(TPB) void kb_occ(BatchMap M, BatchPlan P)
+{
+    const int2 it = M.items[blockIdx.x];
+    const KDesc &D = M.descs[it.x];
+    const int c = it.y * TPB + (int)threadIdx.x;
+    const bool in = c < D.ncell;
+    const int n = in ? D.cnt[c] : 0;
+    if (in && n == 0)
+    {
+        D.cell_pe[c] = 0.0;
+        if (D.has_eam)
+        {
+            D.cell_emb[c] = 0.0;
+            D.htrips[c] = -1;
+        }
+    }
+    if (it.y == 0 && threadIdx.x == 0) D.counters[0] = 0;
+    const int cat = P.sys_cat[it.x];
+    append(P.occ_cell0[cat], &P.occ_n[1 + cat], n > 0, make_int2(it.x, c));
+    if (D.has_eam) append(P.occ_cell1[cat], &P.occ_n[5 + cat], n > 0, make_int2(it.x, c));
+    /* the group of lane l is lanes l & ~(CPB-1) .. | (CPB-1) of the same wave */
+    const unsigned long long occ = __ballot(n > 0);
+    const int lane = (int)(threadIdx.x % WAVE);
+    const bool lead = (lane % CPB) == 0;
+    const bool gocc = ((occ >> (lane & ~(CPB - 1))) & ((1ull << CPB) - 1ull)) != 0ull;
+    const bool gin = in;   /* a group exists when its first cell does */
+    const int g = c / CPB;
+    append(P.occ_slot, &P.occ_n[0], lead && gin && gocc, make_int2(it.x, g));
+    if (lead && gin && !gocc)
+        for (int i = 0; i < 5; i++) D.part[(long)g * 8 + i] = 0.0;
+}
+
+hipError_t launchb_occ(const BatchPlan &P, hipStream_t st)
+{
+    if (P.nsys == 0 || !P.compact) return hipSuccess;
+    BatchMap mc = {P.descs, P.cellblk_items, nullptr, 0};
+    hipLaunchKernelGGL(kb_occ_clear, dim3(1), dim3(64), 0, st, P.occ_n);
+    hipLaunchKernelGGL(kb_occ, dim3(P.n_cellblk), dim3(TPB), 0, st, mc, P);
+    return hipGetLastError();
+}
+
+/* a batch launch over the slot groups: every group, or the occupied ones */
+static inline void slot_map(const BatchPlan &P, BatchMap &m, dim3 &g)
+{
+    if (P.compact)
+    {
+        m = {P.descs, P.occ_slot, P.occ_n, 0};
+        g = dim3((unsigned)P.g_slot);
+    }
+    else
+    {
+        m = {P.descs, P.slot_items, nullptr, 0};
+        g = dim3((unsigned)P.n_slot);
+    }
+}
+
 /* ------------------------------------------------------ batched launchers */
 
 hipError_t launchb_vv_start(const BatchPlan &P, hipStream_t st)
 {
     if (P.nsys == 0) return hipSuccess;
-    BatchMap ms = {P.descs, P.slot_items}, mc = {P.descs, P.cellblk_items};
-    hipLaunchKernelGGL(kb_vv_start, dim3(P.n_slot), dim3(TPB), 0, st, ms);
+    BatchMap ms, mc = {P.descs, P.cellblk_items, nullptr, 0};
+    dim3 gs;
+    slot_map(P, ms, gs);
+    hipLaunchKernelGGL(kb_vv_start, gs, dim3(TPB), 0, st, ms);
     hipLaunchKernelGGL(kb_bin_rank, dim3(P.n_cellblk), dim3(TPB), 0, st, mc);
     hipLaunchKernelGGL(kb_bin_apply, dim3(P.n_cellblk), dim3(TPB), 0, st, mc);
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launchb_occ(P, st);   /* the lists of the atoms' new cells */
 }
 
 hipError_t launchb_vv_finish(const BatchPlan &P, hipStream_t st)
 {
     if (P.nsys == 0) return hipSuccess;
-    BatchMap ms = {P.descs, P.slot_items};
-    hipLaunchKernelGGL(kb_vv_finish, dim3(P.n_slot), dim3(TPB), 0, st, ms);
+    BatchMap ms;
+    dim3 gs;
+    slot_map(P, ms, gs);
+    hipLaunchKernelGGL(kb_vv_finish, gs, dim3(TPB), 0, st, ms);
     hipLaunchKernelGGL(kb_reduce_thermo, dim3(P.nsys), dim3(1024), 0, st, P.descs);
     return hipGetLastError();
 }
@@ -855,7 +943,7 @@ hipError_t launchb_vv_finish(const BatchPlan &P, hipStream_t st)
 hipError_t launchb_thermo(const BatchPlan &P, hipStream_t st)
 {
     if (P.nsys == 0) return hipSuccess;
-    BatchMap ms = {P.descs, P.slot_items};
+    BatchMap ms = {P.descs, P.slot_items, nullptr, 0};
     hipLaunchKernelGGL(kb_thermo, dim3(P.n_slot), dim3(TPB), 0, st, ms);
     hipLaunchKernelGGL(kb_reduce_thermo, dim3(P.nsys), dim3(1024), 0, st, P.descs);
     return hipGetLastError();
@@ -864,7 +952,7 @@ hipError_t launchb_thermo(const BatchPlan &P, hipStream_t st)
 hipError_t launchb_reduce_pe(const BatchPlan &P, hipStream_t st)
 {
     if (P.nsys == 0) return hipSuccess;
-    BatchMap mp = {P.descs, P.pe_items};
+    BatchMap mp = {P.descs, P.pe_items, nullptr, 0};
     hipLaunchKernelGGL(kb_pe_part, dim3(P.n_pe), dim3(TPB), 0, st, mp);
     hipLaunchKernelGGL(kb_pe_final, dim3(P.nsys), dim3(1024), 0, st, P.descs);
     return hipGetLastError();

@@ -247,6 +247,17 @@ int fmdx_batch_size(fmdx_batch_t *b, int *n);
 int fmdx_batch_use_graph(fmdx_batch_t *b, int on);
 /* status words are read (and slot capacities grown) every `steps` steps; default 8 */
 int fmdx_batch_set_check_interval(fmdx_batch_t *b, int steps);
+/* Which cells a step visits.  FMDX_COMPACT_AUTO (default): when under 60 % of the members'
+   cells hold atoms (vacuum around clusters), the step visits only the occupied cells and
+   slot groups, from lists the device rebuilds after every re-binning; fuller batches visit
+   every cell.  FMDX_COMPACT_OFF: every cell.  FMDX_COMPACT_TIGHT: always the occupied
+   cells, with launch grids exactly as long as the lists were at the last status read
+   (cells occupied since then are taken by a looping launch).  Results are the same bits
+   in every mode. */
+#define FMDX_COMPACT_OFF 0
+#define FMDX_COMPACT_AUTO 1
+#define FMDX_COMPACT_TIGHT 2
+int fmdx_batch_set_compaction(fmdx_batch_t *b, int mode);
 /* velocity-Verlet steps of every member; synchronises at its end */
 int fmdx_batch_step(fmdx_batch_t *b, int nsteps);
 /* thermo record and status of every member, in join order; synchronises */

@@ -100,7 +100,9 @@ def test_each_pair_once_variant_compiles(tmp_path):
     found = 0
     for blk in re.split(r"\n  - ", meta)[1:]:
         name = re.search(r"\.name:\s+(\S+)", blk)
-        if not name or not re.search(r"(k|kb)_cellIfLi0ELi[23]ELb[01]ELb1E", name.group(1)):
+        # the pair kernels with each pair once; not the batch's looping launch for the cells
+        # of a compacted list beyond its grid (last template argument true), which rarely runs
+        if not name or not re.search(r"(k|kb)_cellIfLi0ELi[23]ELb[01]ELb1E(Ev|Lb0E)", name.group(1)):
             continue
         get = lambda k: int(re.search(r"\." + k + r":\s+(\d+)", blk).group(1))
         found += 1

@@ -170,3 +170,50 @@ def test_batch_member_follows_oracle(rng):
     assert t.temperature == pytest.approx(en["T"], rel=1e-10)
     assert t.pe == pytest.approx(en["pe"], rel=1e-10)
     b.close()
+
+
+def _cluster(rng, pots, m, a, R, L, T, drift, name="Cu"):
+    """an FCC sphere of radius R at the centre of an open L^3 box, moving at `drift` (A/ps)"""
+    n = int(np.ceil(R / a)) + 1
+    x = lattice_fcc((2 * n, 2 * n, 2 * n), a, rng, jitter=0.03, origin=(-n * a, -n * a, -n * a))
+    x = x[np.linalg.norm(x, axis=1) <= R] + L / 2.0
+    v = maxwell_velocities(len(x), m, T, rng) + np.asarray(drift)
+    return ([L, L, L], [0, 0, 0], [m], [name], pots, x, v)
+
+
+@pytest.mark.parametrize("precision", [F.FMDX_PREC_FP64, F.FMDX_PREC_MIXED])
+def test_compacted_cell_lists_same_bits(rng, cu01, precision):
+    """Clusters in vacuum (a few percent of the cells occupied): a step over the occupied
+    cells only, from lists the device rebuilds after each re-binning, gives the same bits as
+    a step over every cell, also with grids exactly as long as the lists at the last status
+    read, so that cells occupied since then are taken by the looping launch and the step is
+    captured again after a read that finds longer lists."""
+    c = CU_MORSE
+    specs = [
+        _cluster(rng, [("morse", 0, 0, c["D0"], c["alpha"], c["r0"], c["rc"])], c["m"], c["a"], 11.0, 64.0,
+                 900.0, (30.0, -10.0, 5.0)),
+        _cluster(rng, [("eam", cu01, [(0, 0)])], 63.546, 3.615, 9.0, 56.0, 1200.0, (0.0, 25.0, 0.0)),
+        _cluster(rng, [("lj", 0, 0, AR["sig"], AR["eps"], AR["rc"])], AR["m"], AR["a"], 14.0, 70.0, 60.0,
+                 (-8.0, 0.0, 12.0), name="Ar"),
+    ]
+    runs = {}
+    for mode in (F.FMDX_COMPACT_OFF, F.FMDX_COMPACT_AUTO, F.FMDX_COMPACT_TIGHT):
+        b = F.Batch(precision)
+        b.set_compaction(mode)
+        ds = []
+        for sp in specs:
+            _, d = _make(sp, precision)
+            b.add(d, 2e-3)
+            ds.append(d)
+        for _ in range(3):
+            b.step(16)
+        st = [(t.temperature, t.pe, t.err) for t in b.status()]
+        runs[mode] = (ds, st)
+        b.close()
+    ds0, st0 = runs[F.FMDX_COMPACT_OFF]
+    assert all(t[2] == 0 for t in st0)
+    for mode in (F.FMDX_COMPACT_AUTO, F.FMDX_COMPACT_TIGHT):
+        ds, st = runs[mode]
+        assert st == st0, mode
+        for a, b in zip(ds0, ds):
+            _same(a, b)
