@@ -112,6 +112,7 @@ static void cell_layout(fmdx_system_t *s, Carve &c, size_t ncell)
     c(&D.cell_pe, ncell); c(&D.cell_emb, ncell);
     c(&D.htrips, ncell);
     c(&D.htile, ncell);
+    c(&D.btouch, ncell / TPB + 1);
     c(&D.epart, 2 * NPART_MAX);
     c(&D.thermo, 16);
     c(&s->dpairs, MAXK * MAXK);
@@ -176,6 +177,7 @@ extern "C" int fmdx_system_create(fmdx_system_t **out, const double l[3], const 
     hipMemsetAsync(D.cell_emb, 0, sizeof(double) * ncell, s->st);
     hipMemsetAsync(D.htrips, 0xff, sizeof(int) * ncell, s->st);   /* -1: no recorded hits */
     hipMemsetAsync(D.htile, 0, sizeof(unsigned long long) * ncell, s->st);
+    hipMemsetAsync(D.btouch, 0, sizeof(int) * (ncell / TPB + 1), s->st);
     hipMemsetAsync(D.thermo, 0, sizeof(double) * 16, s->st);
     memset(s->hp, 0, sizeof s->hp);
     D.pairs = s->dpairs;

@@ -126,13 +126,15 @@ static int fit_grids(fmdx_batch_t *b, int force)
         const int cap = std::max(8, (full + 7) / 8 * 8);
         const int room = tight ? 0 : n / 4 + 64;
         const int want = std::min(cap, std::max(8, (n + room + 7) / 8 * 8));
-        if (force || n > g || (want < g / 2))
+        if (want != g && (force || n > g || want < g / 2))
         {
-            if (g != want) { g = want; return 1; }
+            g = want;
+            return 1;
         }
         return 0;
     };
     int changed = fit(P.g_slot, b->hocc[0], P.n_slot);
+    changed |= fit(P.g_bin, force ? P.n_cellblk : b->hocc[9], P.n_cellblk);   /* no count before a step */
     for (int c = 0; c < 4; c++)
     {
         changed |= fit(P.g_cell0[c], b->hocc[1 + c], P.n_cell0[c]);
@@ -201,7 +203,7 @@ static int rebuild(fmdx_batch_t *b)
     /* the compacted lists hold at most every slot group and cell of the full ones */
     BatchPlan &P = b->plan;
     P.compact = FMDX_COMPACT && n > 0 && b->compact_mode != FMDX_COMPACT_OFF;
-    size_t nocc = slot.size();
+    size_t nocc = slot.size() + cblk.size();
     for (int c = 0; c < 4; c++) nocc += c0[c].size() + c1[c].size();
     auto occ_layout = [&](Carve &cv, size_t cap, size_t nsys) {
         cv(&P.occ_n, 16);
@@ -230,9 +232,11 @@ static int rebuild(fmdx_batch_t *b)
             P.occ_cell1[c] = at;
             at += c1[c].size();
         }
+        P.occ_bin = at;
         std::vector<int> cats(n);
         for (int i = 0; i < n; i++) cats[i] = force_category(b->sys[i]->D);
         CK(hipMemcpyAsync((void *)P.sys_cat, cats.data(), sizeof(int) * n, hipMemcpyHostToDevice, b->st));
+
     }
     /* members whose force passes evaluate each pair once need their partner sums */
     for (int i = 0; i < n; i++)
@@ -258,7 +262,7 @@ static int rebuild(fmdx_batch_t *b)
     b->hthermo.assign((size_t)std::max(n, 1) * 16, 0.0);
     /* the first step's start and kick walk the lists of the members' current cells */
     CK(launchb_occ(P, b->st));
-    if (P.compact) CK(hipMemcpyAsync(b->hocc, P.occ_n, sizeof(int) * 9, hipMemcpyDeviceToHost, b->st));
+    if (P.compact) CK(hipMemcpyAsync(b->hocc, P.occ_n, sizeof(int) * 10, hipMemcpyDeviceToHost, b->st));
     CK(hipStreamSynchronize(b->st));
     if (b->compact_mode == FMDX_COMPACT_AUTO) P.compact = P.compact && sparse(b);
     fit_grids(b, 1);
@@ -322,7 +326,7 @@ static int collect(fmdx_batch_t *b)
     CK(hipMemcpyAsync(b->hstatus.data(), b->dstatus, sizeof(unsigned) * 8 * n, hipMemcpyDeviceToHost, b->st));
     CK(hipMemcpyAsync(b->hthermo.data(), b->dthermo, sizeof(double) * 16 * n, hipMemcpyDeviceToHost, b->st));
     if (b->plan.compact)
-        CK(hipMemcpyAsync(b->hocc, b->plan.occ_n, sizeof(int) * 9, hipMemcpyDeviceToHost, b->st));
+        CK(hipMemcpyAsync(b->hocc, b->plan.occ_n, sizeof(int) * 10, hipMemcpyDeviceToHost, b->st));
     CK(hipStreamSynchronize(b->st));
     if (b->plan.compact && b->compact_mode == FMDX_COMPACT_AUTO && !sparse(b))
     {

@@ -95,6 +95,8 @@ struct KDesc
     int *mv;
     uint8_t *code;
     int *nleave, *narr, *nearly, *fill;
+    int *btouch;          /* [ncell/TPB + 1] a re-binning moves atoms out of or into block b of
+                             TPB cells (set by the start step, cleared by the batch's kb_touch) */
     unsigned long long *bkey;   /* [ncell*cap] arrival buckets */
     int *bid;
     double *mvx, *mvy, *mvz, *mvvx, *mvvy, *mvvz;   /* mover staging */
@@ -164,7 +166,7 @@ struct BatchMap
    otherwise it takes at most one, and the items beyond gridDim are left to a LOOP launch
    with skip = gridDim. */
 template <bool LOOP, class F>
-static __device__ __forceinline__ void each_item(const BatchMap &M, F f)
+static __device__ __forceinline__ void each_item_xy(const BatchMap &M, F f)
 {
     int base = (int)blockIdx.x, first = 0, len = 1, step = 1;
     if (M.count)
@@ -180,16 +182,20 @@ static __device__ __forceinline__ void each_item(const BatchMap &M, F f)
     if (!LOOP)
     {
         if (first >= len) return;
-        const int2 it = M.items[base + first];
-        f(M.descs[it.x], it.y);
+        f(M.items[base + first]);
         return;
     }
     for (int i = first; i < len; i += step)
     {
-        const int2 it = M.items[base + i];
-        f(M.descs[it.x], it.y);
+        f(M.items[base + i]);
         __syncthreads();   /* the next item reuses the workgroup's LDS */
     }
+}
+
+template <bool LOOP, class F>
+static __device__ __forceinline__ void each_item(const BatchMap &M, F f)
+{
+    each_item_xy<LOOP>(M, [&](int2 it) { f(M.descs[it.x], it.y); });
 }
 
 /* TTM state resident on the device: n = ncx*ncy*(ncz+2) turi cells, index
@@ -269,17 +275,19 @@ struct BatchPlan
     int max_nkinds;
     int n3;                      /* the LJ and Morse members evaluate each pair once (all have partner sums) */
     /* the step over occupied cells only: after each re-binning kb_occ writes the lists below
-       and their lengths occ_n[0] (slot groups), [1 + cat] (force pass), [5 + cat] (density pass) */
+       and their lengths occ_n[0] (slot groups), [1 + cat] (force pass), [5 + cat] (density pass);
+       before each re-binning kb_touch lists the blocks of TPB cells it changes, occ_n[9] */
     int compact;
     /* grids that walk the compacted lists (multiples of 8), sized by the host from the list
        lengths it last read, with room to grow; a list that outgrows its grid is still walked
        whole, some workgroups taking more than one item */
-    int g_slot, g_cell0[4], g_cell1[4];
+    int g_slot, g_cell0[4], g_cell1[4], g_bin;
     const int *sys_cat;          /* [nsys] pair category of each member's force pass */
-    int *occ_n;                  /* [9] */
+    int *occ_n;                  /* [16] */
     int2 *occ_slot;              /* capacity n_slot */
     int2 *occ_cell0[4];          /* capacity n_cell0[cat] */
     int2 *occ_cell1[4];          /* capacity n_cell1[cat] */
+    int2 *occ_bin;               /* capacity n_cellblk */
 };
 
 /* kernel launchers (defined in the kernel translation units); the step

@@ -107,6 +107,7 @@ static __device__ void bin_flag(const KDesc &D, int c, long slot, double x, doub
     }
     int code = cd[2] * 16 + cd[1] * 4 + cd[0];
     atomicAdd(&D.nleave[c], 1);
+    D.btouch[c / TPB] = 1;
     D.code[slot] = (uint8_t)code;
     if (del)
     {
@@ -115,6 +116,7 @@ static __device__ void bin_flag(const KDesc &D, int c, long slot, double x, doub
     }
     int dest = (f[0] * D.nc1 + f[1]) * D.nc2 + f[2];
     D.mv[slot] = dest;
+    D.btouch[dest / TPB] = 1;
     atomicAdd(&D.narr[dest], 1);
     if (code == 0 && c < dest) atomicAdd(&D.nearly[dest], 1);
 }
@@ -831,22 +833,36 @@ hipError_t launch_fill(double *p, long n, double v, hipStream_t st)
 
 /* ------------------------------------------- occupied cells of a batch */
 
-/* wave-aggregated append of v to list where pred holds: one atomic per wave */
-static __device__ __forceinline__ void append(int2 *list, int *len, bool pred, int2 v)
+/* zero n counters (the list lengths); a kernel, so that a captured step holds no memset */
+__global__ __launch_bounds__(64) void kb_occ_clear(int *occ_n, int n)
 {
-    const unsigned long long m = __ballot(pred);
+    if ((int)threadIdx.x < n) occ_n[threadIdx.x] = 0;
+}
+
+/* Before a re-binning: the (system, block of TPB cells) items whose cells lose or gain
+   atoms, as the start step flagged them, into occ_bin (length occ_n[9], zeroed just
+   before; it stays until the next step, for the status read); the flags are cleared.  The re-binning kernels then skip the blocks
+   where nothing moves, which they would leave as they are. */
+__global__ __launch_bounds__(TPB) void kb_touch(const KDesc *descs, const int2 *items, int n, int2 *list, int *len)
+{
+    const int i = (int)(blockIdx.x * TPB + threadIdx.x);
+    bool t = false;
+    int2 it = make_int2(0, 0);
+    if (i < n)
+    {
+        it = items[i];
+        int *const f = &descs[it.x].btouch[it.y];
+        t = *f != 0;
+        if (t) *f = 0;
+    }
+    const unsigned long long m = __ballot(t);
     if (!m) return;
     const int lane = (int)(threadIdx.x % WAVE);
     const int first = __ffsll((long long)m) - 1;
     int base = 0;
     if (lane == first) base = atomicAdd(len, __popcll(m));
     base = __shfl(base, first);
-    if (pred) list[base + __popcll(m & ((1ull << lane) - 1ull))] = v;
-}
-
-__global__ __launch_bounds__(64) void kb_occ_clear(int *occ_n)
-{
-    if (threadIdx.x < 9) occ_n[threadIdx.x] = 0;
+    if (t) list[base + __popcll(m & ((1ull << lane) - 1ull))] = it;
 }
 
 /* After a re-binning, over the (system, block of TPB cells) items: each occupied cell
@@ -855,45 +871,86 @@ __global__ __launch_bounds__(64) void kb_occ_clear(int *occ_n)
    passes write for one (no pair or embedding energy, no recorded hits) and an empty
    group the zero thermo partials the kick writes for it, so that the reductions over
    every cell and group read what they read without the compaction; each system's
-   mover counter is reset for the next re-binning.  The order of a list decides only
-   which workgroup takes an item, never a result. */
-__global__ __launch_bounds__(TPB) void kb_occ(BatchMap M, BatchPlan P)
+   mover counter is reset for the next re-binning.  A workgroup reserves its run of each
+   list with one atomic (one per wave serialised on the list lengths) and writes its
+   items in cell order.  The order of a list decides only which workgroup takes an item,
+   never a result. */
+static __device__ __forceinline__ void b_occ(const BatchPlan &P, int2 it)
 {
-    const int2 it = M.items[blockIdx.x];
-    const KDesc &D = M.descs[it.x];
+    __shared__ int wc[2][TPB / WAVE];
+    __shared__ int run[3];
+    const KDesc &D = P.descs[it.x];
     const int c = it.y * TPB + (int)threadIdx.x;
     const bool in = c < D.ncell;
     const int n = in ? D.cnt[c] : 0;
+    const bool eam = D.has_eam;
     if (in && n == 0)
     {
         D.cell_pe[c] = 0.0;
-        if (D.has_eam)
+        if (eam)
         {
             D.cell_emb[c] = 0.0;
             D.htrips[c] = -1;
         }
     }
     if (it.y == 0 && threadIdx.x == 0) D.counters[0] = 0;
-    const int cat = P.sys_cat[it.x];
-    append(P.occ_cell0[cat], &P.occ_n[1 + cat], n > 0, make_int2(it.x, c));
-    if (D.has_eam) append(P.occ_cell1[cat], &P.occ_n[5 + cat], n > 0, make_int2(it.x, c));
-    /* the group of lane l is lanes l & ~(CPB-1) .. | (CPB-1) of the same wave */
+    /* the group of lane l is lanes l & ~(CPB-1) .. | (CPB-1) of the same wave; a group
+       exists when its first cell does */
+    const int lane = (int)(threadIdx.x % WAVE), w = (int)(threadIdx.x / WAVE);
     const unsigned long long occ = __ballot(n > 0);
-    const int lane = (int)(threadIdx.x % WAVE);
     const bool lead = (lane % CPB) == 0;
     const bool gocc = ((occ >> (lane & ~(CPB - 1))) & ((1ull << CPB) - 1ull)) != 0ull;
-    const bool gin = in;   /* a group exists when its first cell does */
     const int g = c / CPB;
-    append(P.occ_slot, &P.occ_n[0], lead && gin && gocc, make_int2(it.x, g));
-    if (lead && gin && !gocc)
+    const unsigned long long gm = __ballot(lead && in && gocc);
+    if (lead && in && !gocc)
         for (int i = 0; i < 5; i++) D.part[(long)g * 8 + i] = 0.0;
+    if (lane == 0)
+    {
+        wc[0][w] = __popcll(occ);
+        wc[1][w] = __popcll(gm);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        int t0 = 0, t1 = 0;
+        for (int k = 0; k < TPB / WAVE; k++)
+        {
+            t0 += wc[0][k];
+            t1 += wc[1][k];
+        }
+        const int cat = P.sys_cat[it.x];
+        run[0] = t0 ? atomicAdd(&P.occ_n[1 + cat], t0) : 0;
+        run[1] = t0 && eam ? atomicAdd(&P.occ_n[5 + cat], t0) : 0;
+        run[2] = t1 ? atomicAdd(&P.occ_n[0], t1) : 0;
+    }
+    __syncthreads();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int p0 = 0, p1 = 0;
+    for (int k = 0; k < w; k++)
+    {
+        p0 += wc[0][k];
+        p1 += wc[1][k];
+    }
+    if (n > 0)
+    {
+        const int cat = P.sys_cat[it.x];
+        const int k = p0 + __popcll(occ & below);
+        P.occ_cell0[cat][run[0] + k] = make_int2(it.x, c);
+        if (eam) P.occ_cell1[cat][run[1] + k] = make_int2(it.x, c);
+    }
+    if (lead && in && gocc) P.occ_slot[run[2] + p1 + __popcll(gm & below)] = make_int2(it.x, g);
+}
+
+__global__ __launch_bounds__(TPB) void kb_occ(BatchMap M, BatchPlan P)
+{
+    each_item_xy<false>(M, [&](int2 it) { b_occ(P, it); });
 }
 
 hipError_t launchb_occ(const BatchPlan &P, hipStream_t st)
 {
     if (P.nsys == 0 || !P.compact) return hipSuccess;
     BatchMap mc = {P.descs, P.cellblk_items, nullptr, 0};
-    hipLaunchKernelGGL(kb_occ_clear, dim3(1), dim3(64), 0, st, P.occ_n);
+    hipLaunchKernelGGL(kb_occ_clear, dim3(1), dim3(64), 0, st, P.occ_n, 9);
     hipLaunchKernelGGL(kb_occ, dim3(P.n_cellblk), dim3(TPB), 0, st, mc, P);
     return hipGetLastError();
 }
@@ -919,11 +976,19 @@ hipError_t launchb_vv_start(const BatchPlan &P, hipStream_t st)
 {
     if (P.nsys == 0) return hipSuccess;
     BatchMap ms, mc = {P.descs, P.cellblk_items, nullptr, 0};
-    dim3 gs;
+    dim3 gs, gc(P.n_cellblk);
     slot_map(P, ms, gs);
     hipLaunchKernelGGL(kb_vv_start, gs, dim3(TPB), 0, st, ms);
-    hipLaunchKernelGGL(kb_bin_rank, dim3(P.n_cellblk), dim3(TPB), 0, st, mc);
-    hipLaunchKernelGGL(kb_bin_apply, dim3(P.n_cellblk), dim3(TPB), 0, st, mc);
+    if (P.compact)
+    {
+        hipLaunchKernelGGL(kb_occ_clear, dim3(1), dim3(64), 0, st, P.occ_n + 9, 1);
+        hipLaunchKernelGGL(kb_touch, dim3((P.n_cellblk + TPB - 1) / TPB), dim3(TPB), 0, st, P.descs,
+                           P.cellblk_items, P.n_cellblk, P.occ_bin, P.occ_n + 9);
+        mc = {P.descs, P.occ_bin, P.occ_n + 9, 0};
+        gc = dim3((unsigned)P.g_bin);
+    }
+    hipLaunchKernelGGL(kb_bin_rank, gc, dim3(TPB), 0, st, mc);
+    hipLaunchKernelGGL(kb_bin_apply, gc, dim3(TPB), 0, st, mc);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launchb_occ(P, st);   /* the lists of the atoms' new cells */
