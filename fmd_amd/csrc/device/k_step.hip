@@ -397,28 +397,32 @@ static __device__ __forceinline__ void copy_staged(const KDesc &D, int mid, long
     if (D.xq_keep) D.xq[to] = rel_pos(D, cx, cy, cz, D.mvx[mid], D.mvy[mid], D.mvz[mid]);
 }
 
-/* The replays below walk a cell's move flags in a data-dependent order, one
-   dependent read per step.  A cell of up to MVL atoms first copies its flags
-   into an LDS row with independent 16-byte loads (the library's slot
-   capacities are multiples of 32, so the row never reads past the cell and
-   the loads are aligned); larger cells, and capacities set by a caller to
-   other values, read HBM directly. */
+/* The replays below walk a cell's move flags in a data-dependent order.  A cell of up to
+   MVL atoms first turns its flags into a bit mask (bit k: slot k leaves) with independent
+   16-byte loads (the library's slot capacities are multiples of 32, so the loads are
+   aligned and stay inside the cell); larger cells, and capacities set by a caller to other
+   values, read the flags from HBM at each step of the walk.  The walks only decide which
+   atom goes where: the atoms are then staged or moved by all threads of the block, one
+   atom each, so that no thread carries a chain of dependent copies (a slowest cell's chain
+   set the time of the whole re-binning).  A cell's moves read and write disjoint slots
+   (sources at or beyond its new count, destinations below it), so they may run in any
+   order. */
 #define MVL 32
-struct MvRow
+struct MoveFlags
 {
     const int *g;
-    int *row;
+    unsigned m;
     bool local;
-    __device__ __forceinline__ int operator[](int k) const { return local ? row[k] : g[k]; }
+    __device__ __forceinline__ bool leaves(int k) const { return local ? ((m >> k) & 1u) != 0u : g[k] != MV_STAY; }
 };
 
-static __device__ __forceinline__ MvRow mv_row(const KDesc &D, long base, int nc, int (*rows)[MVL + 1])
+static __device__ __forceinline__ MoveFlags move_flags(const KDesc &D, long base, int nc)
 {
-    MvRow r;
-    r.g = D.mv + base;
-    r.row = rows[threadIdx.x];
-    r.local = nc <= MVL && D.cap >= MVL && (D.cap & 3) == 0;
-    if (r.local)
+    MoveFlags f;
+    f.g = D.mv + base;
+    f.m = 0u;
+    f.local = nc <= MVL && D.cap >= MVL && (D.cap & 3) == 0;
+    if (f.local)
     {
         const int4 *src = reinterpret_cast<const int4 *>(D.mv + base);
         int4 q[MVL / 4];
@@ -426,12 +430,10 @@ static __device__ __forceinline__ MvRow mv_row(const KDesc &D, long base, int nc
         for (int k = 0; k < MVL / 4; k++) q[k] = src[k];
 #pragma unroll
         for (int k = 0; k < MVL / 4; k++)
-        {
-            r.row[4 * k] = q[k].x; r.row[4 * k + 1] = q[k].y;
-            r.row[4 * k + 2] = q[k].z; r.row[4 * k + 3] = q[k].w;
-        }
+            f.m |= ((unsigned)(q[k].x != MV_STAY) | ((unsigned)(q[k].y != MV_STAY) << 1) |
+                    ((unsigned)(q[k].z != MV_STAY) << 2) | ((unsigned)(q[k].w != MV_STAY) << 3)) << (4 * k);
     }
-    return r;
+    return f;
 }
 
 #define WAVE 64   /* lanes per wavefront on gfx950 */
@@ -460,85 +462,166 @@ static __device__ __forceinline__ unsigned block_prefix(unsigned v, unsigned &to
     return before + incl - v;
 }
 
+/* one leaver of cell c at slot s into staging entry mid, rank-th leaver of its cell: its
+   position as ccopy_for_migrate leaves it and its arrival key in the destination's bucket */
+static __device__ __forceinline__ void stage_leaver(const KDesc &D, int c, long s, int mid, int rank)
+{
+#pragma clang fp contract(off)
+    const int dst = D.mv[s];
+    if (dst == MV_DELETE)
+    {
+        atomicAdd(&D.counters[1], 1u);
+        return;
+    }
+    const int code = D.code[s];
+    double x = D.x[s], y = D.y[s], z = D.z[s];
+    /* ccopy_for_migrate, md-ghost.c:132-140: x += L (lower face) or x += -L (upper) */
+    if ((code & 3) == 1) x += D.L0;
+    else if ((code & 3) == 2) x += -D.L0;
+    if (((code >> 2) & 3) == 1) y += D.L1;
+    else if (((code >> 2) & 3) == 2) y += -D.L1;
+    if (((code >> 4) & 3) == 1) z += D.L2;
+    else if (((code >> 4) & 3) == 2) z += -D.L2;
+    D.mvx[mid] = x; D.mvy[mid] = y; D.mvz[mid] = z;
+    D.mvvx[mid] = D.vx[s]; D.mvvy[mid] = D.vy[s]; D.mvvz[mid] = D.vz[s];
+    D.mvgrp[mid] = D.grp[s]; D.mvaid[mid] = D.aid[s]; D.mvkind[mid] = D.kind[s];
+    const unsigned long long key = ((unsigned long long)code << 58) | ((unsigned long long)c << 16) |
+                                   (unsigned long long)rank;
+    const int pos = atomicAdd(&D.fill[dst], 1);
+    if (pos < D.cap)
+    {
+        D.bkey[(long)dst * D.cap + pos] = key;
+        D.bid[(long)dst * D.cap + pos] = mid;
+    }
+    else
+        set_err(D, FMDX_DEV_ERR_OVERFLOW);
+}
+
 /* replay of the swap-remove scan (misc.c:279-303) to rank and stage leavers.
    Each block reserves one run of staging entries for all its cells with a
    single atomic (a cell takes one entry per leaver, deletions included, so
    the staging buffer of natoms entries always suffices); one atomic per
-   leaver on the shared counter serialised across the whole device. */
+   leaver on the shared counter serialised across the whole device.  The walks
+   list the block's leavers in staging order; all threads then stage them (a
+   block with more than LVMAX leavers stages them in its walks). */
+#define LVMAX 1024
 static __device__ void b_bin_rank(const KDesc &D, int blk)
 {
-#pragma clang fp contract(off)
-    __shared__ int rows[TPB][MVL + 1];
+    __shared__ int lv_slot[LVMAX];        /* slot of the leaver within its cell */
+    __shared__ uint8_t lv_cell[LVMAX];    /* its cell within the block */
+    __shared__ unsigned lv_off[TPB];      /* each cell's first entry of the block's run */
     __shared__ unsigned run0;
     const int c = blk * TPB + threadIdx.x;
     const int nl = c < D.ncell ? D.nleave[c] : 0;
     unsigned total;
     const unsigned off = block_prefix((unsigned)nl, total);
+    lv_off[threadIdx.x] = off;
     if (threadIdx.x == 0) run0 = total ? atomicAdd(&D.counters[0], total) : 0u;
     __syncthreads();
-    if (nl == 0) return;
-    const unsigned mid0 = run0 + off;
-    const int nc = D.cnt[c];
-    const long base = (long)c * D.cap;
-    const MvRow mv = mv_row(D, base, nc, rows);
-    int i = 0, n = nc + D.nearly[c], cur = 0, rank = 0;
-    while (i < n)
+    if (total == 0) return;
+    const bool listed = total <= LVMAX;
+    if (nl != 0)
     {
-        if (cur < nc && mv[cur] != MV_STAY)
+        const int nc = D.cnt[c];
+        const long base = (long)c * D.cap;
+        const MoveFlags mv = move_flags(D, base, nc);
+        int i = 0, n = nc + D.nearly[c], cur = 0, rank = 0;
+        while (i < n)
         {
-            long s = base + cur;
-            int dst = mv[cur];
-            if (dst == MV_DELETE)
-                atomicAdd(&D.counters[1], 1u);
-            else
+            if (cur < nc && mv.leaves(cur))
             {
-                const int mid = (int)(mid0 + rank);
-                int code = D.code[s];
-                double x = D.x[s], y = D.y[s], z = D.z[s];
-                /* ccopy_for_migrate, md-ghost.c:132-140: x += L (lower face) or x += -L (upper) */
-                if ((code & 3) == 1) x += D.L0;
-                else if ((code & 3) == 2) x += -D.L0;
-                if (((code >> 2) & 3) == 1) y += D.L1;
-                else if (((code >> 2) & 3) == 2) y += -D.L1;
-                if (((code >> 4) & 3) == 1) z += D.L2;
-                else if (((code >> 4) & 3) == 2) z += -D.L2;
-                D.mvx[mid] = x; D.mvy[mid] = y; D.mvz[mid] = z;
-                D.mvvx[mid] = D.vx[s]; D.mvvy[mid] = D.vy[s]; D.mvvz[mid] = D.vz[s];
-                D.mvgrp[mid] = D.grp[s]; D.mvaid[mid] = D.aid[s]; D.mvkind[mid] = D.kind[s];
-                unsigned long long key = ((unsigned long long)code << 58) |
-                                         ((unsigned long long)c << 16) | (unsigned long long)rank;
-                int pos = atomicAdd(&D.fill[dst], 1);
-                if (pos < D.cap)
+                if (listed)
                 {
-                    D.bkey[(long)dst * D.cap + pos] = key;
-                    D.bid[(long)dst * D.cap + pos] = mid;
+                    lv_slot[off + rank] = cur;
+                    lv_cell[off + rank] = (uint8_t)threadIdx.x;
                 }
                 else
-                    set_err(D, FMDX_DEV_ERR_OVERFLOW);
+                    stage_leaver(D, c, base + cur, (int)(run0 + off + rank), rank);
+                rank++;
+                n--;
+                if (n != i) cur = n;
             }
-            rank++;
-            n--;
-            if (n != i) cur = n;
+            else
+            {
+                i++;
+                cur = i;
+            }
         }
-        else
-        {
-            i++;
-            cur = i;
-        }
+    }
+    if (!listed) return;
+    __syncthreads();
+    for (unsigned j = threadIdx.x; j < total; j += TPB)
+    {
+        const int t = lv_cell[j];
+        const int cc = blk * TPB + t;
+        stage_leaver(D, cc, (long)cc * D.cap + lv_slot[j], (int)(run0 + j), (int)(j - lv_off[t]));
     }
 }
 DEF_KERNELS(bin_rank)
 
-/* place stayers, early and late arrivals of every affected cell */
-static __device__ int bin_apply_cell(const KDesc &D, int c, int nl, int na, int (*rows)[MVL + 1]);
+/* place stayers, early and late arrivals of every affected cell: the walks sort each
+   cell's arrival bucket (in LDS when the block's arrivals fit ARMAX) and list the cell's
+   moves, which all threads then carry out in two stages: first the moves within the
+   cell's surviving prefix (stayers into holes, early arrivals), whose sources (slots at or
+   beyond the prefix, or staging) and destinations never meet, then the late arrivals
+   appended after the prefix, which may land on slots the first stage read.  A cell whose
+   moves do not fit the block's list (MOVMAX) makes them itself, in the walk's order. */
+#define ARMAX 1024
+#define MOVMAX 2048
+struct MoveList
+{
+    int2 *m;            /* (cell within block | destination slot << 8 | staged << 20 | late << 21, source),
+                           or x = -1: an entry of a reservation that did not fit */
+    unsigned *n;
+};
+
+static __device__ int bin_apply_cell(const KDesc &D, int c, int nl, int na, unsigned long long *akey, int *amid,
+                                     const MoveList &L);
+
+static __device__ __forceinline__ void do_move(const KDesc &D, int blk, int2 m)
+{
+    const int c = blk * TPB + (m.x & 255);
+    const long base = (long)c * D.cap;
+    const int dst = (m.x >> 8) & 0xfff;
+    if ((m.x >> 20) & 1)
+    {
+        int cx, cy, cz;
+        cell_coords(D, c, cx, cy, cz);
+        copy_staged(D, m.y, base + dst, cx, cy, cz);
+    }
+    else
+        copy_slot(D, base + m.y, base + dst);
+}
 
 static __device__ void b_bin_apply(const KDesc &D, int blk)
 {
-    __shared__ int rows[TPB][MVL + 1];
+    __shared__ unsigned long long akey[ARMAX];
+    __shared__ int amid[ARMAX];
+    __shared__ int2 mov[MOVMAX];
+    __shared__ unsigned nmov;
     const int c = blk * TPB + threadIdx.x;
     const int nl = c < D.ncell ? D.nleave[c] : 0, na = c < D.ncell ? D.narr[c] : 0;
+    const int nb = na < D.cap ? na : D.cap;
+    unsigned atot;
+    const unsigned aoff = block_prefix((unsigned)nb, atot);
+    if (threadIdx.x == 0) nmov = 0u;
+    __syncthreads();
+    const bool lds_sort = atot <= ARMAX;
+    const MoveList L = {mov, &nmov};
     int total = 0;
-    if (nl != 0 || na != 0) total = bin_apply_cell(D, c, nl, na, rows);
+    if (nl != 0 || na != 0)
+        total = bin_apply_cell(D, c, nl, na, lds_sort ? akey + aoff : nullptr, lds_sort ? amid + aoff : nullptr, L);
+    __syncthreads();
+    const unsigned nm = nmov < MOVMAX ? nmov : MOVMAX;
+    for (int late = 0; late < 2; late++)
+    {
+        if (late) __syncthreads();   /* the slots the late arrivals land on have been read */
+        for (unsigned j = threadIdx.x; j < nm; j += TPB)
+        {
+            const int2 m = mov[j];
+            if (m.x >= 0 && ((m.x >> 21) & 1) == late) do_move(D, blk, m);
+        }
+    }
     /* the fullest cell: one atomic per block (a per-cell atomic on the shared word serialises) */
     unsigned m = (unsigned)total;
     for (int o = WAVE / 2; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor(m, o));
@@ -553,57 +636,111 @@ static __device__ void b_bin_apply(const KDesc &D, int blk)
     }
 }
 
-static __device__ int bin_apply_cell(const KDesc &D, int c, int nl, int na, int (*rows)[MVL + 1])
+static __device__ int bin_apply_cell(const KDesc &D, int c, int nl, int na, unsigned long long *akey, int *amid,
+                                     const MoveList &L)
 {
     int cx, cy, cz;
     cell_coords(D, c, cx, cy, cz);
     const int nc = D.cnt[c];
     const int A = D.nearly[c];
     const long base = (long)c * D.cap;
-    const MvRow mv = mv_row(D, base, nc, rows);
+    const MoveFlags mv = move_flags(D, base, nc);
     const int nb = na < D.cap ? na : D.cap;
     /* insertion sort of the arrival bucket by key */
-    for (int a = 1; a < nb; a++)
+    if (akey)
     {
-        unsigned long long k = D.bkey[base + a];
-        int id = D.bid[base + a];
-        int b = a - 1;
-        while (b >= 0 && D.bkey[base + b] > k)
+        for (int a = 0; a < nb; a++)
         {
-            D.bkey[base + b + 1] = D.bkey[base + b];
-            D.bid[base + b + 1] = D.bid[base + b];
-            b--;
+            akey[a] = D.bkey[base + a];
+            amid[a] = D.bid[base + a];
         }
-        D.bkey[base + b + 1] = k;
-        D.bid[base + b + 1] = id;
-    }
-    int i = 0, n = nc + A, cur = 0;
-    while (i < n)
-    {
-        if (cur < nc && mv[cur] != MV_STAY)
+        for (int a = 1; a < nb; a++)
         {
-            n--;
-            if (n != i) cur = n;
+            const unsigned long long k = akey[a];
+            const int id = amid[a];
+            int b = a - 1;
+            while (b >= 0 && akey[b] > k)
+            {
+                akey[b + 1] = akey[b];
+                amid[b + 1] = amid[b];
+                b--;
+            }
+            akey[b + 1] = k;
+            amid[b + 1] = id;
+        }
+    }
+    else
+        for (int a = 1; a < nb; a++)
+        {
+            unsigned long long k = D.bkey[base + a];
+            int id = D.bid[base + a];
+            int b = a - 1;
+            while (b >= 0 && D.bkey[base + b] > k)
+            {
+                D.bkey[base + b + 1] = D.bkey[base + b];
+                D.bid[base + b + 1] = D.bid[base + b];
+                b--;
+            }
+            D.bkey[base + b + 1] = k;
+            D.bid[base + b + 1] = id;
+        }
+    auto staged = [&](int q) { return akey ? amid[q] : D.bid[base + q]; };
+    /* the replay (misc.c:279-303 with the early arrivals appended): act(destination, staged,
+       source, late) for every atom that changes slot; returns the count before the late
+       arrivals */
+    auto walk = [&](auto act) {
+        int i = 0, n = nc + A, cur = 0;
+        while (i < n)
+        {
+            if (cur < nc && mv.leaves(cur))
+            {
+                n--;
+                if (n != i) cur = n;
+            }
+            else
+            {
+                if (cur >= nc)
+                {
+                    int q = cur - nc;
+                    if (q < nb && i < D.cap) act(i, true, q, false);
+                }
+                else if (cur != i)
+                    act(i, false, cur, false);
+                i++;
+                cur = i;
+            }
+        }
+        int t = n;
+        for (int q = A; q < na; q++)
+        {
+            if (q < nb && t < D.cap) act(t, true, q, true);
+            t++;
+        }
+        return t;
+    };
+    int nmv = 0;
+    const int total0 = walk([&](int, bool, int, bool) { nmv++; });
+    if (nmv)
+    {
+        const unsigned j0 = atomicAdd(L.n, (unsigned)nmv);
+        if (j0 + (unsigned)nmv <= MOVMAX)
+        {
+            unsigned j = j0;
+            walk([&](int dst, bool st, int src, bool late) {
+                L.m[j++] = make_int2((int)(threadIdx.x & 255) | (dst << 8) | (st ? 1 << 20 : 0) | (late ? 1 << 21 : 0),
+                                     st ? staged(src) : src);
+            });
         }
         else
         {
-            if (cur >= nc)
-            {
-                int q = cur - nc;
-                if (q < nb && i < D.cap) copy_staged(D, D.bid[base + q], base + i, cx, cy, cz);
-            }
-            else if (cur != i)
-                copy_slot(D, base + cur, base + i);
-            i++;
-            cur = i;
+            for (unsigned j = j0; j < MOVMAX; j++) L.m[j] = make_int2(-1, 0);
+            walk([&](int dst, bool st, int src, bool) {
+                if (st) copy_staged(D, staged(src), base + dst, cx, cy, cz);
+                else copy_slot(D, base + src, base + dst);
+            });
         }
     }
-    int total = n;
-    for (int q = A; q < na; q++)
-    {
-        if (q < nb && total < D.cap) copy_staged(D, D.bid[base + q], base + total, cx, cy, cz);
-        total++;
-    }
+    int total = total0;
     if (total > D.cap)
     {
         set_err(D, FMDX_DEV_ERR_OVERFLOW);
