@@ -398,40 +398,49 @@ static __device__ __forceinline__ void copy_staged(const KDesc &D, int mid, long
 }
 
 /* The replays below walk a cell's move flags in a data-dependent order.  A cell of up to
-   MVL atoms first turns its flags into a bit mask (bit k: slot k leaves) with independent
-   16-byte loads (the library's slot capacities are multiples of 32, so the loads are
-   aligned and stay inside the cell); larger cells, and capacities set by a caller to other
+   64 atoms first turns its flags into a bit mask (bit k: slot k leaves) with independent
+   16-byte loads, two batches of up to 8 (slot capacities that are multiples of 4 keep the
+   loads aligned and inside the cell); larger cells, and capacities set by a caller to other
    values, read the flags from HBM at each step of the walk.  The walks only decide which
    atom goes where: the atoms are then staged or moved by all threads of the block, one
    atom each, so that no thread carries a chain of dependent copies (a slowest cell's chain
-   set the time of the whole re-binning).  A cell's moves read and write disjoint slots
-   (sources at or beyond its new count, destinations below it), so they may run in any
-   order. */
-#define MVL 32
+   set the time of the whole re-binning). */
 struct MoveFlags
 {
     const int *g;
-    unsigned m;
+    unsigned long long m;
     bool local;
-    __device__ __forceinline__ bool leaves(int k) const { return local ? ((m >> k) & 1u) != 0u : g[k] != MV_STAY; }
+    __device__ __forceinline__ bool leaves(int k) const
+    {
+        return local ? ((m >> k) & 1ull) != 0ull : g[k] != MV_STAY;
+    }
 };
 
 static __device__ __forceinline__ MoveFlags move_flags(const KDesc &D, long base, int nc)
 {
     MoveFlags f;
     f.g = D.mv + base;
-    f.m = 0u;
-    f.local = nc <= MVL && D.cap >= MVL && (D.cap & 3) == 0;
+    f.m = 0ull;
+    f.local = nc <= 64 && (D.cap & 3) == 0;
     if (f.local)
     {
         const int4 *src = reinterpret_cast<const int4 *>(D.mv + base);
-        int4 q[MVL / 4];
+        const int nq = (nc + 3) >> 2;   /* 4 nq <= cap: the capacity is a multiple of 4 and holds nc */
 #pragma unroll
-        for (int k = 0; k < MVL / 4; k++) q[k] = src[k];
+        for (int h = 0; h < 2; h++)
+        {
+            int4 q[8];
 #pragma unroll
-        for (int k = 0; k < MVL / 4; k++)
-            f.m |= ((unsigned)(q[k].x != MV_STAY) | ((unsigned)(q[k].y != MV_STAY) << 1) |
-                    ((unsigned)(q[k].z != MV_STAY) << 2) | ((unsigned)(q[k].w != MV_STAY) << 3)) << (4 * k);
+            for (int k = 0; k < 8; k++)
+                if (8 * h + k < nq) q[k] = src[8 * h + k];
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                if (8 * h + k < nq)
+                    f.m |= (unsigned long long)((unsigned)(q[k].x != MV_STAY) | ((unsigned)(q[k].y != MV_STAY) << 1) |
+                                                ((unsigned)(q[k].z != MV_STAY) << 2) |
+                                                ((unsigned)(q[k].w != MV_STAY) << 3))
+                           << (4 * (8 * h + k));
+        }
     }
     return f;
 }
@@ -1002,85 +1011,137 @@ __global__ __launch_bounds__(TPB) void kb_touch(const KDesc *descs, const int2 *
     if (t) list[base + __popcll(m & ((1ull << lane) - 1ull))] = it;
 }
 
-/* After a re-binning, over the (system, block of TPB cells) items: each occupied cell
-   into its system's force-pass list (and density-pass list, EAM), each group of CPB
-   cells holding an atom into the O(N) list.  An empty cell gets the words the force
-   passes write for one (no pair or embedding energy, no recorded hits) and an empty
-   group the zero thermo partials the kick writes for it, so that the reductions over
-   every cell and group read what they read without the compaction; each system's
-   mover counter is reset for the next re-binning.  A workgroup reserves its run of each
-   list with one atomic (one per wave serialised on the list lengths) and writes its
-   items in cell order.  The order of a list decides only which workgroup takes an item,
-   never a result. */
-static __device__ __forceinline__ void b_occ(const BatchPlan &P, int2 it)
+/* After a re-binning, over the (system, block of TPB cells) items, OCC_K consecutive
+   items per workgroup: each occupied cell into its system's force-pass list (and
+   density-pass list, EAM), each group of CPB cells holding an atom into the O(N) list.
+   An empty cell gets the words the force passes write for one (no pair or embedding
+   energy, no recorded hits) and an empty group the zero thermo partials the kick writes
+   for it, so that the reductions over every cell and group read what they read without
+   the compaction; each system's mover counter is reset for the next re-binning.  A
+   workgroup reserves its run of each list it adds to with one atomic (the reservations
+   serialise on the list lengths) and writes its items in cell order.  The order of a
+   list decides only which workgroup takes an item, never a result. */
+#define OCC_K 4
+__global__ __launch_bounds__(TPB) void kb_occ(BatchMap M, BatchPlan P)
 {
-    __shared__ int wc[2][TPB / WAVE];
-    __shared__ int run[3];
-    const KDesc &D = P.descs[it.x];
-    const int c = it.y * TPB + (int)threadIdx.x;
-    const bool in = c < D.ncell;
-    const int n = in ? D.cnt[c] : 0;
-    const bool eam = D.has_eam;
-    if (in && n == 0)
-    {
-        D.cell_pe[c] = 0.0;
-        if (eam)
-        {
-            D.cell_emb[c] = 0.0;
-            D.htrips[c] = -1;
-        }
-    }
-    if (it.y == 0 && threadIdx.x == 0) D.counters[0] = 0;
-    /* the group of lane l is lanes l & ~(CPB-1) .. | (CPB-1) of the same wave; a group
-       exists when its first cell does */
+    __shared__ int wc[OCC_K][2][TPB / WAVE];
+    __shared__ int run[OCC_K][3];
+    __shared__ int lst[OCC_K][3];   /* list index of each item's three appends, -1: none */
     const int lane = (int)(threadIdx.x % WAVE), w = (int)(threadIdx.x / WAVE);
-    const unsigned long long occ = __ballot(n > 0);
     const bool lead = (lane % CPB) == 0;
-    const bool gocc = ((occ >> (lane & ~(CPB - 1))) & ((1ull << CPB) - 1ull)) != 0ull;
-    const int g = c / CPB;
-    const unsigned long long gm = __ballot(lead && in && gocc);
-    if (lead && in && !gocc)
-        for (int i = 0; i < 5; i++) D.part[(long)g * 8 + i] = 0.0;
-    if (lane == 0)
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int n[OCC_K];
+    unsigned long long occ[OCC_K], gm[OCC_K];
+    int2 it[OCC_K];
+#pragma unroll
+    for (int k = 0; k < OCC_K; k++)
     {
-        wc[0][w] = __popcll(occ);
-        wc[1][w] = __popcll(gm);
+        const int idx = (int)blockIdx.x * OCC_K + k;
+        n[k] = 0;
+        occ[k] = gm[k] = 0ull;
+        it[k] = make_int2(-1, 0);
+        if (idx < P.n_cellblk)   /* uniform */
+        {
+            it[k] = M.items[idx];
+            const KDesc &D = P.descs[it[k].x];
+            const int c = it[k].y * TPB + (int)threadIdx.x;
+            const bool in = c < D.ncell;
+            n[k] = in ? D.cnt[c] : 0;
+            if (in && n[k] == 0)
+            {
+                D.cell_pe[c] = 0.0;
+                if (D.has_eam)
+                {
+                    D.cell_emb[c] = 0.0;
+                    D.htrips[c] = -1;
+                }
+            }
+            if (it[k].y == 0 && threadIdx.x == 0) D.counters[0] = 0;
+            /* the group of lane l is lanes l & ~(CPB-1) .. | (CPB-1) of the same wave; a group
+               exists when its first cell does */
+            occ[k] = __ballot(n[k] > 0);
+            const bool gocc = ((occ[k] >> (lane & ~(CPB - 1))) & ((1ull << CPB) - 1ull)) != 0ull;
+            gm[k] = __ballot(lead && in && gocc);
+            if (lead && in && !gocc)
+                for (int i = 0; i < 5; i++) D.part[(long)(c / CPB) * 8 + i] = 0.0;
+            if (threadIdx.x == 0)
+            {
+                const int cat = P.sys_cat[it[k].x];
+                lst[k][0] = 1 + cat;
+                lst[k][1] = D.has_eam ? 5 + cat : -1;
+                lst[k][2] = 0;
+            }
+        }
+        else if (threadIdx.x == 0)
+            lst[k][0] = lst[k][1] = lst[k][2] = -1;
+        if (lane == 0)
+        {
+            wc[k][0][w] = __popcll(occ[k]);
+            wc[k][1][w] = __popcll(gm[k]);
+        }
     }
     __syncthreads();
     if (threadIdx.x == 0)
     {
-        int t0 = 0, t1 = 0;
-        for (int k = 0; k < TPB / WAVE; k++)
+        /* each item's count per append, then one reservation per list the workgroup adds to */
+        int cnt[OCC_K][3];
+        for (int k = 0; k < OCC_K; k++)
         {
-            t0 += wc[0][k];
-            t1 += wc[1][k];
+            int t0 = 0, t1 = 0;
+            for (int v = 0; v < TPB / WAVE; v++)
+            {
+                t0 += wc[k][0][v];
+                t1 += wc[k][1][v];
+            }
+            cnt[k][0] = t0;
+            cnt[k][1] = t0;
+            cnt[k][2] = t1;
         }
-        const int cat = P.sys_cat[it.x];
-        run[0] = t0 ? atomicAdd(&P.occ_n[1 + cat], t0) : 0;
-        run[1] = t0 && eam ? atomicAdd(&P.occ_n[5 + cat], t0) : 0;
-        run[2] = t1 ? atomicAdd(&P.occ_n[0], t1) : 0;
+        for (int k = 0; k < OCC_K; k++)
+            for (int a = 0; a < 3; a++)
+            {
+                const int l = lst[k][a];
+                if (l < 0) continue;
+                /* first item with this list: reserve for every item that shares it */
+                bool seen = false;
+                for (int k2 = 0; k2 < k; k2++)
+                    for (int a2 = 0; a2 < 3; a2++) seen |= lst[k2][a2] == l;
+                if (seen) continue;
+                int tot = 0;
+                for (int k2 = k; k2 < OCC_K; k2++)
+                    for (int a2 = 0; a2 < 3; a2++)
+                        if (lst[k2][a2] == l) tot += cnt[k2][a2];
+                int at = tot ? atomicAdd(&P.occ_n[l], tot) : 0;
+                for (int k2 = k; k2 < OCC_K; k2++)
+                    for (int a2 = 0; a2 < 3; a2++)
+                        if (lst[k2][a2] == l)
+                        {
+                            run[k2][a2] = at;
+                            at += cnt[k2][a2];
+                        }
+            }
     }
     __syncthreads();
-    const unsigned long long below = (1ull << lane) - 1ull;
-    int p0 = 0, p1 = 0;
-    for (int k = 0; k < w; k++)
+#pragma unroll
+    for (int k = 0; k < OCC_K; k++)
     {
-        p0 += wc[0][k];
-        p1 += wc[1][k];
+        if (it[k].x < 0) continue;   /* uniform */
+        int p0 = 0, p1 = 0;
+        for (int v = 0; v < w; v++)
+        {
+            p0 += wc[k][0][v];
+            p1 += wc[k][1][v];
+        }
+        const int c = it[k].y * TPB + (int)threadIdx.x;
+        if (n[k] > 0)
+        {
+            const int cat = lst[k][0] - 1;
+            const int q = p0 + __popcll(occ[k] & below);
+            P.occ_cell0[cat][run[k][0] + q] = make_int2(it[k].x, c);
+            if (lst[k][1] >= 0) P.occ_cell1[cat][run[k][1] + q] = make_int2(it[k].x, c);
+        }
+        if ((gm[k] >> lane) & 1ull) P.occ_slot[run[k][2] + p1 + __popcll(gm[k] & below)] = make_int2(it[k].x, c / CPB);
     }
-    if (n > 0)
-    {
-        const int cat = P.sys_cat[it.x];
-        const int k = p0 + __popcll(occ & below);
-        P.occ_cell0[cat][run[0] + k] = make_int2(it.x, c);
-        if (eam) P.occ_cell1[cat][run[1] + k] = make_int2(it.x, c);
-    }
-    if (lead && in && gocc) P.occ_slot[run[2] + p1 + __popcll(gm & below)] = make_int2(it.x, g);
-}
-
-__global__ __launch_bounds__(TPB) void kb_occ(BatchMap M, BatchPlan P)
-{
-    each_item_xy<false>(M, [&](int2 it) { b_occ(P, it); });
 }
 
 hipError_t launchb_occ(const BatchPlan &P, hipStream_t st)
@@ -1088,7 +1149,7 @@ hipError_t launchb_occ(const BatchPlan &P, hipStream_t st)
     if (P.nsys == 0 || !P.compact) return hipSuccess;
     BatchMap mc = {P.descs, P.cellblk_items, nullptr, 0};
     hipLaunchKernelGGL(kb_occ_clear, dim3(1), dim3(64), 0, st, P.occ_n, 9);
-    hipLaunchKernelGGL(kb_occ, dim3(P.n_cellblk), dim3(TPB), 0, st, mc, P);
+    hipLaunchKernelGGL(kb_occ, dim3((P.n_cellblk + OCC_K - 1) / OCC_K), dim3(TPB), 0, st, mc, P);
     return hipGetLastError();
 }
 
