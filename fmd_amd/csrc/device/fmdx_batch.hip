@@ -47,6 +47,7 @@ struct fmdx_batch
     std::vector<double> hthermo;
     int hocc[16];              /* lengths of the compacted lists, as last read */
     int compact_mode;          /* FMDX_COMPACT_* */
+    int ncells;                /* cells of all members */
     BatchPlan plan;
     int dirty;
     int use_graph;
@@ -135,6 +136,7 @@ static int fit_grids(fmdx_batch_t *b, int force)
     };
     int changed = fit(P.g_slot, b->hocc[0], P.n_slot);
     changed |= fit(P.g_bin, force ? P.n_cellblk : b->hocc[9], P.n_cellblk);   /* no count before a step */
+    changed |= fit(P.g_all, (b->hocc[10] + CPB - 1) / CPB, (b->ncells + CPB - 1) / CPB);   /* CPB cells a pass */
     for (int c = 0; c < 4; c++)
     {
         changed |= fit(P.g_cell0[c], b->hocc[1 + c], P.n_cell0[c]);
@@ -203,8 +205,12 @@ static int rebuild(fmdx_batch_t *b)
     /* the compacted lists hold at most every slot group and cell of the full ones */
     BatchPlan &P = b->plan;
     P.compact = FMDX_COMPACT && n > 0 && b->compact_mode != FMDX_COMPACT_OFF;
-    size_t nocc = slot.size() + cblk.size();
-    for (int c = 0; c < 4; c++) nocc += c0[c].size() + c1[c].size();
+    size_t nocc = slot.size() + cblk.size(), ncells = 0;
+    for (int c = 0; c < 4; c++)
+    {
+        nocc += 2 * c0[c].size() + c1[c].size();   /* the per-category lists and the list of all cells */
+        ncells += c0[c].size();
+    }
     auto occ_layout = [&](Carve &cv, size_t cap, size_t nsys) {
         cv(&P.occ_n, 16);
         cv((int **)&P.sys_cat, nsys);
@@ -233,6 +239,8 @@ static int rebuild(fmdx_batch_t *b)
             at += c1[c].size();
         }
         P.occ_bin = at;
+        P.occ_all = at + cblk.size();
+        b->ncells = (int)ncells;
         std::vector<int> cats(n);
         for (int i = 0; i < n; i++) cats[i] = force_category(b->sys[i]->D);
         CK(hipMemcpyAsync((void *)P.sys_cat, cats.data(), sizeof(int) * n, hipMemcpyHostToDevice, b->st));
@@ -262,7 +270,7 @@ static int rebuild(fmdx_batch_t *b)
     b->hthermo.assign((size_t)std::max(n, 1) * 16, 0.0);
     /* the first step's start and kick walk the lists of the members' current cells */
     CK(launchb_occ(P, b->st));
-    if (P.compact) CK(hipMemcpyAsync(b->hocc, P.occ_n, sizeof(int) * 10, hipMemcpyDeviceToHost, b->st));
+    if (P.compact) CK(hipMemcpyAsync(b->hocc, P.occ_n, sizeof(int) * 11, hipMemcpyDeviceToHost, b->st));
     CK(hipStreamSynchronize(b->st));
     if (b->compact_mode == FMDX_COMPACT_AUTO) P.compact = P.compact && sparse(b);
     fit_grids(b, 1);
@@ -326,7 +334,7 @@ static int collect(fmdx_batch_t *b)
     CK(hipMemcpyAsync(b->hstatus.data(), b->dstatus, sizeof(unsigned) * 8 * n, hipMemcpyDeviceToHost, b->st));
     CK(hipMemcpyAsync(b->hthermo.data(), b->dthermo, sizeof(double) * 16 * n, hipMemcpyDeviceToHost, b->st));
     if (b->plan.compact)
-        CK(hipMemcpyAsync(b->hocc, b->plan.occ_n, sizeof(int) * 10, hipMemcpyDeviceToHost, b->st));
+        CK(hipMemcpyAsync(b->hocc, b->plan.occ_n, sizeof(int) * 11, hipMemcpyDeviceToHost, b->st));
     CK(hipStreamSynchronize(b->st));
     if (b->plan.compact && b->compact_mode == FMDX_COMPACT_AUTO && !sparse(b))
     {

@@ -276,18 +276,20 @@ struct BatchPlan
     int n3;                      /* the LJ and Morse members evaluate each pair once (all have partner sums) */
     /* the step over occupied cells only: after each re-binning kb_occ writes the lists below
        and their lengths occ_n[0] (slot groups), [1 + cat] (force pass), [5 + cat] (density pass);
-       before each re-binning kb_touch lists the blocks of TPB cells it changes, occ_n[9] */
+       before each re-binning kb_touch lists the blocks of TPB cells it changes, occ_n[9];
+       occ_n[10]: every occupied cell, for the start step */
     int compact;
     /* grids that walk the compacted lists (multiples of 8), sized by the host from the list
        lengths it last read, with room to grow; a list that outgrows its grid is still walked
        whole, some workgroups taking more than one item */
-    int g_slot, g_cell0[4], g_cell1[4], g_bin;
+    int g_slot, g_cell0[4], g_cell1[4], g_bin, g_all;
     const int *sys_cat;          /* [nsys] pair category of each member's force pass */
     int *occ_n;                  /* [16] */
     int2 *occ_slot;              /* capacity n_slot */
     int2 *occ_cell0[4];          /* capacity n_cell0[cat] */
     int2 *occ_cell1[4];          /* capacity n_cell1[cat] */
     int2 *occ_bin;               /* capacity n_cellblk */
+    int2 *occ_all;               /* capacity the sum of n_cell0 */
 };
 
 /* kernel launchers (defined in the kernel translation units); the step

@@ -121,40 +121,68 @@ static __device__ void bin_flag(const KDesc &D, int c, long slot, double x, doub
     if (code == 0 && c < dest) atomicAdd(&D.nearly[dest], 1);
 }
 
-/* VelocityVerlet_startStep (integrators.c:40-75) fused with the binning flag */
-static __device__ void b_vv_start(const KDesc &D, int blk)
+/* VelocityVerlet_startStep (integrators.c:40-75) fused with the binning flag: one atom */
+static __device__ __forceinline__ void vv_start_atom(const KDesc &D, long slot, int c, double lambda)
 {
 #pragma clang fp contract(off)
-    if (blk == 0 && threadIdx.x == 0) D.counters[0] = 0;
-    double lambda = 1.0;
-    if (D.thermostat) lambda = sqrt(1 + D.dt / D.tau * (D.T0 / D.thermo[TH_T] - 1));
-    each_slot(D, blk, [&](long slot, int c) {
-        double x = D.x[slot], y = D.y[slot], z = D.z[slot];
-        if (D.ag < 0 || D.grp[slot] == D.ag)
+    double x = D.x[slot], y = D.y[slot], z = D.z[slot];
+    if (D.ag < 0 || D.grp[slot] == D.ag)
+    {
+        const double dt = D.dt;
+        const double h = D.hdtm[D.kind[slot]];
+        double vx = D.vx[slot], vy = D.vy[slot], vz = D.vz[slot];
+        if (D.thermostat)
         {
-            const double dt = D.dt;
-            const double h = D.hdtm[D.kind[slot]];
-            double vx = D.vx[slot], vy = D.vy[slot], vz = D.vz[slot];
-            if (D.thermostat)
-            {
-                vx *= lambda;
-                vy *= lambda;
-                vz *= lambda;
-            }
-            vx += h * D.fx[slot];
-            x += dt * vx;
-            vy += h * D.fy[slot];
-            y += dt * vy;
-            vz += h * D.fz[slot];
-            z += dt * vz;
-            D.vx[slot] = vx; D.vy[slot] = vy; D.vz[slot] = vz;
-            D.x[slot] = x; D.y[slot] = y; D.z[slot] = z;
+            vx *= lambda;
+            vy *= lambda;
+            vz *= lambda;
         }
-        keep_rel(D, c, slot, x, y, z);
-        bin_flag(D, c, slot, x, y, z);
-    });
+        vx += h * D.fx[slot];
+        x += dt * vx;
+        vy += h * D.fy[slot];
+        y += dt * vy;
+        vz += h * D.fz[slot];
+        z += dt * vz;
+        D.vx[slot] = vx; D.vy[slot] = vy; D.vz[slot] = vz;
+        D.x[slot] = x; D.y[slot] = y; D.z[slot] = z;
+    }
+    keep_rel(D, c, slot, x, y, z);
+    bin_flag(D, c, slot, x, y, z);
+}
+
+static __device__ __forceinline__ double berendsen_lambda(const KDesc &D)
+{
+#pragma clang fp contract(off)
+    return D.thermostat ? sqrt(1 + D.dt / D.tau * (D.T0 / D.thermo[TH_T] - 1)) : 1.0;
+}
+
+static __device__ void b_vv_start(const KDesc &D, int blk)
+{
+    if (blk == 0 && threadIdx.x == 0) D.counters[0] = 0;
+    const double lambda = berendsen_lambda(D);
+    each_slot(D, blk, [&](long slot, int c) { vv_start_atom(D, slot, c, lambda); });
 }
 DEF_KERNELS(vv_start)
+
+/* The start step over a list of occupied (system, cell) items (length *M.count), CPB
+   items per workgroup pass, LPC lanes per cell; the workgroups stride over the list.  A
+   batch whose members are mostly vacuum has few atoms in each 16-cell group, which the
+   group list would visit all of.  The mover counters are reset by kb_occ. */
+__global__ __launch_bounds__(TPB) void kb_vv_start_cells(BatchMap M)
+{
+    const int n = *M.count;
+    const int sub = (int)threadIdx.x / LPC, l = (int)threadIdx.x % LPC;
+    for (int i = (int)blockIdx.x * CPB + sub; i - sub < n; i += (int)gridDim.x * CPB)
+    {
+        if (i >= n) continue;
+        const int2 it = M.items[i];
+        const KDesc &D = M.descs[it.x];
+        const int c = it.y;
+        const double lambda = berendsen_lambda(D);
+        const int cnt = D.cnt[c];
+        for (int k = l; k < cnt; k += LPC) vv_start_atom(D, (long)c * D.cap + k, c, lambda);
+    }
+}
 
 /* SymplecticEuler_takeOneStep, first half (integrators.c:134-147) + binning flag */
 static __device__ void b_se_step(const KDesc &D, int blk)
@@ -979,10 +1007,10 @@ hipError_t launch_fill(double *p, long n, double v, hipStream_t st)
 
 /* ------------------------------------------- occupied cells of a batch */
 
-/* zero n counters (the list lengths); a kernel, so that a captured step holds no memset */
-__global__ __launch_bounds__(64) void kb_occ_clear(int *occ_n, int n)
+/* zero the list lengths whose bits are set; a kernel, so that a captured step holds no memset */
+__global__ __launch_bounds__(64) void kb_occ_clear(int *occ_n, unsigned mask)
 {
-    if ((int)threadIdx.x < n) occ_n[threadIdx.x] = 0;
+    if (threadIdx.x < 16 && ((mask >> threadIdx.x) & 1u)) occ_n[threadIdx.x] = 0;
 }
 
 /* Before a re-binning: the (system, block of TPB cells) items whose cells lose or gain
@@ -1013,7 +1041,8 @@ __global__ __launch_bounds__(TPB) void kb_touch(const KDesc *descs, const int2 *
 
 /* After a re-binning, over the (system, block of TPB cells) items, OCC_K consecutive
    items per workgroup: each occupied cell into its system's force-pass list (and
-   density-pass list, EAM), each group of CPB cells holding an atom into the O(N) list.
+   density-pass list, EAM) and the list of all occupied cells (the start step), each group
+   of CPB cells holding an atom into the O(N) list.
    An empty cell gets the words the force passes write for one (no pair or embedding
    energy, no recorded hits) and an empty group the zero thermo partials the kick writes
    for it, so that the reductions over every cell and group read what they read without
@@ -1025,8 +1054,8 @@ __global__ __launch_bounds__(TPB) void kb_touch(const KDesc *descs, const int2 *
 __global__ __launch_bounds__(TPB) void kb_occ(BatchMap M, BatchPlan P)
 {
     __shared__ int wc[OCC_K][2][TPB / WAVE];
-    __shared__ int run[OCC_K][3];
-    __shared__ int lst[OCC_K][3];   /* list index of each item's three appends, -1: none */
+    __shared__ int run[OCC_K][4];
+    __shared__ int lst[OCC_K][4];   /* list index of each item's four appends, -1: none */
     const int lane = (int)(threadIdx.x % WAVE), w = (int)(threadIdx.x / WAVE);
     const bool lead = (lane % CPB) == 0;
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -1070,10 +1099,11 @@ __global__ __launch_bounds__(TPB) void kb_occ(BatchMap M, BatchPlan P)
                 lst[k][0] = 1 + cat;
                 lst[k][1] = D.has_eam ? 5 + cat : -1;
                 lst[k][2] = 0;
+                lst[k][3] = 10;
             }
         }
         else if (threadIdx.x == 0)
-            lst[k][0] = lst[k][1] = lst[k][2] = -1;
+            lst[k][0] = lst[k][1] = lst[k][2] = lst[k][3] = -1;
         if (lane == 0)
         {
             wc[k][0][w] = __popcll(occ[k]);
@@ -1084,7 +1114,7 @@ __global__ __launch_bounds__(TPB) void kb_occ(BatchMap M, BatchPlan P)
     if (threadIdx.x == 0)
     {
         /* each item's count per append, then one reservation per list the workgroup adds to */
-        int cnt[OCC_K][3];
+        int cnt[OCC_K][4];
         for (int k = 0; k < OCC_K; k++)
         {
             int t0 = 0, t1 = 0;
@@ -1096,24 +1126,25 @@ __global__ __launch_bounds__(TPB) void kb_occ(BatchMap M, BatchPlan P)
             cnt[k][0] = t0;
             cnt[k][1] = t0;
             cnt[k][2] = t1;
+            cnt[k][3] = t0;
         }
         for (int k = 0; k < OCC_K; k++)
-            for (int a = 0; a < 3; a++)
+            for (int a = 0; a < 4; a++)
             {
                 const int l = lst[k][a];
                 if (l < 0) continue;
                 /* first item with this list: reserve for every item that shares it */
                 bool seen = false;
                 for (int k2 = 0; k2 < k; k2++)
-                    for (int a2 = 0; a2 < 3; a2++) seen |= lst[k2][a2] == l;
+                    for (int a2 = 0; a2 < 4; a2++) seen |= lst[k2][a2] == l;
                 if (seen) continue;
                 int tot = 0;
                 for (int k2 = k; k2 < OCC_K; k2++)
-                    for (int a2 = 0; a2 < 3; a2++)
+                    for (int a2 = 0; a2 < 4; a2++)
                         if (lst[k2][a2] == l) tot += cnt[k2][a2];
                 int at = tot ? atomicAdd(&P.occ_n[l], tot) : 0;
                 for (int k2 = k; k2 < OCC_K; k2++)
-                    for (int a2 = 0; a2 < 3; a2++)
+                    for (int a2 = 0; a2 < 4; a2++)
                         if (lst[k2][a2] == l)
                         {
                             run[k2][a2] = at;
@@ -1139,6 +1170,7 @@ __global__ __launch_bounds__(TPB) void kb_occ(BatchMap M, BatchPlan P)
             const int q = p0 + __popcll(occ[k] & below);
             P.occ_cell0[cat][run[k][0] + q] = make_int2(it[k].x, c);
             if (lst[k][1] >= 0) P.occ_cell1[cat][run[k][1] + q] = make_int2(it[k].x, c);
+            P.occ_all[run[k][3] + q] = make_int2(it[k].x, c);
         }
         if ((gm[k] >> lane) & 1ull) P.occ_slot[run[k][2] + p1 + __popcll(gm[k] & below)] = make_int2(it[k].x, c / CPB);
     }
@@ -1148,7 +1180,7 @@ hipError_t launchb_occ(const BatchPlan &P, hipStream_t st)
 {
     if (P.nsys == 0 || !P.compact) return hipSuccess;
     BatchMap mc = {P.descs, P.cellblk_items, nullptr, 0};
-    hipLaunchKernelGGL(kb_occ_clear, dim3(1), dim3(64), 0, st, P.occ_n, 9);
+    hipLaunchKernelGGL(kb_occ_clear, dim3(1), dim3(64), 0, st, P.occ_n, 0x5ffu);   /* all but the touched blocks' */
     hipLaunchKernelGGL(kb_occ, dim3((P.n_cellblk + OCC_K - 1) / OCC_K), dim3(TPB), 0, st, mc, P);
     return hipGetLastError();
 }
@@ -1176,10 +1208,16 @@ hipError_t launchb_vv_start(const BatchPlan &P, hipStream_t st)
     BatchMap ms, mc = {P.descs, P.cellblk_items, nullptr, 0};
     dim3 gs, gc(P.n_cellblk);
     slot_map(P, ms, gs);
-    hipLaunchKernelGGL(kb_vv_start, gs, dim3(TPB), 0, st, ms);
     if (P.compact)
     {
-        hipLaunchKernelGGL(kb_occ_clear, dim3(1), dim3(64), 0, st, P.occ_n + 9, 1);
+        const BatchMap ma = {P.descs, P.occ_all, P.occ_n + 10, 0};
+        hipLaunchKernelGGL(kb_vv_start_cells, dim3((unsigned)P.g_all), dim3(TPB), 0, st, ma);
+    }
+    else
+        hipLaunchKernelGGL(kb_vv_start, gs, dim3(TPB), 0, st, ms);
+    if (P.compact)
+    {
+        hipLaunchKernelGGL(kb_occ_clear, dim3(1), dim3(64), 0, st, P.occ_n, 1u << 9);
         hipLaunchKernelGGL(kb_touch, dim3((P.n_cellblk + TPB - 1) / TPB), dim3(TPB), 0, st, P.descs,
                            P.cellblk_items, P.n_cellblk, P.occ_bin, P.occ_n + 9);
         mc = {P.descs, P.occ_bin, P.occ_n + 9, 0};
