@@ -277,6 +277,31 @@ def batched_ensemble(nsys, steps, plain_sample, seed=3):
                 members_with_errors=errs)
 
 
+def collision_batch(njobs, steps, seed=4):
+    """The served leg's kernels without its request loop: njobs collision jobs (mixed) stepped
+    together as one batch, for profiling (the loop's admissions and retirements re-capture the
+    step; this batch replays one capture)."""
+    from fmd_amd import ensemble as E
+    specs = E.collision_specs(njobs, seed=seed, rate=1.0)
+    jobs = [E.collision_job(sp, F.FMDX_PREC_MIXED) for sp in specs]
+    b = F.Batch(F.FMDX_PREC_MIXED)
+    try:
+        for j in jobs:
+            b.add(j.dev, j.dt)
+        b.step(8)
+        t0 = time.perf_counter()
+        b.step(steps)
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        st = b.status()
+        natoms = sum(j.natoms for j in jobs)
+        return dict(workload="%d collision jobs (mixed) as one batch" % njobs, natoms=natoms, steps=steps,
+                    ms_per_step=ms, atom_steps_per_s=natoms / ms * 1e3, members_with_errors=sum(1 for t in st if t.err))
+    finally:
+        b.close()
+        for j in jobs:
+            j.free()
+
+
 def served_collisions(njobs, rates, seed=4):
     """Served two-cluster Morse Cu collision jobs through the continuous-batching loop in mixed
     precision, at each Poisson arrival rate, against the plain FP64 one-job-at-a-time loop (first
@@ -538,6 +563,9 @@ def main():
     ap.add_argument("--drift-steps", type=int, default=10000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain MIB-MiB device copy (counter calibration) and exit")
+    ap.add_argument("--collision-batch", type=int, default=0, metavar="JOBS",
+                    help="only step JOBS served-collision jobs together as one batch (the served leg's kernels, "
+                         "for profiling) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
@@ -547,6 +575,9 @@ def main():
         sys.exit(1)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.collision_batch:
+        print(json.dumps(collision_batch(args.collision_batch, args.steps)), flush=True)
         return
     prec = F.FMDX_PREC_FP64 if args.precision == "fp64" else F.FMDX_PREC_MIXED
     with tempfile.TemporaryDirectory() as td:
