@@ -24,9 +24,13 @@ def _specs(n, seed):
     return sp
 
 
-def test_served_jobs_equal_plain_runs():
+@pytest.mark.parametrize("builders", [0, 1])
+def test_served_jobs_equal_plain_runs(builders):
+    """with the jobs built on the loop's thread at quantum boundaries, or on a builder thread
+    while the batch steps"""
     specs = _specs(5, seed=11)
-    served = S.Server(E.collision_job, precision=F.FMDX_PREC_FP64, quantum=7, keep_positions=True).run(specs)
+    served = S.Server(E.collision_job, precision=F.FMDX_PREC_FP64, quantum=7, keep_positions=True,
+                      builders=builders).run(specs)
     plain = S.run_plain(specs, E.collision_job, keep_positions=True)
     assert len(served) == len(plain) == 5
     for a, b in zip(served, plain):
