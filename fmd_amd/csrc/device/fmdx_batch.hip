@@ -21,6 +21,13 @@
  * Between steps the host reads every member's status word with one copy
  * (kb_collect) to grow slot capacities before they can overflow and to
  * report members whose device error word is set.
+ *
+ * Members that are mostly vacuum (clusters in an open box) are stepped over
+ * their occupied cells only: after each re-binning the device compacts the
+ * cell, group and block lists (kb_occ, kb_touch in k_step.hip), and the
+ * captured step walks them with grids the host sizes from the list lengths
+ * it reads with the status words (fit_grids).  Empty cells and groups get the
+ * words the skipped kernels would have written, so every result is the same.
  */
 #include <algorithm>
 #include <vector>
@@ -114,10 +121,10 @@ static bool sparse(const fmdx_batch_t *b)
     return occ * 10 < all * 6;
 }
 
-/* Grids of the compacted lists from their lengths: room for a quarter more (and 64
-   items; none when tight), at most one workgroup per item of the full list.  A grid is resized when its
-   list outgrows it or shrinks below half of it; returns whether any grid changed (the
-   captured step then no longer matches). */
+/* Grids of the compacted lists from their lengths: room for a quarter more and 64 items
+   (none when tight), at most one workgroup per item of the full list.  A grid is resized
+   when its list outgrows it or needs less than half of it; returns whether any grid
+   changed (the captured step then no longer matches). */
 static int fit_grids(fmdx_batch_t *b, int force)
 {
     BatchPlan &P = b->plan;
@@ -244,7 +251,6 @@ static int rebuild(fmdx_batch_t *b)
         std::vector<int> cats(n);
         for (int i = 0; i < n; i++) cats[i] = force_category(b->sys[i]->D);
         CK(hipMemcpyAsync((void *)P.sys_cat, cats.data(), sizeof(int) * n, hipMemcpyHostToDevice, b->st));
-
     }
     /* members whose force passes evaluate each pair once need their partner sums */
     for (int i = 0; i < n; i++)
