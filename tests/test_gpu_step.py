@@ -121,3 +121,51 @@ def test_symplectic_euler_and_thermo(rng):
     t, e = d.read(), o.energies()
     assert abs(t.temperature - e["T"]) / e["T"] < 1e-9
     assert int(t.n_active) == e["n"]
+
+
+def test_single_atom_crosses_every_face(rng):
+    """One atom alone (no pair partner, zero force) flying through a periodic box: every step
+    is exact arithmetic on both sides, and the atom migrates through faces in all three
+    dimensions; the device matches the oracle bit for bit, as do a second, empty system and a
+    batch holding both."""
+    L = [4 * AR["a"]] * 3
+    x = np.array([[0.5 * L[0], 0.5 * L[1], 0.5 * L[2]]])
+    v = np.array([[1.7, -2.3, 3.1]]) * L[0] / 20 / 1e-2   # ~1/20 to 1/7 of the box per step
+    pot = [("lj", 0, 0, AR["sig"], AR["eps"], AR["rc"])]
+    o, d = build(L, [1, 1, 1], [AR["m"]], ["Ar"], pot, x, v=v)
+    for _ in range(30):
+        o.vv_start(1e-2)
+        d.vv_start(1e-2)
+        d.forces()
+        d.vv_finish(1e-2)
+        o.forces()
+        o.vv_finish(1e-2)
+        got = _state_equal(d, o)
+        assert np.array_equal(got["f"], np.zeros((1, 3)))
+        assert d.read().err == 0
+    # an empty system, alone and in a batch with a one-atom system
+    nc = o.nc
+    e = F.DeviceSystem(L, [1, 1, 1], list(nc), list(o.cellh), [AR["m"] / F.MASS_UNIT])
+    e.set_pair(0, 0, F.POT_LJ, AR["rc"] ** 2, 4.0 * AR["eps"], AR["sig"] ** 2, 0.0)
+    e.upload(np.zeros(int(np.prod(nc)), np.uint32), np.zeros((0, 3)), np.zeros((0, 3)))
+    e.forces()
+    e.thermo()
+    t = e.read()
+    assert t.err == 0 and t.pe == 0.0 and t.natoms == 0
+    o2, d2 = build(L, [1, 1, 1], [AR["m"]], ["Ar"], pot, x, v=v)
+    b = F.Batch(F.FMDX_PREC_FP64)
+    b.add(e, 1e-2)
+    b.add(d2, 1e-2)
+    b.step(12)
+    st = b.status()
+    assert st[0].err == 0 and st[0].natoms == 0 and st[1].err == 0 and st[1].natoms == 1
+    b.close()
+    o2.forces()
+    o2.thermo()
+    for _ in range(12):
+        o2.vv_start(1e-2)
+        o2.forces()
+        o2.vv_finish(1e-2)
+    _state_equal(d2, o2)
+    assert e.download()["x"].shape == (0, 3)
+    e.close()
