@@ -168,8 +168,9 @@ DEF_KERNELS(vv_start)
    items per workgroup pass, LPC lanes per cell; the workgroups stride over the list.  A
    batch whose members are mostly vacuum has few atoms in each 16-cell group, which the
    group list would visit all of.  The mover counters are reset by kb_occ. */
-__global__ __launch_bounds__(TPB) void kb_vv_start_cells(BatchMap M)
+__global__ __launch_bounds__(TPB) void kb_vv_start_cells(BatchMap M, int *touched)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *touched = 0;   /* kb_touch's list starts empty */
     const int n = *M.count;
     const int sub = (int)threadIdx.x / LPC, l = (int)threadIdx.x % LPC;
     for (int i = (int)blockIdx.x * CPB + sub; i - sub < n; i += (int)gridDim.x * CPB)
@@ -1014,11 +1015,15 @@ __global__ __launch_bounds__(64) void kb_occ_clear(int *occ_n, unsigned mask)
 }
 
 /* Before a re-binning: the (system, block of TPB cells) items whose cells lose or gain
-   atoms, as the start step flagged them, into occ_bin (length occ_n[9], zeroed just
-   before; it stays until the next step, for the status read); the flags are cleared.  The re-binning kernels then skip the blocks
+   atoms, as the start step flagged them, into occ_bin (length occ_n[9], zeroed by the start
+   step; it stays until the next step, for the status read); the flags are cleared.  The re-binning kernels then skip the blocks
    where nothing moves, which they would leave as they are. */
-__global__ __launch_bounds__(TPB) void kb_touch(const KDesc *descs, const int2 *items, int n, int2 *list, int *len)
+__global__ __launch_bounds__(TPB) void kb_touch(const KDesc *descs, const int2 *items, int n, int2 *list, int *occ_n)
 {
+    int *const len = occ_n + 9;
+    /* the lists kb_occ is about to write start empty (their readers of the last step and
+       the start step, which read occ_n[10], have finished) */
+    if (blockIdx.x == 0 && threadIdx.x < 11 && threadIdx.x != 9) occ_n[threadIdx.x] = 0;
     const int i = (int)(blockIdx.x * TPB + threadIdx.x);
     bool t = false;
     int2 it = make_int2(0, 0);
@@ -1176,12 +1181,18 @@ __global__ __launch_bounds__(TPB) void kb_occ(BatchMap M, BatchPlan P)
     }
 }
 
+static void launch_occ(const BatchPlan &P, hipStream_t st)
+{
+    BatchMap mc = {P.descs, P.cellblk_items, nullptr, 0};
+    hipLaunchKernelGGL(kb_occ, dim3((P.n_cellblk + OCC_K - 1) / OCC_K), dim3(TPB), 0, st, mc, P);
+}
+
+/* outside a step (a batch's set-up): every list length zeroed, then the lists */
 hipError_t launchb_occ(const BatchPlan &P, hipStream_t st)
 {
     if (P.nsys == 0 || !P.compact) return hipSuccess;
-    BatchMap mc = {P.descs, P.cellblk_items, nullptr, 0};
-    hipLaunchKernelGGL(kb_occ_clear, dim3(1), dim3(64), 0, st, P.occ_n, 0x5ffu);   /* all but the touched blocks' */
-    hipLaunchKernelGGL(kb_occ, dim3((P.n_cellblk + OCC_K - 1) / OCC_K), dim3(TPB), 0, st, mc, P);
+    hipLaunchKernelGGL(kb_occ_clear, dim3(1), dim3(64), 0, st, P.occ_n, 0xffffu);
+    launch_occ(P, st);
     return hipGetLastError();
 }
 
@@ -1211,15 +1222,14 @@ hipError_t launchb_vv_start(const BatchPlan &P, hipStream_t st)
     if (P.compact)
     {
         const BatchMap ma = {P.descs, P.occ_all, P.occ_n + 10, 0};
-        hipLaunchKernelGGL(kb_vv_start_cells, dim3((unsigned)P.g_all), dim3(TPB), 0, st, ma);
+        hipLaunchKernelGGL(kb_vv_start_cells, dim3((unsigned)P.g_all), dim3(TPB), 0, st, ma, P.occ_n + 9);
     }
     else
         hipLaunchKernelGGL(kb_vv_start, gs, dim3(TPB), 0, st, ms);
     if (P.compact)
     {
-        hipLaunchKernelGGL(kb_occ_clear, dim3(1), dim3(64), 0, st, P.occ_n, 1u << 9);
         hipLaunchKernelGGL(kb_touch, dim3((P.n_cellblk + TPB - 1) / TPB), dim3(TPB), 0, st, P.descs,
-                           P.cellblk_items, P.n_cellblk, P.occ_bin, P.occ_n + 9);
+                           P.cellblk_items, P.n_cellblk, P.occ_bin, P.occ_n);
         mc = {P.descs, P.occ_bin, P.occ_n + 9, 0};
         gc = dim3((unsigned)P.g_bin);
     }
@@ -1227,7 +1237,8 @@ hipError_t launchb_vv_start(const BatchPlan &P, hipStream_t st)
     hipLaunchKernelGGL(kb_bin_apply, gc, dim3(TPB), 0, st, mc);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return launchb_occ(P, st);   /* the lists of the atoms' new cells */
+    if (P.compact) launch_occ(P, st);   /* the lists of the atoms' new cells (kb_touch zeroed their lengths) */
+    return hipGetLastError();
 }
 
 hipError_t launchb_vv_finish(const BatchPlan &P, hipStream_t st)
