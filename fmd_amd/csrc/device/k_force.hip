@@ -1812,14 +1812,15 @@ hipError_t launch_forces(const KDesc &D, hipStream_t st, int precision, bool rel
     return D.energy ? launch_reduce_pe(D, st) : hipSuccess;
 }
 
-/* one category's pass over a list: one workgroup per cell, and, on a compacted list, a
-   small looping launch for the cells beyond that grid (it exits at once when there are none) */
-#define GRID_MORE 256
+/* one category's pass over a list: one workgroup per cell, and, on a compacted list whose
+   grid is shorter than the full list, a small looping launch for the cells beyond that grid
+   (it exits at once when there are none) */
+#define GRID_MORE 64
 template <typename R, int PASS, int CAT, bool EN, bool N3>
-static void launchb_cat(const BatchMap &mc, dim3 g, size_t ppb, hipStream_t st)
+static void launchb_cat(const BatchMap &mc, dim3 g, int full, size_t ppb, hipStream_t st)
 {
     hipLaunchKernelGGL((kb_cell<R, PASS, CAT, EN, N3>), g, dim3(NT), ppb, st, mc);
-    if (mc.count)
+    if (mc.count && (int)g.x < full)
     {
         BatchMap mo = mc;
         mo.skip = (int)g.x;
@@ -1847,16 +1848,16 @@ static void launchb_cells(const BatchPlan &P, hipStream_t st, const int2 *const 
         }
         switch (cat)
         {
-            case CAT_EAM1: launchb_cat<R, PASS, CAT_EAM1, EN, false>(mc, g, ppb, st); break;
+            case CAT_EAM1: launchb_cat<R, PASS, CAT_EAM1, EN, false>(mc, g, n[cat], ppb, st); break;
             case CAT_LJ:
-                if (N3OK && P.n3) launchb_cat<R, PASS, CAT_LJ, EN, N3OK>(mc, g, ppb, st);
-                else launchb_cat<R, PASS, CAT_LJ, EN, false>(mc, g, ppb, st);
+                if (N3OK && P.n3) launchb_cat<R, PASS, CAT_LJ, EN, N3OK>(mc, g, n[cat], ppb, st);
+                else launchb_cat<R, PASS, CAT_LJ, EN, false>(mc, g, n[cat], ppb, st);
                 break;
             case CAT_MORSE:
-                if (N3OK && P.n3) launchb_cat<R, PASS, CAT_MORSE, EN, N3OK>(mc, g, ppb, st);
-                else launchb_cat<R, PASS, CAT_MORSE, EN, false>(mc, g, ppb, st);
+                if (N3OK && P.n3) launchb_cat<R, PASS, CAT_MORSE, EN, N3OK>(mc, g, n[cat], ppb, st);
+                else launchb_cat<R, PASS, CAT_MORSE, EN, false>(mc, g, n[cat], ppb, st);
                 break;
-            default: launchb_cat<R, PASS, CAT_GEN, EN, false>(mc, g, ppb, st); break;
+            default: launchb_cat<R, PASS, CAT_GEN, EN, false>(mc, g, n[cat], ppb, st); break;
         }
     }
     /* each pair once: the partner sums folded into the forces (members without them pass) */
