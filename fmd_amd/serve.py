@@ -72,6 +72,13 @@ class _Built:
         return self.job
 
 
+def _release(job):
+    try:
+        job.free()
+    except Exception:
+        pass
+
+
 class Server:
     """`builders` threads build arrived jobs (atoms, device system, upload) while the GPU runs
     the batch's steps: ctypes releases the GIL inside the library's calls, and a quantum of
@@ -154,7 +161,16 @@ class Server:
         finally:
             if pool:
                 pool.shutdown(wait=True)
-            b.close()
+            b.close()   # the members leave the batch
+            # after an exception: release the jobs still running or built and not yet admitted
+            for a in active:
+                _release(a[0])
+            for sp, fut, est in building:
+                try:
+                    job = fut.result()
+                except BaseException:
+                    continue
+                _release(job)
         return [results[k] for k in sorted(results)]
 
     def _estimate(self, sp):

@@ -122,3 +122,22 @@ def test_request_loop_schedule(monkeypatch, builders):
             assert not r.failed
     s = S.summarize(out)
     assert s["jobs"] == len(specs) and s["failed"] == 1
+
+
+@pytest.mark.parametrize("builders", [0, 1])
+def test_failed_build_releases_every_built_job(monkeypatch, builders):
+    """A build that raises ends the run with its exception, after every job already built,
+    admitted or not, has been released."""
+    monkeypatch.setattr(S, "Batch", _FakeBatch)
+    monkeypatch.setattr(S, "memory_of", lambda dev: 1 << 20)
+    specs = [dict(index=i, arrival=0.0, nsteps=50, natoms_hint=10) for i in range(5)]
+    made = {}
+
+    def build(sp, precision):
+        if sp["index"] == 3:
+            raise RuntimeError("broken build")
+        made[sp["index"]] = j = _FakeJob(sp)
+        return j
+    with pytest.raises(RuntimeError, match="broken build"):
+        S.Server(build, builders=builders, quantum=10).run(specs)
+    assert sorted(made) == [0, 1, 2, 4] and all(j.freed for j in made.values())
