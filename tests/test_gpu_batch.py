@@ -187,7 +187,8 @@ def test_compacted_cell_lists_same_bits(rng, cu01, precision):
     cells only, from lists the device rebuilds after each re-binning, gives the same bits as
     a step over every cell, also with grids exactly as long as the lists at the last status
     read, so that cells occupied since then are taken by the looping launch and the step is
-    captured again after a read that finds longer lists."""
+    captured again after a read that finds longer lists.  Members: LJ, Morse and EAM
+    clusters, and a two-kind EAM + LJ pair of clusters (the general pair category)."""
     c = CU_MORSE
     specs = [
         _cluster(rng, [("morse", 0, 0, c["D0"], c["alpha"], c["r0"], c["rc"])], c["m"], c["a"], 11.0, 64.0,
@@ -196,6 +197,16 @@ def test_compacted_cell_lists_same_bits(rng, cu01, precision):
         _cluster(rng, [("lj", 0, 0, AR["sig"], AR["eps"], AR["rc"])], AR["m"], AR["a"], 14.0, 70.0, 60.0,
                  (-8.0, 0.0, 12.0), name="Ar"),
     ]
+    # a Cu (EAM) cluster and an Ar (LJ) cluster in one open box: two kinds, the general category
+    cu = _cluster(rng, [], 63.546, 3.615, 8.0, 60.0, 900.0, (10.0, 0.0, 0.0))[5] - np.array([12.0, 0.0, 0.0])
+    ar = _cluster(rng, [], AR["m"], AR["a"], 9.0, 60.0, 60.0, (-10.0, 0.0, 0.0))[5] + np.array([12.0, 0.0, 0.0])
+    x = np.vstack([cu, ar])
+    v = np.vstack([maxwell_velocities(len(cu), 63.546, 900.0, rng) + [10.0, 0.0, 0.0],
+                   maxwell_velocities(len(ar), AR["m"], 60.0, rng) + [-10.0, 0.0, 0.0]])
+    kind = np.r_[np.zeros(len(cu)), np.ones(len(ar))].astype(np.int32)
+    hyb = ([60.0] * 3, [0, 0, 0], [63.546, AR["m"]], ["Cu", "Ar"],
+           [("eam", cu01, [(0, 0)]), ("lj", 1, 1, AR["sig"], AR["eps"], AR["rc"]), ("lj", 0, 1, 2.87, 0.0652, 2.5 * 2.87)],
+           x, v, kind)
     runs = {}
     for mode in (F.FMDX_COMPACT_OFF, F.FMDX_COMPACT_AUTO, F.FMDX_COMPACT_TIGHT):
         b = F.Batch(precision)
@@ -205,6 +216,10 @@ def test_compacted_cell_lists_same_bits(rng, cu01, precision):
             _, d = _make(sp, precision)
             b.add(d, 2e-3)
             ds.append(d)
+        L, pbc, m, names, pots, xh, vh, kh = hyb
+        _, d = build(L, pbc, m, names, pots, xh, v=vh, kind=kh, precision=precision)
+        b.add(d, 2e-3)
+        ds.append(d)
         for _ in range(3):
             b.step(16)
         st = [(t.temperature, t.pe, t.err) for t in b.status()]
