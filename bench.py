@@ -330,9 +330,12 @@ def served_collisions(njobs, rates, seed=4):
             ferr.append(mixed_force_error(j.f, F.FMDX_PREC_FP64))
         finally:
             j.free()
+    one = collision_batch(njobs, 48, seed=seed)
     return dict(workload="%d jobs per rate, Poisson arrivals; each two Morse-Cu FCC spheres R=15-50 A, 20 A gap, "
                          "10-50 A/ps, impact parameter 0-(R1+R2), vacuum box, 1000-4000 steps of 1 fs" % njobs,
                 rates=per_rate, plain_run_seconds=plain_s,
+                all_jobs_as_one_batch=dict(ms_per_step=one["ms_per_step"], atom_steps_per_s=one["atom_steps_per_s"],
+                                           natoms=one["natoms"], steps=one["steps"]),
                 max_rel_force_err_mixed_vs_fp64=max(e[0] for e in ferr),
                 rms_rel_force_err_mixed_vs_fp64=max(e[1] for e in ferr),
                 force_err_sample="initial states of the first %d jobs" % len(ferr))
