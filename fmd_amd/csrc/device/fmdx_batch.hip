@@ -234,6 +234,9 @@ static int rebuild(fmdx_batch_t *b)
         b->occ_cap = cap;
         b->occ_sys_cap = scap;
     }
+    /* source of an asynchronous copy below: it lives until the stream synchronisation
+       in upload_descs (or before an early return), like `all` */
+    std::vector<int> cats;
     if (P.compact)
     {
         /* one carve of the whole list area: slot groups first, then each category's lists */
@@ -248,7 +251,7 @@ static int rebuild(fmdx_batch_t *b)
         P.occ_bin = at;
         P.occ_all = at + cblk.size();
         b->ncells = (int)ncells;
-        std::vector<int> cats(n);
+        cats.resize(n);
         for (int i = 0; i < n; i++) cats[i] = force_category(b->sys[i]->D);
         CK(hipMemcpyAsync((void *)P.sys_cat, cats.data(), sizeof(int) * n, hipMemcpyHostToDevice, b->st));
     }
@@ -256,7 +259,11 @@ static int rebuild(fmdx_batch_t *b)
     for (int i = 0; i < n; i++)
     {
         const int e = ensure_n3(b->sys[i], b->precision);
-        if (e) return e;
+        if (e)
+        {
+            hipStreamSynchronize(b->st);   /* the copies from `all` and `cats` end first */
+            return e;
+        }
     }
     int r = upload_descs(b);
     if (r) return r;
