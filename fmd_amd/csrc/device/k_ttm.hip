@@ -56,18 +56,26 @@ static __device__ __forceinline__ void md_range(int i, int n, double tw, double 
     hi = hi < lo ? lo : (hi > nc - 1 ? nc - 1 : hi);
 }
 
-/* sum of lane values in a fixed order (tree over the workgroup); every lane gets the result */
-static __device__ __forceinline__ double block_sum(double v, double *sh)
+/* sums of lane values in a fixed order (tree over the workgroup), N values at once
+   (sh holds N x TTPB) with one barrier per tree level; every lane gets the results */
+template <int N>
+static __device__ __forceinline__ void block_sums(double (&v)[N], double *sh)
 {
     __syncthreads();
-    sh[threadIdx.x] = v;
+#pragma unroll
+    for (int i = 0; i < N; i++) sh[i * TTPB + threadIdx.x] = v[i];
     __syncthreads();
     for (int o = TTPB / 2; o > 0; o >>= 1)
     {
-        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        if ((int)threadIdx.x < o)
+        {
+#pragma unroll
+            for (int i = 0; i < N; i++) sh[i * TTPB + threadIdx.x] += sh[i * TTPB + threadIdx.x + o];
+        }
         __syncthreads();
     }
-    return sh[0];
+#pragma unroll
+    for (int i = 0; i < N; i++) v[i] = sh[i * TTPB];
 }
 
 /* turi index of every atom, into the binning scratch word mv (free between
@@ -100,7 +108,7 @@ template <bool G3>
 __global__ __launch_bounds__(TTPB) void k_ttm_sums(KDesc D, TtmState S)
 {
 #pragma clang fp contract(off)
-    __shared__ double sh[TTPB];
+    __shared__ double sh[4 * TTPB];
     const int a = blockIdx.x;
     const int iz = a % S.nzg, col = a / S.nzg;
     int x0 = 0, x1 = D.nc0 - 1, y0 = 0, y1 = D.nc1 - 1, z0, z1;
@@ -114,41 +122,144 @@ __global__ __launch_bounds__(TTPB) void k_ttm_sums(KDesc D, TtmState S)
     if (iz == S.nzg - 1) { z1 = D.nc2 - 1; }
     const int nx = x1 - x0 + 1, ny = y1 - y0 + 1, nz = z1 - z0 + 1;
     const int items = nx * ny * nz * LPC;
-    /* item k: MD cell k / LPC of the box, atoms k % LPC, + LPC, ... of it */
-    auto visit = [&](auto f) __attribute__((always_inline)) {
-        for (int k = threadIdx.x; k < items; k += TTPB)
-        {
-            const int b = k / LPC, cz = z0 + b % nz, t = b / nz;
-            const int c = ((x0 + t / ny) * D.nc1 + (y0 + t % ny)) * D.nc2 + cz;
-            const long base = (long)c * D.cap;
-            for (int j = k % LPC, n = D.cnt[c]; j < n; j += LPC)
-                if (D.mv[base + j] == a) f(base + j);
-        }
+    /* item k: MD cell k / LPC of the box, atoms k % LPC, + LPC, ... of it.  A lane
+       walks its items in order and adds its atoms in that order in both sums.  The
+       walk is latency-bound (per item a chain cnt -> mv -> kind, v), so the counts
+       and first turi indices of U items load together, their first atoms' kind and
+       velocity load together, and the first sum keeps the slots it takes (KMAX per
+       lane, in walking order) for the second sum to replay instead of walking again;
+       a lane with more resumes its walk at its (KMAX+1)-th slot.  Slots are kept as
+       32 bits: a slot holds more than 64 bytes of state, so 2^32 of them exceed HBM. */
+    constexpr int U = 4, KMAX = 16;
+    const int j0 = (int)threadIdx.x % LPC;   /* TTPB is a multiple of LPC: each item starts at j0 */
+    auto cell_of = [&](int k) __attribute__((always_inline)) {
+        const int b = k / LPC, cz = z0 + b % nz, t = b / nz;
+        return ((x0 + t / ny) * D.nc1 + (y0 + t % ny)) * D.nc2 + cz;
     };
+    unsigned hit[KMAX];
+    int nh = 0, kres = items, jres = 0;
     double m = 0.0, px = 0.0, py = 0.0, pz = 0.0;
-    visit([&](long s) {
-        const double mm = D.mass[D.kind[s]];
+    auto take = [&](long s, int kd, double vx, double vy, double vz, int k, int j) __attribute__((always_inline)) {
+        const double mm = D.mass[kd];
         m += mm;
-        px += mm * D.vx[s];
-        py += mm * D.vy[s];
-        pz += mm * D.vz[s];
-    });
-    m = block_sum(m, sh);
-    px = block_sum(px, sh);
-    py = block_sum(py, sh);
-    pz = block_sum(pz, sh);
+        px += mm * vx;
+        py += mm * vy;
+        pz += mm * vz;
+#pragma unroll
+        for (int q = 0; q < KMAX; q++)
+            if (q == nh) hit[q] = (unsigned)s;
+        if (nh == KMAX)
+        {
+            kres = k;
+            jres = j;
+        }
+        nh++;
+    };
+    for (int k0 = threadIdx.x; k0 < items; k0 += U * TTPB)
+    {
+        int n_[U], w_[U], kd_[U];
+        long b_[U];
+        double vx_[U], vy_[U], vz_[U];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+        {
+            const int k = k0 + u * TTPB;
+            n_[u] = 0;
+            w_[u] = -1;
+            b_[u] = 0;
+            if (k < items)
+            {
+                const int c = cell_of(k);
+                b_[u] = (long)c * D.cap;
+                n_[u] = D.cnt[c];
+                if (j0 < D.cap) w_[u] = D.mv[b_[u] + j0];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+        {
+            kd_[u] = 0;
+            vx_[u] = vy_[u] = vz_[u] = 0.0;
+            if (j0 < n_[u] && w_[u] == a)
+            {
+                const long s = b_[u] + j0;
+                kd_[u] = D.kind[s];
+                vx_[u] = D.vx[s];
+                vy_[u] = D.vy[s];
+                vz_[u] = D.vz[s];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+        {
+            const int k = k0 + u * TTPB;
+            if (j0 < n_[u] && w_[u] == a) take(b_[u] + j0, kd_[u], vx_[u], vy_[u], vz_[u], k, j0);
+            for (int j = j0 + LPC; j < n_[u]; j += LPC)
+            {
+                const long s = b_[u] + j;
+                if (D.mv[s] == a) take(s, D.kind[s], D.vx[s], D.vy[s], D.vz[s], k, j);
+            }
+        }
+    }
+    {
+        double v4[4] = {m, px, py, pz};
+        block_sums(v4, sh);
+        m = v4[0];
+        px = v4[1];
+        py = v4[2];
+        pz = v4[3];
+    }
     const bool owned = iz >= 1 && iz <= S.ncz;
     const bool divide = G3 ? m != 0.0 : owned;
     const double vx0 = divide ? px / m : px, vy0 = divide ? py / m : py, vz0 = divide ? pz / m : pz;
     double e = 0.0, cnt = 0.0;
-    visit([&](long s) {
-        const double mm = D.mass[D.kind[s]];
-        const double vx = D.vx[s] - vx0, vy = D.vy[s] - vy0, vz = D.vz[s] - vz0;
+    auto add2 = [&](int kd, double vxs, double vys, double vzs) __attribute__((always_inline)) {
+        const double mm = D.mass[kd];
+        const double vx = vxs - vx0, vy = vys - vy0, vz = vzs - vz0;
         e += mm * (vx * vx + vy * vy + vz * vz);
         cnt += 1.0;
-    });
-    e = block_sum(e, sh);
-    cnt = block_sum(cnt, sh);
+    };
+    /* the kept slots, eight loads in flight at a time */
+#pragma unroll
+    for (int q0 = 0; q0 < KMAX; q0 += 8)
+    {
+        int kd[8];
+        double ux[8], uy[8], uz[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+        {
+            kd[q] = 0;
+            ux[q] = uy[q] = uz[q] = 0.0;
+            if (q0 + q < nh)
+            {
+                const long s = hit[q0 + q];
+                kd[q] = D.kind[s];
+                ux[q] = D.vx[s];
+                uy[q] = D.vy[s];
+                uz[q] = D.vz[s];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            if (q0 + q < nh) add2(kd[q], ux[q], uy[q], uz[q]);
+    }
+    /* a lane with more than KMAX slots walks on from its (KMAX+1)-th */
+    for (int k = kres; k < items; k += TTPB)
+    {
+        const int c = cell_of(k);
+        const long base = (long)c * D.cap;
+        for (int j = k == kres ? jres : j0, n = D.cnt[c]; j < n; j += LPC)
+        {
+            const long s = base + j;
+            if (D.mv[s] == a) add2(D.kind[s], D.vx[s], D.vy[s], D.vz[s]);
+        }
+    }
+    {
+        double v2[2] = {e, cnt};
+        block_sums(v2, sh);
+        e = v2[0];
+        cnt = v2[1];
+    }
     if (threadIdx.x == 0)
     {
         const unsigned nn = (unsigned)cnt;
