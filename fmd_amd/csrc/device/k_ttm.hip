@@ -20,7 +20,8 @@
  * ncx x ncy x ncz turies (no counterpart in the reference, whose 3-D
  * updater is missing, ttm.c:451-452): type 1 with Ce = gamma Te, constant G,
  * and constant conductivity or kappa = K Te / Ti, periodic in x and y.  The
- * presolve runs one lane per (x, y) column, each scanning z as the 1-D scan;
+ * presolve decides every cell at once and then runs one lane per (x, y) column
+ * through its activations in z order, as the 1-D scan takes them;
  * one workgroup of 1024 lanes keeps Te (and the cells' kappa) of up to
  * FMDX_TTM_MAX_TCELLS cells in LDS for the sub-steps, each lane owning the
  * same cells throughout so that its new values and xi sums stay in registers.
@@ -408,18 +409,37 @@ __global__ __launch_bounds__(TTPB) void k_ttm_solve(TtmState S, double time, dou
 /* ------------------------------------------------ ncx x ncy x ncz turies */
 
 /* Activation / deactivation on the grid (the extension of
-   presolve_common_types_1_2, ttm.c:194-230): one lane per (x, y) column scans
-   z upwards as the 1-D scan does (the cell below already processed in this
-   pass, the cell above not yet); the x and y neighbours contribute their Te
-   from before the pass, kept in LDS, so the columns are independent.  A newly
+   presolve_common_types_1_2, ttm.c:194-230): each column is scanned along z
+   upwards as the 1-D scan does (the cell below already processed in this pass,
+   the cell above not yet); the x and y neighbours contribute their Te from
+   before the pass, kept in LDS, so the columns are independent.  A newly
    active cell takes the atom-weighted mean Te of its active neighbours in the
-   order z-, z+, x-, x+, y-, y+, or its Ti when it has none. */
+   order z-, z+, x-, x+, y-, y+, or its Ti when it has none.
+   Whether a cell activates (Te < 0 with enough atoms) or is switched off (too
+   few atoms) rests on its own Te before the pass and its atom count, so all
+   lanes decide every cell and write the switch-offs and the xi reset at once;
+   one lane per column then walks only the activations in z order, each reading
+   the cell below as the scan left it and the cell above as it was before. */
 __global__ __launch_bounds__(T3PB_P) void k_ttm_presolve3(TtmState S)
 {
 #pragma clang fp contract(off)
-    extern __shared__ double before[];
-    for (int a = threadIdx.x; a < S.n; a += T3PB_P) before[a] = S.Te[a];
-    __syncthreads();
+    extern __shared__ double before[];   /* S.n values of Te, then S.n activation flags */
+    unsigned char *act = (unsigned char *)(before + S.n);
+    for (int a = threadIdx.x; a < S.n; a += T3PB_P)
+    {
+        const double te = S.Te[a];
+        before[a] = te;
+        const int iz = a % S.nzg;
+        bool on = false;
+        if (iz >= 1 && iz <= S.ncz)
+        {
+            S.xi[a] = 0.0;
+            on = te < 0.0 && S.num[a] >= S.min_atoms;
+            if (S.num[a] < S.min_atoms) S.Te[a] = S.Te2[a] = -1.0;
+        }
+        act[a] = on;
+    }
+    __syncthreads();   /* the switch-offs above are what the activations below read */
     for (int col = threadIdx.x; col < S.ncx * S.ncy; col += T3PB_P)
     {
         const int ix = col / S.ncy, iy = col % S.ncy;
@@ -428,24 +448,20 @@ __global__ __launch_bounds__(T3PB_P) void k_ttm_presolve3(TtmState S)
         for (int iz = 1; iz <= S.ncz; iz++)
         {
             const int a = col * S.nzg + iz;
-            S.xi[a] = 0.0;
-            if (S.Te[a] < 0.0 && S.num[a] >= S.min_atoms)
+            if (!act[a]) continue;
+            const int nb[6] = {a - 1, a + 1, xl * S.nzg + iz, xh * S.nzg + iz, yl * S.nzg + iz, yh * S.nzg + iz};
+            unsigned count = 0;
+            double sum = 0.0;
+            for (int k = 0; k < 6; k++)
             {
-                const int nb[6] = {a - 1, a + 1, xl * S.nzg + iz, xh * S.nzg + iz, yl * S.nzg + iz, yh * S.nzg + iz};
-                unsigned count = 0;
-                double sum = 0.0;
-                for (int k = 0; k < 6; k++)
+                const double te = k == 0 ? S.Te[nb[k]] : before[nb[k]];
+                if (te >= 0.0)
                 {
-                    const double te = k < 2 ? S.Te[nb[k]] : before[nb[k]];
-                    if (te >= 0.0)
-                    {
-                        sum += te * S.num[nb[k]];
-                        count += S.num[nb[k]];
-                    }
+                    sum += te * S.num[nb[k]];
+                    count += S.num[nb[k]];
                 }
-                S.Te[a] = (count > 0) ? sum / (int)count : S.Ti[a];
             }
-            if (S.num[a] < S.min_atoms) S.Te[a] = S.Te2[a] = -1.0;
+            S.Te[a] = (count > 0) ? sum / (int)count : S.Ti[a];
         }
     }
 }
@@ -606,7 +622,7 @@ hipError_t launch_ttm_fields(const KDesc &D, const TtmState &S, hipStream_t st)
 
 hipError_t launch_ttm_presolve(const TtmState &S, unsigned *err, hipStream_t st)
 {
-    if (S.dim3) hipLaunchKernelGGL(k_ttm_presolve3, dim3(1), dim3(T3PB_P), sizeof(double) * S.n, st, S);
+    if (S.dim3) hipLaunchKernelGGL(k_ttm_presolve3, dim3(1), dim3(T3PB_P), (sizeof(double) + 1) * S.n, st, S);
     else hipLaunchKernelGGL(k_ttm_presolve, dim3(1), dim3(TTPB), 0, st, S, err);
     return hipGetLastError();
 }
