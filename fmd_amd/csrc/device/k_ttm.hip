@@ -530,11 +530,23 @@ __global__ __launch_bounds__(T3PB) void k_ttm_solve3(TtmState S, double time, do
             depth[k] = exp(S.laser_absdepth_inv * (z - S.frontsurf));
         }
     }
+    /* the source's time factor of sub-step j depends on j alone: with up to 64
+       sub-steps, lane j of every wave evaluates it once and sub-step j reads it
+       from that lane */
+    const bool spc_lanes = calc_source && S.ratio <= 64;
+    double spc_l = 0.0;
+    if (spc_lanes && (int)(threadIdx.x & 63) < S.ratio)
+    {
+        const int j = threadIdx.x & 63;
+        const double q = time + j * dt - S.laser_t0;
+        spc_l = S.laser_factor * exp(S.laser_2sig2_inv * (q * q));
+    }
     __syncthreads();
     for (int j = 0; j < S.ratio; j++)
     {
         double spc = 0.0;
-        if (calc_source)
+        if (spc_lanes) spc = __shfl(spc_l, j, 64);
+        else if (calc_source)
         {
             const double q = time + j * dt - S.laser_t0;
             spc = S.laser_factor * exp(S.laser_2sig2_inv * (q * q));
