@@ -131,7 +131,7 @@ __global__ __launch_bounds__(TTPB) void k_ttm_sums(KDesc D, TtmState S)
        lane, in walking order) for the second sum to replay instead of walking again;
        a lane with more resumes its walk at its (KMAX+1)-th slot.  Slots are kept as
        32 bits: a slot holds more than 64 bytes of state, so 2^32 of them exceed HBM. */
-    constexpr int U = 4, KMAX = 16;
+    constexpr int U = 4, KMAX = 16;   /* 8: 145 VGPRs, 3 waves per SIMD, slower */
     const int j0 = (int)threadIdx.x % LPC;   /* TTPB is a multiple of LPC: each item starts at j0 */
     auto cell_of = [&](int k) __attribute__((always_inline)) {
         const int b = k / LPC, cz = z0 + b % nz, t = b / nz;
